@@ -1,0 +1,139 @@
+// Internal declarations shared by the libdfmi translation units.
+// Grid conventions (same numbering as deep_frame/topology_optimization.py:301-319):
+//   node (i, j, l) of an (X+1, Y+1, Z+1) node grid has index (i*(Y+1) + j)*(Z+1) + l,
+//   element (i, j, l) of an (X, Y, Z) element grid has index (i*Y + j)*Z + l,
+//   a multi-vector is stored RHS-major: v[r][node][comp], comp fastest.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+namespace dfmi {
+
+void set_error(const std::string& msg);
+const char* get_error();
+
+#define DFMI_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_));  \
+    }                                                                               \
+  } while (0)
+
+// Node/element geometry of one level.
+struct Geom {
+  int ex, ey, ez;   // elements per axis
+  int nx, ny, nz;   // nodes per axis (= elements + 1)
+  __host__ __device__ long long nodes() const { return (long long)nx * ny * nz; }
+  __host__ __device__ long long elems() const { return (long long)ex * ey * ez; }
+};
+
+inline Geom make_geom(int ex, int ey, int ez) { return Geom{ex, ey, ez, ex + 1, ey + 1, ez + 1}; }
+
+// Hex8 stiffness in the corner-Hadamard basis: 8 parity classes of 3x3 blocks,
+// pre-scaled by 1/64 (see hex8_blocks() in the host code).
+struct KeBlocks {
+  double b[8][3][3];
+};
+
+// Apply epilogues shared by the fine (matrix-free) and coarse (stencil) kernels.
+enum ApplyMode : int {
+  MODE_APPLY = 0,   // y = M A M x
+  MODE_RESID = 1,   // y = M (b - A M x)
+  MODE_CHEB = 2,    // y = x + c1 (x - xp) + c2 Dinv (b - A x)   (y may alias xp)
+  MODE_DOT = 3,     // y = M A M x and partial sums of x.y per workgroup
+  MODE_ENERGY = 4,  // fine only: per-element u_e^T Ke u_e (no modulus)
+};
+
+struct ApplyArgs {
+  const void* x;      // operand, [k][nodes][3]
+  void* y;            // output
+  const void* b;      // right-hand side for RESID / CHEB
+  const void* xp;     // previous iterate for CHEB
+  const void* dinv;   // [nodes][3] inverse diagonal (masked), CHEB only
+  double c1, c2;      // Chebyshev coefficients, CHEB only
+  double* partial;    // [k][nblocks] partial dot products, DOT only
+  const int* active;  // per-column flag (device), nullptr = all active
+  int k;
+};
+
+// ---- fine level (matrix-free hex8) -------------------------------------------------
+template <typename T>
+void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlocks& kb, int mode,
+                const ApplyArgs& a, hipStream_t s, int* nblocks_out);
+int fine_blocks(const Geom& g);
+void fine_element_energy(const Geom& pad, int ux, int uy, int uz, const double* u, const KeBlocks& kb,
+                         double* energy, int k, hipStream_t s);
+void fine_diagonal(const Geom& g, const double* moduli, const uint8_t* mask, const double* ke_diag, double* diag,
+                   hipStream_t s);
+void fine_stencil(const Geom& g, const double* moduli, const uint8_t* mask, const double* ke, double* stencil,
+                  hipStream_t s);
+void active_mask(const Geom& g, const double* moduli, const uint8_t* fixed, uint8_t* mask, hipStream_t s);
+// same with the fixed flags given per DOF of the unpadded (ux+1, uy+1, uz+1) node grid
+void active_mask_flat(const Geom& pad, int ux, int uy, int uz, const double* moduli, const uint8_t* fixed,
+                      uint8_t* mask, hipStream_t s);
+
+// ---- coarse levels (27-point block stencils, SoA [27*9][nodes]) -----------------------
+template <typename T>
+void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, hipStream_t s,
+                   int* nblocks_out);
+int stencil_blocks(const Geom& g, int k);
+void stencil_rap(const Geom& fine, const double* fst, const Geom& coarse, double* cst, hipStream_t s);
+void stencil_diagonal(const Geom& g, const double* st, double* diag, hipStream_t s);
+void stencil_mask(const Geom& g, double* st, const uint8_t* mask, hipStream_t s);
+void dead_mask(const Geom& g, const double* diag, double threshold, uint8_t* mask, hipStream_t s);
+template <typename T>
+void restrict_full(const Geom& fine, const T* f, const Geom& coarse, const uint8_t* cmask, T* c, int k,
+                   const int* active, hipStream_t s);
+template <typename T>
+void prolong_add(const Geom& coarse, const T* c, const Geom& fine, const uint8_t* fmask, T* f, int k,
+                 const int* active, hipStream_t s);
+void coarsest_assemble(const Geom& g, const double* st, const int* live_index, int n, double* dense, hipStream_t s);
+void dense_sweep_inverse(double* a, int n, hipStream_t s);
+template <typename T>
+void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out,
+                    double* work_in, double* work_out, int k, const int* active, hipStream_t s);
+
+// ---- vectors ----------------------------------------------------------------------------
+void reduce_partials(const double* partial, int nblocks, int k, double* out, hipStream_t s);
+template <typename T>
+void column_dot(const T* a, const T* b, long long n, int k, double* partial, int nblocks, const int* active,
+                hipStream_t s);
+template <typename T>
+void column_absmax(const T* a, long long n, int k, double* partial, int nblocks, const int* active, hipStream_t s);
+void reduce_max(const double* partial, int nblocks, int k, double* out, hipStream_t s);
+template <typename T>
+void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active,
+                hipStream_t s);
+void power_init(const uint8_t* mask, double* v, long long nodes, unsigned seed, hipStream_t s);
+void dinv_scale(const double* y, const double* dinv, const double* inv_norm, double* v, long long nodes,
+                hipStream_t s);
+void pcg_beta(const double* rzn, const double* rzc, double* rz, double* beta, int k, const int* active,
+              hipStream_t s);
+void copy_columns(const double* src, double* dst, long long n, int k, const int* select, hipStream_t s);
+template <typename T>
+void fill_zero(T* x, long long n, hipStream_t s);
+void pcg_update(double* x, double* r, const double* p, const double* ap, const double* alpha, long long n, int k,
+                const int* active, double* partial_rr, int nblocks, hipStream_t s);
+void pcg_direction(double* p, const double* z, const double* beta, long long n, int k, const int* active,
+                   hipStream_t s);
+void pcg_scalars_alpha(const double* rz, const double* pap, double* alpha, int k, const int* active, hipStream_t s);
+template <typename T>
+void to_work(const double* r, const double* scale, T* w, long long n, int k, const int* active, hipStream_t s);
+template <typename T>
+void from_work(const T* w, const double* scale, double* z, long long n, int k, const int* active, hipStream_t s);
+void flat_to_grid(const Geom& pad, int ux, int uy, int uz, const double* flat, double* grid, int k, hipStream_t s);
+void grid_to_flat(const Geom& pad, int ux, int uy, int uz, const double* grid, double* flat, int k, hipStream_t s);
+void pad_moduli(const Geom& pad, int ux, int uy, int uz, const double* moduli, double* m64, float* m32,
+                hipStream_t s);
+void scale_vector(double* v, double a, long long n, hipStream_t s);
+void mask_vector(const uint8_t* mask, double* v, long long nodes, int k, hipStream_t s);
+int reduce_blocks(long long n);
+void compliance_sensitivity(const double* u_flat, const double* f_flat, long long ndof, int k, const double* energy,
+                            long long ne, const double* dmod, double factor, double* comp, double* sens,
+                            double* partial, hipStream_t s);
+
+}  // namespace dfmi

@@ -1,0 +1,661 @@
+// SIMP step around the solve: interpolation (deep_frame/topology_optimization.py:657-661),
+// compliance/sensitivity reduction (:721-734), cone density filter and pullback
+// (:1082-1092, :1135-1142, :1171-1178), OC update (:1197-1233) and one MMA step
+// (Svanberg's method as called by deep_frame/topology_optimizers.py:55-67).
+// All element-wise work is HBM-bound; reductions use per-block partials and a
+// fixed-order second pass.
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "dfmi_internal.h"
+#include "dfmi.h"
+
+namespace dfmi {
+
+static __device__ double blk_sum(double v) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+static __device__ double blk_max(double v) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// ---- SIMP interpolation ---------------------------------------------------------------------
+__global__ void k_simp(const double* rho, const uint8_t* active, long long n, double young, double p, double eps,
+                       double* moduli, double* dmoduli) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = rho[i];
+  const bool on = !active || active[i];
+  moduli[i] = on ? young * (eps + (1.0 - eps) * pow(r, p)) : 0.0;
+  if (dmoduli) dmoduli[i] = on ? young * (1.0 - eps) * p * pow(r, p - 1.0) : 0.0;
+}
+
+// sensitivity[e] = -dmod[e] * factor * sum_r energy[r][e]
+__global__ void k_sensitivity(const double* energy, long long ne, int k, const double* dmod, double factor,
+                              double* sens) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  double acc = 0.0;
+  for (int r = 0; r < k; ++r) acc += energy[(long long)r * ne + e];
+  sens[e] = -dmod[e] * factor * acc;
+}
+
+__global__ void k_flat_dot(const double* a, const double* b, long long n, double* partial) {
+  const int r = blockIdx.y;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    acc += a[(long long)r * n + i] * b[(long long)r * n + i];
+  acc = blk_sum(acc);
+  if (threadIdx.x == 0) partial[(long long)r * gridDim.x + blockIdx.x] = acc;
+}
+
+__global__ void k_finish_compliance(const double* partial, int nb, double factor, double* out) {
+  const int r = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) acc += partial[(long long)r * nb + i];
+  acc = blk_sum(acc);
+  if (threadIdx.x == 0) out[r] = factor * acc;
+}
+
+void compliance_sensitivity(const double* u_flat, const double* f_flat, long long ndof, int k, const double* energy,
+                            long long ne, const double* dmod, double factor, double* comp, double* sens,
+                            double* partial, hipStream_t s) {
+  const int nb = reduce_blocks(ndof);
+  hipLaunchKernelGGL(k_flat_dot, dim3(nb, k), dim3(256), 0, s, u_flat, f_flat, ndof, partial);
+  hipLaunchKernelGGL(k_finish_compliance, dim3(k), dim3(256), 0, s, partial, nb, factor, comp);
+  hipLaunchKernelGGL(k_sensitivity, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, energy, ne, k, dmod, factor,
+                     sens);
+  DFMI_HIP(hipGetLastError());
+}
+
+// ---- cone filter -----------------------------------------------------------------------------
+struct FilterGeom {
+  int nx, ny, nz;
+  int taps;
+};
+
+// out[e] = a_e sum_o w_o (x a)[e + o] * (scale ? scale[e] : 1), optionally zeroed outside `keep`
+__global__ void k_cone(FilterGeom g, const int* offs, const double* w, const uint8_t* allowed, const double* x,
+                       const double* pre, const double* post, const uint8_t* keep, double* out) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int l = (int)(e % g.nz), j = (int)((e / g.nz) % g.ny), i = (int)(e / ((long long)g.nz * g.ny));
+  double acc = 0.0;
+  if (allowed[e]) {
+    for (int t = 0; t < g.taps; ++t) {
+      const int a = i + offs[3 * t], b = j + offs[3 * t + 1], c = l + offs[3 * t + 2];
+      if (a < 0 || b < 0 || c < 0 || a >= g.nx || b >= g.ny || c >= g.nz) continue;
+      const long long m = ((long long)a * g.ny + b) * g.nz + c;
+      if (!allowed[m]) continue;
+      acc += w[t] * x[m] * (pre ? pre[m] : 1.0);
+    }
+  }
+  if (post) acc *= post[e];
+  if (keep && !keep[e]) acc = 0.0;
+  out[e] = acc;
+}
+
+__global__ void k_inverse(const double* a, double* b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = 1.0 / a[i];
+}
+
+__global__ void k_sums_fix(const uint8_t* allowed, double* sums, long long n, int* bad) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!allowed[i]) sums[i] = 1.0;
+  if (!(sums[i] > 0)) atomicAdd(bad, 1);
+}
+
+__global__ void k_mul(const double* a, const double* b, double* out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] * (b ? b[i] : 1.0);
+}
+
+__global__ void k_sum(const double* a, long long n, double* partial) {
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) acc += a[i];
+  acc = blk_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+struct Filter {
+  FilterGeom g{};
+  hipStream_t stream = nullptr;
+  long long n = 0;
+  void *offs = nullptr, *w = nullptr, *allowed = nullptr, *free_ = nullptr, *sums = nullptr, *inv = nullptr,
+       *tmp = nullptr, *tmp2 = nullptr, *partial = nullptr, *lo = nullptr, *hi = nullptr, *ratio = nullptr,
+       *prop = nullptr;
+  int nb = 1;
+  ~Filter() {
+    for (void* p : {offs, w, allowed, free_, sums, inv, tmp, tmp2, partial, lo, hi, ratio, prop})
+      if (p) (void)hipFree(p);
+  }
+  double sum(const double* a) {
+    hipLaunchKernelGGL(k_sum, dim3(nb), dim3(256), 0, stream, a, n, (double*)partial);
+    std::vector<double> h(nb);
+    DFMI_HIP(hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, stream));
+    DFMI_HIP(hipStreamSynchronize(stream));
+    double acc = 0.0;
+    for (double v : h) acc += v;
+    return acc;
+  }
+  void forward(const double* x, double* out) {
+    hipLaunchKernelGGL(k_cone, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, g, (const int*)offs,
+                       (const double*)w, (const uint8_t*)allowed, x, nullptr, (const double*)inv, nullptr, out);
+    DFMI_HIP(hipGetLastError());
+  }
+};
+
+// ---- OC --------------------------------------------------------------------------------------
+__global__ void k_oc_prep(const double* x, const double* dc, const double* dv, const uint8_t* free_, double dvmax,
+                          double xmin, double move, double* ratio, double* lo, double* hi, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!free_[i]) return;
+  const double d = fmax(dv[i], 1e-12 * dvmax);
+  ratio[i] = fmax(1e-30, -dc[i] / d);
+  lo[i] = fmax(xmin, x[i] - move);
+  hi[i] = fmin(1.0, x[i] + move);
+}
+
+__global__ void k_oc_prop(const double* x, const uint8_t* free_, const double* ratio, const double* lo,
+                          const double* hi, double lmb, int lowest, double* out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!free_[i]) {
+    out[i] = x[i];
+    return;
+  }
+  out[i] = lowest ? lo[i] : fmin(fmax(x[i] * sqrt(ratio[i] / fmax(lmb, 1e-100)), lo[i]), hi[i]);
+}
+
+// min/max/finite check of dv and dc over free cells: partial[0..nb) min dv, [nb..2nb) max dv, [2nb..3nb) bad count
+__global__ void k_oc_check(const double* dc, const double* dv, const uint8_t* free_, long long n, double* partial,
+                           int nb) {
+  double mn = 1e300, mx = -1e300, bad = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    if (!free_[i]) continue;
+    mn = fmin(mn, dv[i]);
+    mx = fmax(mx, dv[i]);
+    if (!isfinite(dv[i]) || !isfinite(dc[i])) bad += 1.0;
+  }
+  const double a = -blk_max(-mn);
+  const double b = blk_max(mx);
+  const double c = blk_sum(bad);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = a;
+    partial[nb + blockIdx.x] = b;
+    partial[2 * nb + blockIdx.x] = c;
+  }
+}
+
+__global__ void k_ratio_max(const double* ratio, const uint8_t* free_, long long n, double* partial) {
+  double mx = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    if (free_[i]) mx = fmax(mx, ratio[i]);
+  mx = blk_max(mx);
+  if (threadIdx.x == 0) partial[blockIdx.x] = mx;
+}
+
+// ---- MMA (m = 1, a = 0) ------------------------------------------------------------------------
+struct MmaScalars {
+  double y, z, lam, mu, zet, s;
+};
+
+__global__ void k_mma_prepare(long long n, int it, const double* x, const double* xold1, const double* xold2,
+                              double* low, double* upp, const double* df0, const double* dg, double move,
+                              double scale, double* alfa, double* beta, double* p0, double* q0, double* pp,
+                              double* qq, double* xs, double* xsi, double* eta, double* partial) {
+  const double asyinit = 0.5, asydecr = 0.7, asyincr = 1.2, raa0 = 1e-5, albefa = 0.1;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double xv = x[i], span = 1.0;  // bounds [0, 1]
+    double lo, up;
+    if (it <= 2) {
+      lo = xv - asyinit * span;
+      up = xv + asyinit * span;
+    } else {
+      const double sg = (xv - xold1[i]) * (xold1[i] - xold2[i]);
+      const double f = sg > 0 ? asyincr : (sg < 0 ? asydecr : 1.0);
+      lo = xv - f * (xold1[i] - low[i]);
+      up = xv + f * (upp[i] - xold1[i]);
+      lo = fmin(fmax(lo, xv - 10.0 * span), xv - 0.01 * span);
+      up = fmax(fmin(up, xv + 10.0 * span), xv + 0.01 * span);
+    }
+    low[i] = lo;
+    upp[i] = up;
+    const double al = fmax(fmax(lo + albefa * (xv - lo), xv - move * span), 0.0);
+    const double be = fmin(fmin(up - albefa * (up - xv), xv + move * span), 1.0);
+    alfa[i] = al;
+    beta[i] = be;
+    const double ux2 = (up - xv) * (up - xv), xl2 = (xv - lo) * (xv - lo);
+    const double inv = 1.0 / fmax(span, 1e-5);
+    const double d0 = scale * df0[i], d1 = scale * dg[i];
+    double a0p = fmax(d0, 0.0), a0q = fmax(-d0, 0.0);
+    const double pq0 = 0.001 * (a0p + a0q) + raa0 * inv;
+    p0[i] = (a0p + pq0) * ux2;
+    q0[i] = (a0q + pq0) * xl2;
+    double a1p = fmax(d1, 0.0), a1q = fmax(-d1, 0.0);
+    const double pq = 0.001 * (a1p + a1q) + raa0 * inv;
+    pp[i] = (a1p + pq) * ux2;
+    qq[i] = (a1q + pq) * xl2;
+    acc += pp[i] / (up - xv) + qq[i] / (xv - lo);
+    const double x0 = 0.5 * (al + be);
+    xs[i] = x0;
+    xsi[i] = fmax(1.0 / (x0 - al), 1.0);
+    eta[i] = fmax(1.0 / (be - x0), 1.0);
+  }
+  acc = blk_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+struct MmaVecs {
+  const double *low, *upp, *alfa, *beta, *p0, *q0, *pp, *qq;
+};
+
+// Newton system pieces: sums of gvec, GG*delx/diagx and GG^2/diagx
+__global__ void k_mma_newton(long long n, MmaVecs v, const double* x, const double* xsi, const double* eta,
+                             double lam, double epsi, double* partial, int nb) {
+  double sg = 0, sgd = 0, sgg = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double ux = v.upp[i] - x[i], xl = x[i] - v.low[i];
+    const double plam = v.p0[i] + lam * v.pp[i], qlam = v.q0[i] + lam * v.qq[i];
+    const double gg = v.pp[i] / (ux * ux) - v.qq[i] / (xl * xl);
+    const double delx = plam / (ux * ux) - qlam / (xl * xl) - epsi / (x[i] - v.alfa[i]) + epsi / (v.beta[i] - x[i]);
+    const double diagx = 2.0 * (plam / (ux * ux * ux) + qlam / (xl * xl * xl)) + xsi[i] / (x[i] - v.alfa[i]) +
+                         eta[i] / (v.beta[i] - x[i]);
+    sg += v.pp[i] / ux + v.qq[i] / xl;
+    sgd += gg * delx / diagx;
+    sgg += gg * gg / diagx;
+  }
+  const double a = blk_sum(sg), b = blk_sum(sgd), c = blk_sum(sgg);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = a;
+    partial[nb + blockIdx.x] = b;
+    partial[2 * nb + blockIdx.x] = c;
+  }
+}
+
+__global__ void k_mma_direction(long long n, MmaVecs v, const double* x, const double* xsi, const double* eta,
+                                double lam, double dlam, double epsi, double* dx, double* dxsi, double* deta,
+                                double* partial) {
+  double mx = -1e300;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double ux = v.upp[i] - x[i], xl = x[i] - v.low[i];
+    const double plam = v.p0[i] + lam * v.pp[i], qlam = v.q0[i] + lam * v.qq[i];
+    const double gg = v.pp[i] / (ux * ux) - v.qq[i] / (xl * xl);
+    const double xa = x[i] - v.alfa[i], bx = v.beta[i] - x[i];
+    const double delx = plam / (ux * ux) - qlam / (xl * xl) - epsi / xa + epsi / bx;
+    const double diagx = 2.0 * (plam / (ux * ux * ux) + qlam / (xl * xl * xl)) + xsi[i] / xa + eta[i] / bx;
+    const double d = -delx / diagx - gg * dlam / diagx;
+    const double ds = -xsi[i] + epsi / xa - xsi[i] * d / xa;
+    const double de = -eta[i] + epsi / bx + eta[i] * d / bx;
+    dx[i] = d;
+    dxsi[i] = ds;
+    deta[i] = de;
+    mx = fmax(mx, fmax(fmax(-1.01 * ds / xsi[i], -1.01 * de / eta[i]), fmax(-1.01 * d / xa, 1.01 * d / bx)));
+  }
+  mx = blk_max(mx);
+  if (threadIdx.x == 0) partial[blockIdx.x] = mx;
+}
+
+// trial point at step t and its residual pieces: sum gvec, sum of squares, max |.| of the n-sized residuals
+__global__ void k_mma_trial(long long n, MmaVecs v, const double* x, const double* xsi, const double* eta,
+                            const double* dx, const double* dxsi, const double* deta, double t, double lam,
+                            double epsi, double* xt, double* xsit, double* etat, double* partial, int nb) {
+  double sg = 0, ss = 0, mx = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double xv = x[i] + t * (dx ? dx[i] : 0.0);
+    const double sv = xsi[i] + t * (dxsi ? dxsi[i] : 0.0);
+    const double ev = eta[i] + t * (deta ? deta[i] : 0.0);
+    if (xt) {
+      xt[i] = xv;
+      xsit[i] = sv;
+      etat[i] = ev;
+    }
+    const double ux = v.upp[i] - xv, xl = xv - v.low[i];
+    const double plam = v.p0[i] + lam * v.pp[i], qlam = v.q0[i] + lam * v.qq[i];
+    const double rex = plam / (ux * ux) - qlam / (xl * xl) - sv + ev;
+    const double rexsi = sv * (xv - v.alfa[i]) - epsi;
+    const double reeta = ev * (v.beta[i] - xv) - epsi;
+    sg += v.pp[i] / ux + v.qq[i] / xl;
+    ss += rex * rex + rexsi * rexsi + reeta * reeta;
+    mx = fmax(mx, fmax(fabs(rex), fmax(fabs(rexsi), fabs(reeta))));
+  }
+  const double a = blk_sum(sg), b = blk_sum(ss), c = blk_max(mx);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = a;
+    partial[nb + blockIdx.x] = b;
+    partial[2 * nb + blockIdx.x] = c;
+  }
+}
+
+__global__ void k_copy(const double* a, double* b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_clip01(double* x, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = fmin(fmax(x[i], 0.0), 1.0);
+}
+
+static void mma_step(long long n, int it, double* x, double* xold1, double* xold2, double* low, double* upp, double f0,
+                     const double* df0, double g, const double* dg, double move, double scale, hipStream_t s) {
+  (void)f0;
+  const int nb = std::max(1, std::min(256, (int)((n + 1023) / 1024)));
+  const unsigned ge = (unsigned)((n + 255) / 256);
+  std::vector<double*> buf(13, nullptr);
+  for (auto& p : buf) DFMI_HIP(hipMallocAsync((void**)&p, sizeof(double) * n, s));
+  double* partial = nullptr;
+  DFMI_HIP(hipMallocAsync((void**)&partial, sizeof(double) * 3 * nb, s));
+  double *alfa = buf[0], *beta = buf[1], *p0 = buf[2], *q0 = buf[3], *pp = buf[4], *qq = buf[5], *xs = buf[6],
+         *xsi = buf[7], *eta = buf[8], *dx = buf[9], *dxsi = buf[10], *deta = buf[11], *tx = buf[12];
+  double *tsi = nullptr, *te = nullptr;
+  DFMI_HIP(hipMallocAsync((void**)&tsi, sizeof(double) * n, s));
+  DFMI_HIP(hipMallocAsync((void**)&te, sizeof(double) * n, s));
+  std::vector<double> h(3 * nb);
+  auto fetch = [&](int parts) {
+    DFMI_HIP(hipMemcpyAsync(h.data(), partial, sizeof(double) * parts * nb, hipMemcpyDeviceToHost, s));
+    DFMI_HIP(hipStreamSynchronize(s));
+  };
+  auto psum = [&](int part) {
+    double a = 0;
+    for (int i = 0; i < nb; ++i) a += h[part * nb + i];
+    return a;
+  };
+  auto pmax = [&](int part) {
+    double a = -1e300;
+    for (int i = 0; i < nb; ++i) a = std::max(a, h[part * nb + i]);
+    return a;
+  };
+  hipLaunchKernelGGL(k_mma_prepare, dim3(nb), dim3(256), 0, s, n, it, x, xold1, xold2, low, upp, df0, dg, move, scale,
+                     alfa, beta, p0, q0, pp, qq, xs, xsi, eta, partial);
+  fetch(1);
+  const double gs = scale * g;
+  const double b = psum(0) - gs;
+  const double a0 = 1.0, c = 1e4, d = 1.0;
+  MmaVecs v{low, upp, alfa, beta, p0, q0, pp, qq};
+  MmaScalars sc{1.0, 1.0, 1.0, std::max(1.0, 0.5 * c), 1.0, 1.0};
+  double epsi = 1.0;
+  auto residual = [&](const MmaScalars& q, double gvec, double ss_n, double mx_n, double& norm, double& mx) {
+    const double rey = c + d * q.y - q.mu - q.lam;
+    const double rez = a0 - q.zet;
+    const double relam = gvec - q.y + q.s - b;
+    const double remu = q.mu * q.y - epsi, rezet = q.zet * q.z - epsi, res = q.lam * q.s - epsi;
+    norm = std::sqrt(ss_n + rey * rey + rez * rez + relam * relam + remu * remu + rezet * rezet + res * res);
+    mx = std::max({mx_n, std::fabs(rey), std::fabs(rez), std::fabs(relam), std::fabs(remu), std::fabs(rezet),
+                   std::fabs(res)});
+  };
+  while (epsi > 1e-7) {
+    hipLaunchKernelGGL(k_mma_trial, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, (const double*)nullptr,
+                       (const double*)nullptr, (const double*)nullptr, 0.0, sc.lam, epsi, (double*)nullptr,
+                       (double*)nullptr, (double*)nullptr, partial, nb);
+    fetch(3);
+    double norm, rmax;
+    residual(sc, psum(0), psum(1), pmax(2), norm, rmax);
+    int inner = 0;
+    while (rmax > 0.9 * epsi && inner < 200) {
+      ++inner;
+      hipLaunchKernelGGL(k_mma_newton, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, sc.lam, epsi, partial, nb);
+      fetch(3);
+      const double gvec = psum(0), sgd = psum(1), sgg = psum(2);
+      const double dely = c + d * sc.y - sc.lam - epsi / sc.y;
+      const double delz = a0 - epsi / sc.z;
+      const double dellam = gvec - sc.y - b + epsi / sc.lam;
+      const double diagy = d + sc.mu / sc.y;
+      const double diaglamyi = sc.s / sc.lam + 1.0 / diagy;
+      const double blam = dellam + dely / diagy - sgd;
+      const double alam = diaglamyi + sgg;
+      const double dlam = blam / alam;
+      const double dz = -delz * sc.z / sc.zet;  // second block row with a = 0
+      hipLaunchKernelGGL(k_mma_direction, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, sc.lam, dlam, epsi, dx, dxsi,
+                         deta, partial);
+      fetch(1);
+      const double dy = -dely / diagy + dlam / diagy;
+      const double dmu = -sc.mu + epsi / sc.y - sc.mu * dy / sc.y;
+      const double dzet = -sc.zet + epsi / sc.z - sc.zet * dz / sc.z;
+      const double ds = -sc.s + epsi / sc.lam - sc.s * dlam / sc.lam;
+      double stm = pmax(0);
+      for (double r : {-1.01 * dy / sc.y, -1.01 * dz / sc.z, -1.01 * dlam / sc.lam, -1.01 * dmu / sc.mu,
+                       -1.01 * dzet / sc.zet, -1.01 * ds / sc.s})
+        stm = std::max(stm, r);
+      double step = 1.0 / std::max(stm, 1.0);
+      const MmaScalars old = sc;
+      double newnorm = 2.0 * norm;
+      int tries = 0;
+      while (newnorm > norm && tries < 50) {
+        ++tries;
+        MmaScalars t{old.y + step * dy, old.z + step * dz, old.lam + step * dlam, old.mu + step * dmu,
+                     old.zet + step * dzet, old.s + step * ds};
+        hipLaunchKernelGGL(k_mma_trial, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, dx, dxsi, deta, step, t.lam,
+                           epsi, tx, tsi, te, partial, nb);
+        fetch(3);
+        residual(t, psum(0), psum(1), pmax(2), newnorm, rmax);
+        sc = t;
+        step *= 0.5;
+      }
+      std::swap(xs, tx);
+      std::swap(xsi, tsi);
+      std::swap(eta, te);
+      norm = newnorm;
+    }
+    epsi *= 0.1;
+  }
+  // xold2 <- xold1, xold1 <- x, x <- clip(subproblem solution)
+  hipLaunchKernelGGL(k_copy, dim3(ge), dim3(256), 0, s, xold1, xold2, n);
+  hipLaunchKernelGGL(k_copy, dim3(ge), dim3(256), 0, s, x, xold1, n);
+  hipLaunchKernelGGL(k_copy, dim3(ge), dim3(256), 0, s, xs, x, n);
+  hipLaunchKernelGGL(k_clip01, dim3(ge), dim3(256), 0, s, x, n);
+  DFMI_HIP(hipGetLastError());
+  for (auto p : {alfa, beta, p0, q0, pp, qq, xs, xsi, eta, dx, dxsi, deta, tx, tsi, te, partial})
+    DFMI_HIP(hipFreeAsync(p, s));
+  DFMI_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace dfmi
+
+using namespace dfmi;
+
+struct dfmi_filter {
+  Filter f;
+};
+
+#define DFMI_GUARD2(...)                 \
+  try {                                  \
+    __VA_ARGS__;                         \
+    return 0;                            \
+  } catch (const std::exception& e) {    \
+    dfmi::set_error(e.what());           \
+    return 1;                            \
+  } catch (...) {                        \
+    dfmi::set_error("unknown failure");  \
+    return 1;                            \
+  }
+
+extern "C" {
+
+int dfmi_simp(const double* rho, const uint8_t* active, int64_t n, double young, double penal, double eps,
+              double* moduli, double* dmoduli, void* stream) {
+  DFMI_GUARD2({
+    if (penal < 1 || !(eps > 0 && eps < 1)) throw std::invalid_argument("Invalid SIMP interpolation settings");
+    hipLaunchKernelGGL(k_simp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, rho, active,
+                       (long long)n, young, penal, eps, moduli, dmoduli);
+    DFMI_HIP(hipGetLastError());
+  })
+}
+
+int dfmi_filter_create(const int32_t shape[3], const double spacing[3], double radius, const uint8_t* allowed,
+                       const uint8_t* preserve, dfmi_filter** out) {
+  DFMI_GUARD2({
+    auto fp = std::make_unique<dfmi_filter>();
+    Filter& F = fp->f;
+    F.g.nx = shape[0];
+    F.g.ny = shape[1];
+    F.g.nz = shape[2];
+    F.n = (long long)shape[0] * shape[1] * shape[2];
+    std::vector<int> offs;
+    std::vector<double> w;
+    int pad[3];
+    for (int a = 0; a < 3; ++a) pad[a] = (int)std::floor(radius / spacing[a]);
+    for (int i = -pad[0]; i <= pad[0]; ++i)
+      for (int j = -pad[1]; j <= pad[1]; ++j)
+        for (int l = -pad[2]; l <= pad[2]; ++l) {
+          const double dd = std::sqrt(std::pow(i * spacing[0], 2) + std::pow(j * spacing[1], 2) +
+                                      std::pow(l * spacing[2], 2));
+          const double wt = std::max(radius - dd, 0.0);
+          if (wt > 0) {
+            offs.insert(offs.end(), {i, j, l});
+            w.push_back(wt);
+          }
+        }
+    F.g.taps = (int)w.size();
+    F.nb = std::max(1, std::min(256, (int)((F.n + 1023) / 1024)));
+    DFMI_HIP(hipMalloc(&F.offs, sizeof(int) * offs.size() + 4));
+    DFMI_HIP(hipMalloc(&F.w, sizeof(double) * w.size() + 8));
+    DFMI_HIP(hipMemcpy(F.offs, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice));
+    DFMI_HIP(hipMemcpy(F.w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    std::vector<uint8_t> fr(F.n);
+    for (long long i = 0; i < F.n; ++i) fr[i] = allowed[i] && !(preserve && preserve[i]);
+    for (void** p : {&F.allowed, &F.free_}) DFMI_HIP(hipMalloc(p, F.n));
+    DFMI_HIP(hipMemcpy(F.allowed, allowed, F.n, hipMemcpyHostToDevice));
+    DFMI_HIP(hipMemcpy(F.free_, fr.data(), F.n, hipMemcpyHostToDevice));
+    for (void** p : {&F.sums, &F.inv, &F.tmp, &F.tmp2, &F.lo, &F.hi, &F.ratio, &F.prop})
+      DFMI_HIP(hipMalloc(p, sizeof(double) * F.n));
+    DFMI_HIP(hipMalloc(&F.partial, sizeof(double) * 3 * F.nb));
+    // sums = (W a) a, 1 on forbidden cells
+    std::vector<double> ones(F.n, 1.0);
+    DFMI_HIP(hipMemcpy(F.tmp, ones.data(), sizeof(double) * F.n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_cone, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, nullptr, F.g, (const int*)F.offs,
+                       (const double*)F.w, (const uint8_t*)F.allowed, (const double*)F.tmp, nullptr, nullptr,
+                       nullptr, (double*)F.sums);
+    int* bad = nullptr;
+    DFMI_HIP(hipMalloc((void**)&bad, sizeof(int)));
+    DFMI_HIP(hipMemset(bad, 0, sizeof(int)));
+    hipLaunchKernelGGL(k_sums_fix, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, nullptr,
+                       (const uint8_t*)F.allowed, (double*)F.sums, F.n, bad);
+    hipLaunchKernelGGL(k_inverse, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, nullptr, (const double*)F.sums,
+                       (double*)F.inv, F.n);
+    int hb = 0;
+    DFMI_HIP(hipMemcpy(&hb, bad, sizeof(int), hipMemcpyDeviceToHost));
+    (void)hipFree(bad);
+    if (hb) throw std::invalid_argument("Density filter contains an empty allowed-cell neighborhood");
+    *out = fp.release();
+  })
+}
+
+int dfmi_filter_destroy(dfmi_filter* f) { DFMI_GUARD2({ delete f; }) }
+
+int dfmi_filter_set_stream(dfmi_filter* f, void* stream) { DFMI_GUARD2({ f->f.stream = (hipStream_t)stream; }) }
+
+int dfmi_filter_forward(dfmi_filter* f, const double* x, double* out) {
+  DFMI_GUARD2({
+    f->f.forward(x, out);
+    DFMI_HIP(hipStreamSynchronize(f->f.stream));
+  })
+}
+
+int dfmi_filter_pullback(dfmi_filter* f, const double* sens, const double* dproj, double* out) {
+  DFMI_GUARD2({
+    Filter& F = f->f;
+    // out = a (W (a s dproj / sums)), zero outside the free cells
+    hipLaunchKernelGGL(k_mul, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, F.stream, sens, dproj,
+                       (double*)F.tmp, F.n);
+    hipLaunchKernelGGL(k_cone, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, F.stream, F.g, (const int*)F.offs,
+                       (const double*)F.w, (const uint8_t*)F.allowed, (const double*)F.tmp, (const double*)F.inv,
+                       nullptr, (const uint8_t*)F.free_, out);
+    DFMI_HIP(hipGetLastError());
+    DFMI_HIP(hipStreamSynchronize(F.stream));
+  })
+}
+
+int dfmi_filter_volume(dfmi_filter* f, const double* x, double* volume) {
+  DFMI_GUARD2({
+    Filter& F = f->f;
+    F.forward(x, (double*)F.tmp2);
+    *volume = F.sum((const double*)F.tmp2);
+  })
+}
+
+int dfmi_oc_update(dfmi_filter* f, const double* x, const double* dc, const double* dv, double target, double xmin,
+                   double move, double* out) {
+  DFMI_GUARD2({
+    Filter& F = f->f;
+    hipStream_t s = F.stream;
+    const unsigned ge = (unsigned)((F.n + 255) / 256);
+    const uint8_t* fr = (const uint8_t*)F.free_;
+    hipLaunchKernelGGL(k_oc_check, dim3(F.nb), dim3(256), 0, s, dc, dv, fr, F.n, (double*)F.partial, F.nb);
+    std::vector<double> h(3 * F.nb);
+    DFMI_HIP(hipMemcpyAsync(h.data(), F.partial, sizeof(double) * 3 * F.nb, hipMemcpyDeviceToHost, s));
+    DFMI_HIP(hipStreamSynchronize(s));
+    double mn = 1e300, mx = -1e300, bad = 0;
+    for (int i = 0; i < F.nb; ++i) {
+      mn = std::min(mn, h[i]);
+      mx = std::max(mx, h[F.nb + i]);
+      bad += h[2 * F.nb + i];
+    }
+    if (mn < 0 || !(mx > 0) || bad > 0) throw std::runtime_error("Invalid OC sensitivities");
+    double *ratio = (double*)F.ratio, *lo = (double*)F.lo, *hi = (double*)F.hi, *prop = (double*)F.prop;
+    hipLaunchKernelGGL(k_oc_prep, dim3(ge), dim3(256), 0, s, x, dc, dv, fr, mx, xmin, move, ratio, lo, hi, F.n);
+    auto volume_of = [&](double lmb, int lowest, double* dst) {
+      hipLaunchKernelGGL(k_oc_prop, dim3(ge), dim3(256), 0, s, x, fr, ratio, lo, hi, lmb, lowest, dst, F.n);
+      F.forward(dst, (double*)F.tmp2);
+      return F.sum((const double*)F.tmp2);
+    };
+    if (volume_of(0.0, 1, out) > target) return 0;
+    hipLaunchKernelGGL(k_ratio_max, dim3(F.nb), dim3(256), 0, s, ratio, fr, F.n, (double*)F.partial);
+    DFMI_HIP(hipMemcpyAsync(h.data(), F.partial, sizeof(double) * F.nb, hipMemcpyDeviceToHost, s));
+    DFMI_HIP(hipStreamSynchronize(s));
+    double rmax = 0;
+    for (int i = 0; i < F.nb; ++i) rmax = std::max(rmax, h[i]);
+    double upper = std::max(rmax, 1e-12);
+    bool ok = false;
+    for (int t = 0; t < 100; ++t) {
+      if (volume_of(upper, 0, prop) <= target + 1e-9) {
+        ok = true;
+        break;
+      }
+      upper *= 2;
+    }
+    if (!ok) throw std::runtime_error("OC could not bracket the volume multiplier");
+    double lower = 0.0;
+    volume_of(upper, 0, out);
+    for (int t = 0; t < 70; ++t) {
+      const double mid = 0.5 * (lower + upper);
+      if (volume_of(mid, 0, prop) > target) {
+        lower = mid;
+      } else {
+        upper = mid;
+        hipLaunchKernelGGL(k_copy, dim3(ge), dim3(256), 0, s, (const double*)prop, out, F.n);
+      }
+      if ((upper - lower) / std::max(upper + lower, 1e-100) < 1e-8) break;
+    }
+    DFMI_HIP(hipStreamSynchronize(s));
+  })
+}
+
+int dfmi_mma_step(int64_t n, int32_t iteration, double* x, double* xold1, double* xold2, double* low, double* upp,
+                  double f0, const double* df0, double g, const double* dg, double move, double scale,
+                  void* stream) {
+  DFMI_GUARD2({
+    if (n < 1) throw std::invalid_argument("MMA needs n >= 1");
+    mma_step((long long)n, iteration, x, xold1, xold2, low, upp, f0, df0, g, dg, move, scale, (hipStream_t)stream);
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,865 @@
+// Host side of libdfmi: handle, Galerkin hierarchy set-up, V-cycle, batched
+// flexible PCG and the C ABI of include/dfmi.h.
+//
+// Semantics follow deep_frame/topology_multigrid.py (GeometricMultigrid):
+//   levels/padding :58-63, masks :138-144, :180-181, hierarchy :145-208,
+//   lambda_max :214-223, Chebyshev :284-305, V-cycle :306-317,
+//   precondition :318-324, PCG :407-454, solve :336-375.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+
+#include "dfmi_internal.h"
+#include "dfmi.h"
+
+namespace dfmi {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+const char* get_error() { return g_error.c_str(); }
+
+// reference corner k -> corner bits q = a + 2b + 4c
+static const int kRefToBits[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    release();
+    DFMI_HIP(hipMalloc(&p, n));
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+  ~DevBuf() { release(); }
+};
+
+struct Level {
+  Geom g{};
+  DevBuf mask, dinv64, dinv32, st64, st32;
+  double lambda_max = 0.0;
+  // V-cycle work vectors (precision of the V-cycle), capacity k_cap columns
+  DevBuf rhs, x, xa, res;
+};
+
+struct Hierarchy {
+  bool built = false;
+  std::vector<std::unique_ptr<Level>> levels;
+  int coarsest_n = 0;
+  DevBuf live_dofs, ainv, work_in, work_out;
+  double setup_s = 0.0;
+};
+
+struct Handle {
+  int ux = 0, uy = 0, uz = 0;
+  Geom pad{};
+  int nlev = 0;
+  double spacing[3]{};
+  double ke_bits[576]{};
+  KeBlocks kb{};
+  dfmi_settings st{};
+  hipStream_t stream = nullptr;
+  DevBuf m64, m32, ke_dev, kediag_dev;
+  bool have_moduli = false;
+  Hierarchy slots[8];
+  // PCG workspace (padded grid, FP64)
+  int k_cap = 0;
+  DevBuf b, x, r, z, zn, p, ap, tmp, partial, scal, active;
+  std::vector<Geom> geoms;
+};
+
+static void hex8_blocks(const double* ke_bits, KeBlocks& kb) {
+  // B' = T Ke T^T with T = H (x) I3, H[m][q] = (-1)^popcount(m & q)
+  double t[24][24];
+  for (int m = 0; m < 8; ++m)
+    for (int q = 0; q < 8; ++q)
+      for (int c = 0; c < 3; ++c)
+        for (int cc = 0; cc < 3; ++cc)
+          t[3 * m + c][3 * q + cc] = (c == cc) ? ((__builtin_popcount(m & q) & 1) ? -1.0 : 1.0) : 0.0;
+  double tmp[24][24], bp[24][24];
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) {
+      double s = 0;
+      for (int k = 0; k < 24; ++k) s += t[i][k] * ke_bits[k * 24 + j];
+      tmp[i][j] = s;
+    }
+  double big = 0;
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) {
+      double s = 0;
+      for (int k = 0; k < 24; ++k) s += tmp[i][k] * t[j][k];
+      bp[i][j] = s;
+      big = std::max(big, std::fabs(s));
+    }
+  // entries outside the parity classes must vanish (mirror symmetry of the brick element)
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) {
+      const int pi = (i / 3) ^ (1 << (i % 3)), pj = (j / 3) ^ (1 << (j % 3));
+      if (pi != pj && std::fabs(bp[i][j]) > 1e-10 * big)
+        throw std::runtime_error("element stiffness lacks the mirror symmetry of an axis-aligned brick");
+    }
+  for (int p = 0; p < 8; ++p)
+    for (int c = 0; c < 3; ++c)
+      for (int cc = 0; cc < 3; ++cc) {
+        const int i = 3 * (p ^ (1 << c)) + c, j = 3 * (p ^ (1 << cc)) + cc;
+        kb.b[p][c][cc] = bp[i][j] / 64.0;
+      }
+}
+
+static int level_count(const int shape[3], const dfmi_settings& s, int padded[3]) {
+  int count = 2;
+  if (s.levels > 0) {
+    count = s.levels;
+  } else {
+    auto ok = [&](int cnt) {
+      double coarse = 1.0;
+      for (int a = 0; a < 3; ++a) {
+        if (std::ceil(shape[a] / std::pow(2.0, cnt)) < 1) return false;
+        coarse *= std::ceil(shape[a] / std::pow(2.0, cnt - 1)) + 1;
+      }
+      return coarse * 3 > s.coarsest_dofs;
+    };
+    while (count < s.max_levels && ok(count)) ++count;
+  }
+  const int factor = 1 << (count - 1);
+  for (int a = 0; a < 3; ++a) padded[a] = (shape[a] + factor - 1) / factor * factor;
+  return count;
+}
+
+template <typename T>
+static T* vec(Level& L, DevBuf Level::*field) {
+  return (L.*field).as<T>();
+}
+
+static long long col(const Geom& g) { return g.nodes() * 3; }
+
+// ---- V-cycle -----------------------------------------------------------------------------
+struct VArgs {
+  Handle* h;
+  Hierarchy* H;
+  int k;
+  const int* active;
+};
+
+template <typename T>
+static const T* level_moduli(Handle* h);
+template <>
+const float* level_moduli<float>(Handle* h) {
+  return h->m32.as<float>();
+}
+template <>
+const double* level_moduli<double>(Handle* h) {
+  return h->m64.as<double>();
+}
+
+template <typename T>
+static const T* level_dinv(Level& L);
+template <>
+const float* level_dinv<float>(Level& L) {
+  return L.dinv32.as<float>();
+}
+template <>
+const double* level_dinv<double>(Level& L) {
+  return L.dinv64.as<double>();
+}
+
+template <typename T>
+static const T* level_stencil(Level& L);
+template <>
+const float* level_stencil<float>(Level& L) {
+  return L.st32.as<float>();
+}
+template <>
+const double* level_stencil<double>(Level& L) {
+  return L.st64.as<double>();
+}
+
+template <typename T>
+static void level_apply(VArgs& v, int idx, int mode, ApplyArgs& a) {
+  Level& L = *v.H->levels[idx];
+  a.k = v.k;
+  a.active = v.active;
+  if (idx == 0)
+    fine_apply<T>(L.g, level_moduli<T>(v.h), L.mask.as<uint8_t>(), v.h->kb, mode, a, v.h->stream, nullptr);
+  else
+    stencil_apply<T>(L.g, level_stencil<T>(L), L.mask.as<uint8_t>(), mode, a, v.h->stream, nullptr);
+}
+
+// Chebyshev smoothing of A x = b (topology_multigrid.py:284-305) in the three-term form
+// x_{j+1} = x_j + rho_j rho_{j-1} (x_j - x_{j-1}) + (2 rho_j / delta) Dinv (b - A x_j).
+// On exit the iterate is in L.x.
+template <typename T>
+static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) {
+  Level& L = *v.H->levels[idx];
+  const double upper = 1.1 * L.lambda_max, lower = upper / v.h->st.chebyshev_ratio;
+  const double theta = 0.5 * (upper + lower), delta = 0.5 * (upper - lower);
+  const double sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  T* cur = L.x.as<T>();
+  T* other = L.xa.as<T>();
+  const T* dinv = level_dinv<T>(L);
+  bool prev_zero;
+  if (from_zero) {
+    cheb_start<T>(b, dinv, 1.0 / theta, cur, L.g.nodes(), v.k, v.active, v.h->stream);
+    prev_zero = true;
+  } else {
+    ApplyArgs a{};
+    a.x = cur;
+    a.y = other;
+    a.b = b;
+    a.xp = nullptr;
+    a.dinv = dinv;
+    a.c1 = 0.0;
+    a.c2 = 1.0 / theta;
+    level_apply<T>(v, idx, MODE_CHEB, a);
+    std::swap(cur, other);
+    prev_zero = false;
+  }
+  for (int j = 1; j < count; ++j) {
+    const double nr = 1.0 / (2.0 * sigma - rho);
+    ApplyArgs a{};
+    a.x = cur;
+    a.y = other;
+    a.b = b;
+    a.xp = prev_zero ? nullptr : other;
+    a.dinv = dinv;
+    a.c1 = nr * rho;
+    a.c2 = 2.0 * nr / delta;
+    level_apply<T>(v, idx, MODE_CHEB, a);
+    std::swap(cur, other);
+    prev_zero = false;
+    rho = nr;
+  }
+  if (cur != L.x.as<T>()) std::swap(L.x.p, L.xa.p), std::swap(L.x.bytes, L.xa.bytes);
+}
+
+// V-cycle on level idx for right-hand side b (device, T); result in levels[idx].x
+template <typename T>
+static void vcycle(VArgs& v, int idx, const T* b) {
+  Hierarchy& H = *v.H;
+  Level& L = *H.levels[idx];
+  hipStream_t s = v.h->stream;
+  if (idx == (int)H.levels.size() - 1) {
+    coarsest_solve<T>(L.g, H.ainv.as<double>(), H.live_dofs.as<int>(), H.coarsest_n, b, L.x.as<T>(),
+                      H.work_in.as<double>(), H.work_out.as<double>(), v.k, v.active, s);
+    return;
+  }
+  Level& C = *H.levels[idx + 1];
+  const int count =
+      std::max(1, (int)std::lround(v.h->st.chebyshev_degree * std::pow(v.h->st.growth, (double)idx)));
+  chebyshev<T>(v, idx, b, true, count);
+  ApplyArgs a{};
+  a.x = L.x.as<T>();
+  a.y = L.res.as<T>();
+  a.b = b;
+  level_apply<T>(v, idx, MODE_RESID, a);
+  restrict_full<T>(L.g, L.res.as<T>(), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(), v.k, v.active, s);
+  vcycle<T>(v, idx + 1, C.rhs.as<T>());
+  prolong_add<T>(C.g, C.x.as<T>(), L.g, L.mask.as<uint8_t>(), L.x.as<T>(), v.k, v.active, s);
+  chebyshev<T>(v, idx, b, false, count);
+}
+
+// ---- hierarchy ------------------------------------------------------------------------------
+static void sync(Handle* h) { DFMI_HIP(hipStreamSynchronize(h->stream)); }
+
+static void ensure_level_vectors(Handle* h, Hierarchy& H, int k) {
+  const size_t el = h->st.precision == 1 ? sizeof(float) : sizeof(double);
+  for (auto& Lp : H.levels) {
+    Level& L = *Lp;
+    const size_t n = (size_t)k * col(L.g) * el;
+    L.rhs.ensure(n);
+    L.x.ensure(n);
+    L.xa.ensure(n);
+    L.res.ensure(n);
+  }
+  if (H.coarsest_n > 0) {
+    H.work_in.ensure(sizeof(double) * (size_t)k * H.coarsest_n);
+    H.work_out.ensure(sizeof(double) * (size_t)k * H.coarsest_n);
+  }
+}
+
+static void ensure_workspace(Handle* h, int k) {
+  const size_t n = sizeof(double) * (size_t)k * col(h->pad);
+  for (DevBuf* b : {&h->b, &h->x, &h->r, &h->z, &h->zn, &h->p, &h->ap, &h->tmp}) b->ensure(n);
+  const int nb = std::max(fine_blocks(h->pad), reduce_blocks(col(h->pad)));
+  h->partial.ensure(sizeof(double) * (size_t)k * nb);
+  h->scal.ensure(sizeof(double) * (size_t)k * 16);
+  h->active.ensure(sizeof(int) * (size_t)k);
+  h->k_cap = std::max(h->k_cap, k);
+}
+
+__global__ void k_to_float(const double* a, float* b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_inverse_diag(const double* d, const uint8_t* mask, double* inv64, float* inv32, long long nodes) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nodes * 3) return;
+  const bool on = (mask[i / 3] >> (i % 3)) & 1;
+  const double v = on && d[i] > 0 ? 1.0 / d[i] : 0.0;
+  inv64[i] = v;
+  inv32[i] = (float)v;
+}
+
+static void require_workspace(Handle* h) {
+  if (!h->partial.p || !h->scal.p) throw std::logic_error("reduction workspace used before allocation");
+}
+
+static double column_norm(Handle* h, const double* v, long long n) {
+  require_workspace(h);
+  const int nb = reduce_blocks(n);
+  column_dot<double>(v, v, n, 1, h->partial.as<double>(), nb, nullptr, h->stream);
+  reduce_partials(h->partial.as<double>(), nb, 1, h->scal.as<double>(), h->stream);
+  double out = 0;
+  DFMI_HIP(hipMemcpyAsync(&out, h->scal.as<double>(), sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  sync(h);
+  return std::sqrt(out);
+}
+
+static double device_max(Handle* h, const double* v, long long n) {
+  require_workspace(h);
+  const int nb = reduce_blocks(n);
+  column_absmax<double>(v, n, 1, h->partial.as<double>(), nb, nullptr, h->stream);
+  reduce_max(h->partial.as<double>(), nb, 1, h->scal.as<double>(), h->stream);
+  double out = 0;
+  DFMI_HIP(hipMemcpyAsync(&out, h->scal.as<double>(), sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  sync(h);
+  return out;
+}
+
+// lambda_max(Dinv A) by power iteration (topology_multigrid.py:214-223), in FP64
+static double spectrum(Handle* h, Hierarchy& H, int idx) {
+  Level& L = *H.levels[idx];
+  const long long n = col(L.g);
+  DevBuf v, y, inv;
+  v.ensure(sizeof(double) * n);
+  y.ensure(sizeof(double) * n);
+  inv.ensure(sizeof(double));
+  power_init(L.mask.as<uint8_t>(), v.as<double>(), L.g.nodes(), (unsigned)h->st.seed + 7919u * idx, h->stream);
+  double est = 1.0;
+  for (int it = 0; it < h->st.power_iterations; ++it) {
+    ApplyArgs a{};
+    a.x = v.as<double>();
+    a.y = y.as<double>();
+    a.k = 1;
+    if (idx == 0)
+      fine_apply<double>(L.g, h->m64.as<double>(), L.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, h->stream, nullptr);
+    else
+      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, h->stream, nullptr);
+    const double one = 1.0;
+    DFMI_HIP(hipMemcpyAsync(inv.as<double>(), &one, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    dinv_scale(y.as<double>(), L.dinv64.as<double>(), inv.as<double>(), v.as<double>(), L.g.nodes(), h->stream);
+    est = column_norm(h, v.as<double>(), n);
+    if (!(est > 0)) break;
+    scale_vector(v.as<double>(), 1.0 / est, n, h->stream);
+  }
+  sync(h);
+  return est;
+}
+
+static void build(Handle* h, int slot, const uint8_t* fixed_dev) {
+  if (!h->have_moduli) throw std::runtime_error("update(moduli) must precede hierarchy construction");
+  auto t0 = std::chrono::steady_clock::now();
+  Hierarchy& H = h->slots[slot];
+  H = Hierarchy();
+  hipStream_t s = h->stream;
+  ensure_workspace(h, 1);  // reductions below use the partial-sum and scalar buffers
+  for (int l = 0; l < h->nlev; ++l) {
+    auto L = std::make_unique<Level>();
+    L->g = h->geoms[l];
+    const long long nn = L->g.nodes();
+    L->mask.ensure(nn);
+    L->dinv64.ensure(sizeof(double) * nn * 3);
+    L->dinv32.ensure(sizeof(float) * nn * 3);
+    H.levels.push_back(std::move(L));
+  }
+  Level& F = *H.levels[0];
+  // fine mask and diagonal
+  active_mask_flat(F.g, h->ux, h->uy, h->uz, h->m64.as<double>(), fixed_dev, F.mask.as<uint8_t>(), s);
+  DevBuf diag;
+  diag.ensure(sizeof(double) * F.g.nodes() * 3);
+  fine_diagonal(F.g, h->m64.as<double>(), F.mask.as<uint8_t>(), h->kediag_dev.as<double>(), diag.as<double>(), s);
+  const long long n0 = F.g.nodes() * 3;
+  hipLaunchKernelGGL(k_inverse_diag, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, diag.as<double>(),
+                     F.mask.as<uint8_t>(), F.dinv64.as<double>(), F.dinv32.as<float>(), F.g.nodes());
+  // Galerkin levels
+  DevBuf fst;
+  fst.ensure(sizeof(double) * 243 * (size_t)F.g.nodes());
+  fine_stencil(F.g, h->m64.as<double>(), F.mask.as<uint8_t>(), h->ke_dev.as<double>(), fst.as<double>(), s);
+  const double* prev = fst.as<double>();
+  for (int l = 1; l < h->nlev; ++l) {
+    Level& P = *H.levels[l - 1];
+    Level& L = *H.levels[l];
+    const long long nn = L.g.nodes();
+    L.st64.ensure(sizeof(double) * 243 * (size_t)nn);
+    stencil_rap(P.g, prev, L.g, L.st64.as<double>(), s);
+    DevBuf d;
+    d.ensure(sizeof(double) * nn * 3);
+    stencil_diagonal(L.g, L.st64.as<double>(), d.as<double>(), s);
+    const double dmax = device_max(h, d.as<double>(), nn * 3);
+    dead_mask(L.g, d.as<double>(), h->st.dead_ratio * dmax, L.mask.as<uint8_t>(), s);
+    stencil_mask(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), s);
+    hipLaunchKernelGGL(k_inverse_diag, dim3((unsigned)((nn * 3 + 255) / 256)), dim3(256), 0, s, d.as<double>(),
+                       L.mask.as<uint8_t>(), L.dinv64.as<double>(), L.dinv32.as<float>(), nn);
+    L.st32.ensure(sizeof(float) * 243 * (size_t)nn);
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)((243 * nn + 255) / 256)), dim3(256), 0, s, L.st64.as<double>(),
+                       L.st32.as<float>(), 243 * nn);
+    prev = L.st64.as<double>();
+    sync(h);
+  }
+  fst.release();
+  ensure_workspace(h, 1);
+  for (int l = 0; l + 1 < h->nlev; ++l) H.levels[l]->lambda_max = spectrum(h, H, l);
+  // coarsest dense inverse over live DOFs
+  Level& C = *H.levels.back();
+  const long long nc = C.g.nodes();
+  std::vector<uint8_t> mk(nc);
+  DFMI_HIP(hipMemcpyAsync(mk.data(), C.mask.p, nc, hipMemcpyDeviceToHost, s));
+  sync(h);
+  std::vector<int> index(nc * 3, -1), live;
+  for (long long i = 0; i < nc * 3; ++i)
+    if ((mk[i / 3] >> (i % 3)) & 1) {
+      index[i] = (int)live.size();
+      live.push_back((int)i);
+    }
+  H.coarsest_n = (int)live.size();
+  if (H.coarsest_n > 40000) throw std::runtime_error("coarsest level too large for the dense solve; add levels");
+  if (H.coarsest_n > 0) {
+    DevBuf idx;
+    idx.ensure(sizeof(int) * nc * 3);
+    H.live_dofs.ensure(sizeof(int) * live.size());
+    DFMI_HIP(hipMemcpyAsync(idx.p, index.data(), sizeof(int) * nc * 3, hipMemcpyHostToDevice, s));
+    DFMI_HIP(hipMemcpyAsync(H.live_dofs.p, live.data(), sizeof(int) * live.size(), hipMemcpyHostToDevice, s));
+    const int n = H.coarsest_n;
+    H.ainv.ensure(sizeof(double) * (size_t)n * n);
+    const double* cst = h->nlev > 1 ? C.st64.as<double>() : nullptr;
+    if (!cst) throw std::runtime_error("multigrid needs at least two levels");
+    coarsest_assemble(C.g, cst, idx.as<int>(), n, H.ainv.as<double>(), s);
+    dense_sweep_inverse(H.ainv.as<double>(), n, s);
+    sync(h);
+  }
+  H.built = true;
+  H.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ---- PCG ----------------------------------------------------------------------------------
+template <typename T>
+static void precondition(Handle* h, Hierarchy& H, const double* r, double* z, int k, const int* active) {
+  hipStream_t s = h->stream;
+  Level& F = *H.levels[0];
+  const long long n = col(F.g);
+  ensure_level_vectors(h, H, k);
+  VArgs v{h, &H, k, active};
+  double* scale = h->scal.as<double>() + 8 * (size_t)h->k_cap;
+  if (sizeof(T) == sizeof(double)) {
+    vcycle<T>(v, 0, reinterpret_cast<const T*>(r));
+    copy_columns(reinterpret_cast<const double*>(F.x.as<T>()), z, n, k, active, s);
+    return;
+  }
+  const int nb = reduce_blocks(n);
+  column_absmax<double>(r, n, k, h->partial.as<double>(), nb, active, s);
+  reduce_max(h->partial.as<double>(), nb, k, scale, s);
+  to_work<T>(r, scale, F.rhs.as<T>(), n, k, active, s);
+  vcycle<T>(v, 0, F.rhs.as<T>());
+  from_work<T>(F.x.as<T>(), scale, z, n, k, active, s);
+}
+
+static void precond_any(Handle* h, Hierarchy& H, const double* r, double* z, int k, const int* active) {
+  if (h->st.precision == 1)
+    precondition<float>(h, H, r, z, k, active);
+  else
+    precondition<double>(h, H, r, z, k, active);
+}
+
+static void fine_fp64(Handle* h, Hierarchy& H, int mode, const double* x, double* y, const double* b, int k,
+                      const int* active, double* partial, int* nblocks) {
+  ApplyArgs a{};
+  a.x = x;
+  a.y = y;
+  a.b = b;
+  a.k = k;
+  a.active = active;
+  a.partial = partial;
+  fine_apply<double>(h->pad, h->m64.as<double>(), H.levels[0]->mask.as<uint8_t>(), h->kb, mode, a, h->stream,
+                     nblocks);
+}
+
+static void column_norms(Handle* h, const double* v, long long n, int k, const int* active, std::vector<double>& out) {
+  const int nb = reduce_blocks(n);
+  column_dot<double>(v, v, n, k, h->partial.as<double>(), nb, active, h->stream);
+  reduce_partials(h->partial.as<double>(), nb, k, h->scal.as<double>(), h->stream);
+  out.resize(k);
+  DFMI_HIP(hipMemcpyAsync(out.data(), h->scal.as<double>(), sizeof(double) * k, hipMemcpyDeviceToHost, h->stream));
+  sync(h);
+  for (auto& x : out) x = std::sqrt(x);
+}
+
+static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, int k, dfmi_report* rep) {
+  Hierarchy& H = h->slots[slot];
+  if (!H.built) throw std::runtime_error("hierarchy slot not built");
+  ensure_workspace(h, k);
+  ensure_level_vectors(h, H, k);
+  hipStream_t s = h->stream;
+  const long long n = col(h->pad);
+  double* b = h->b.as<double>();
+  double* x = h->x.as<double>();
+  double* r = h->r.as<double>();
+  double* z = h->z.as<double>();
+  double* zn = h->zn.as<double>();
+  double* p = h->p.as<double>();
+  double* ap = h->ap.as<double>();
+  double* tmp = h->tmp.as<double>();
+  double* sc = h->scal.as<double>();
+  double* alpha = sc;
+  double* rz = sc + (size_t)h->k_cap;
+  double* pap = sc + 2 * (size_t)h->k_cap;
+  double* rzn = sc + 3 * (size_t)h->k_cap;
+  double* rzc = sc + 4 * (size_t)h->k_cap;
+  double* beta = sc + 5 * (size_t)h->k_cap;
+  double* rr = sc + 6 * (size_t)h->k_cap;
+  int* act = h->active.as<int>();
+  Level& F = *H.levels[0];
+
+  flat_to_grid(h->pad, h->ux, h->uy, h->uz, rhs_flat, b, k, s);
+  mask_vector(F.mask.as<uint8_t>(), b, h->pad.nodes(), k, s);
+  std::vector<double> nb;
+  column_norms(h, b, n, k, nullptr, nb);
+  std::vector<int> active(k);
+  for (int c = 0; c < k; ++c) {
+    active[c] = nb[c] > 0 ? 1 : 0;
+    rep[c].iterations = 0;
+    rep[c].relative_residual = 0.0;
+    rep[c].converged = 1;
+  }
+  DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+  fill_zero<double>(x, (long long)k * n, s);
+  copy_columns(b, r, n, k, nullptr, s);
+  precond_any(h, H, r, z, k, act);
+  copy_columns(z, p, n, k, nullptr, s);
+  const int nbr = reduce_blocks(n);
+  column_dot<double>(r, z, n, k, h->partial.as<double>(), nbr, act, s);
+  reduce_partials(h->partial.as<double>(), nbr, k, rz, s);
+  const double tol = h->st.tolerance;
+  int step = 0;
+  std::vector<double> rrh(k), rel(k);
+  auto any = [&]() {
+    for (int c = 0; c < k; ++c)
+      if (active[c]) return true;
+    return false;
+  };
+  while (any() && step < h->st.max_iterations) {
+    int nbf = 0;
+    fine_fp64(h, H, MODE_DOT, p, ap, nullptr, k, act, h->partial.as<double>(), &nbf);
+    reduce_partials(h->partial.as<double>(), nbf, k, pap, s);
+    pcg_scalars_alpha(rz, pap, alpha, k, act, s);
+    pcg_update(x, r, p, ap, alpha, n, k, act, h->partial.as<double>(), nbr, s);
+    reduce_partials(h->partial.as<double>(), nbr, k, rr, s);
+    DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    sync(h);
+    ++step;
+    std::vector<int> check(k, 0);
+    bool need = false;
+    for (int c = 0; c < k; ++c) {
+      if (!active[c]) continue;
+      rel[c] = std::sqrt(rrh[c]) / nb[c];
+      if (rel[c] < tol) check[c] = 1, need = true;
+    }
+    if (need) {
+      // true residual confirmation (topology_multigrid.py:431-437)
+      DFMI_HIP(hipMemcpyAsync(act, check.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+      fine_fp64(h, H, MODE_RESID, x, tmp, b, k, act, nullptr, nullptr);
+      std::vector<double> tn;
+      column_norms(h, tmp, n, k, act, tn);
+      std::vector<int> replace(k, 0);
+      for (int c = 0; c < k; ++c) {
+        if (!check[c]) continue;
+        const double t = tn[c] / nb[c];
+        if (t < tol) {
+          active[c] = 0;
+          rep[c].iterations = step;
+          rep[c].relative_residual = t;
+          rep[c].converged = 1;
+        } else {
+          replace[c] = 1;
+        }
+      }
+      DFMI_HIP(hipMemcpyAsync(act, replace.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+      copy_columns(tmp, r, n, k, act, s);
+      DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+    }
+    if (!any()) break;
+    precond_any(h, H, r, zn, k, act);
+    column_dot<double>(r, zn, n, k, h->partial.as<double>(), nbr, act, s);
+    reduce_partials(h->partial.as<double>(), nbr, k, rzn, s);
+    column_dot<double>(r, z, n, k, h->partial.as<double>(), nbr, act, s);
+    reduce_partials(h->partial.as<double>(), nbr, k, rzc, s);
+    pcg_beta(rzn, rzc, rz, beta, k, act, s);
+    pcg_direction(p, zn, beta, n, k, act, s);
+    std::swap(z, zn);
+  }
+  if (any()) {
+    fine_fp64(h, H, MODE_RESID, x, tmp, b, k, act, nullptr, nullptr);
+    std::vector<double> tn;
+    column_norms(h, tmp, n, k, act, tn);
+    for (int c = 0; c < k; ++c)
+      if (active[c]) {
+        rep[c].iterations = step;
+        rep[c].relative_residual = tn[c] / nb[c];
+        rep[c].converged = rep[c].relative_residual < tol;
+      }
+  }
+  // keep buffer roles stable for the next call
+  if (z != h->z.as<double>()) std::swap(h->z.p, h->zn.p);
+  grid_to_flat(h->pad, h->ux, h->uy, h->uz, x, x_flat, k, s);
+  sync(h);
+}
+
+}  // namespace dfmi
+
+// ---- C ABI ---------------------------------------------------------------------------------
+using namespace dfmi;
+
+struct dfmi_mg {
+  Handle h;
+};
+
+#define DFMI_GUARD(...)                   \
+  try {                                   \
+    __VA_ARGS__;                          \
+    return 0;                             \
+  } catch (const std::exception& e) {     \
+    dfmi::set_error(e.what());            \
+    return 1;                             \
+  } catch (...) {                         \
+    dfmi::set_error("unknown failure");   \
+    return 1;                             \
+  }
+
+extern "C" {
+
+const char* dfmi_last_error(void) { return dfmi::get_error(); }
+
+void dfmi_default_settings(dfmi_settings* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->coarsest_dofs = 80000;
+  o->max_levels = 8;
+  o->levels = 0;
+  o->chebyshev_degree = 3;
+  o->chebyshev_ratio = 20.0;
+  o->growth = 2.0;
+  o->power_iterations = 20;
+  o->precision = 1;
+  o->tolerance = 1e-8;
+  o->max_iterations = 2000;
+  o->dead_ratio = 1e-12;
+  o->seed = 0;
+  o->refine = 0;
+  o->inner_tolerance = 1e-4;
+}
+
+int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
+                dfmi_mg** out) {
+  DFMI_GUARD({
+    if (!shape || !spacing || !ke || !out) throw std::invalid_argument("null argument");
+    for (int a = 0; a < 3; ++a) {
+      if (shape[a] < 1) throw std::invalid_argument("grid shape must contain three positive integers");
+      if (!(spacing[a] > 0) || !std::isfinite(spacing[a])) throw std::invalid_argument("spacing must be positive");
+    }
+    auto mg = std::make_unique<dfmi_mg>();
+    Handle& h = mg->h;
+    dfmi_settings st;
+    dfmi_default_settings(&st);
+    if (settings) st = *settings;
+    if (st.precision != 0 && st.precision != 1) throw std::invalid_argument("precision must be 0 or 1");
+    if (st.chebyshev_degree < 1 || st.max_iterations < 1 || !(st.tolerance > 0))
+      throw std::invalid_argument("invalid multigrid settings");
+    h.st = st;
+    h.ux = shape[0];
+    h.uy = shape[1];
+    h.uz = shape[2];
+    int sh[3] = {shape[0], shape[1], shape[2]}, padded[3];
+    h.nlev = level_count(sh, st, padded);
+    if (h.nlev < 2) throw std::invalid_argument("multigrid needs at least two levels");
+    h.pad = make_geom(padded[0], padded[1], padded[2]);
+    for (int l = 0; l < h.nlev; ++l)
+      h.geoms.push_back(make_geom(padded[0] >> l, padded[1] >> l, padded[2] >> l));
+    for (int a = 0; a < 3; ++a) h.spacing[a] = spacing[a];
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        const int qi = kRefToBits[i / 3] * 3 + i % 3, qj = kRefToBits[j / 3] * 3 + j % 3;
+        h.ke_bits[qi * 24 + qj] = ke[i * 24 + j];
+      }
+    hex8_blocks(h.ke_bits, h.kb);
+    double kd[24];
+    for (int i = 0; i < 24; ++i) kd[i] = h.ke_bits[i * 24 + i];
+    h.ke_dev.ensure(sizeof(double) * 576);
+    h.kediag_dev.ensure(sizeof(double) * 24);
+    DFMI_HIP(hipMemcpy(h.ke_dev.p, h.ke_bits, sizeof(double) * 576, hipMemcpyHostToDevice));
+    DFMI_HIP(hipMemcpy(h.kediag_dev.p, kd, sizeof(double) * 24, hipMemcpyHostToDevice));
+    h.m64.ensure(sizeof(double) * h.pad.elems());
+    h.m32.ensure(sizeof(float) * h.pad.elems());
+    *out = mg.release();
+  })
+}
+
+int dfmi_destroy(dfmi_mg* mg) {
+  DFMI_GUARD({
+    if (mg) {
+      (void)hipStreamSynchronize(mg->h.stream);
+      delete mg;
+    }
+  })
+}
+
+int dfmi_set_stream(dfmi_mg* mg, void* stream) { DFMI_GUARD({ mg->h.stream = (hipStream_t)stream; }) }
+
+int dfmi_levels(dfmi_mg* mg, int32_t* levels, int32_t padded[3]) {
+  DFMI_GUARD({
+    *levels = mg->h.nlev;
+    padded[0] = mg->h.pad.ex;
+    padded[1] = mg->h.pad.ey;
+    padded[2] = mg->h.pad.ez;
+  })
+}
+
+int dfmi_update(dfmi_mg* mg, const double* moduli) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    pad_moduli(h.pad, h.ux, h.uy, h.uz, moduli, h.m64.as<double>(), h.m32.as<float>(), h.stream);
+    for (auto& H : h.slots) H = Hierarchy();
+    h.have_moduli = true;
+    sync(&h);
+  })
+}
+
+int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed, double* setup_seconds) {
+  DFMI_GUARD({
+    if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
+    build(&mg->h, slot, fixed);
+    if (setup_seconds) *setup_seconds = mg->h.slots[slot].setup_s;
+  })
+}
+
+int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t k, dfmi_report* report) {
+  DFMI_GUARD({
+    if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
+    if (k < 1 || !rhs || !x || !report) throw std::invalid_argument("solve needs k >= 1 and buffers");
+    if (mg->h.st.refine) throw std::invalid_argument("refine mode is not available in this build");
+    solve(&mg->h, slot, rhs, x, k, report);
+  })
+}
+
+int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x, double* y, int32_t k) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (!h.have_moduli) throw std::runtime_error("update(moduli) must precede apply");
+    if (k < 1) throw std::invalid_argument("k must be >= 1");
+    ensure_workspace(&h, k);
+    const long long n = col(h.pad);
+    hipStream_t s = h.stream;
+    DevBuf unmasked;
+    const uint8_t* mask;
+    if (slot >= 0) {
+      if (slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+      mask = h.slots[slot].levels[0]->mask.as<uint8_t>();
+    } else {
+      unmasked.ensure(h.pad.nodes());
+      active_mask(h.pad, h.m64.as<double>(), nullptr, unmasked.as<uint8_t>(), s);
+      mask = unmasked.as<uint8_t>();
+    }
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, x, h.b.as<double>(), k, s);
+    ApplyArgs a{};
+    a.k = k;
+    if (precision == 0) {
+      a.x = h.b.as<double>();
+      a.y = h.tmp.as<double>();
+      fine_apply<double>(h.pad, h.m64.as<double>(), mask, h.kb, MODE_APPLY, a, s, nullptr);
+    } else {
+      DevBuf xf, yf;
+      xf.ensure(sizeof(float) * k * n);
+      yf.ensure(sizeof(float) * k * n);
+      hipLaunchKernelGGL(k_to_float, dim3((unsigned)((k * n + 255) / 256)), dim3(256), 0, s, h.b.as<double>(),
+                         xf.as<float>(), k * n);
+      a.x = xf.as<float>();
+      a.y = yf.as<float>();
+      fine_apply<float>(h.pad, h.m32.as<float>(), mask, h.kb, MODE_APPLY, a, s, nullptr);
+      const double one = 1.0;
+      DFMI_HIP(hipMemcpyAsync(h.scal.as<double>(), &one, sizeof(double), hipMemcpyHostToDevice, s));
+      std::vector<double> ones(k, 1.0);
+      DFMI_HIP(hipMemcpyAsync(h.scal.as<double>(), ones.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      from_work<float>(yf.as<float>(), h.scal.as<double>(), h.tmp.as<double>(), n, k, nullptr, s);
+      sync(&h);
+    }
+    grid_to_flat(h.pad, h.ux, h.uy, h.uz, h.tmp.as<double>(), y, k, s);
+    sync(&h);
+  })
+}
+
+int dfmi_element_energy(dfmi_mg* mg, const double* u, int32_t k, double* energy) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    ensure_workspace(&h, k);
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, u, h.b.as<double>(), k, h.stream);
+    fine_element_energy(h.pad, h.ux, h.uy, h.uz, h.b.as<double>(), h.kb, energy, k, h.stream);
+    sync(&h);
+  })
+}
+
+int dfmi_compliance_sensitivity(dfmi_mg* mg, const double* u, const double* f, int32_t k, const double* dmoduli,
+                                double factor, double* compliance, double* sensitivity) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    ensure_workspace(&h, k);
+    const long long ne = (long long)h.ux * h.uy * h.uz;
+    const long long ndof = 3LL * (h.ux + 1) * (h.uy + 1) * (h.uz + 1);
+    DevBuf energy;
+    energy.ensure(sizeof(double) * (size_t)k * ne);
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, u, h.b.as<double>(), k, h.stream);
+    fine_element_energy(h.pad, h.ux, h.uy, h.uz, h.b.as<double>(), h.kb, energy.as<double>(), k, h.stream);
+    compliance_sensitivity(u, f, ndof, k, energy.as<double>(), ne, dmoduli, factor, compliance, sensitivity,
+                           h.partial.as<double>(), h.stream);
+    sync(&h);
+  })
+}
+
+int dfmi_level_apply(dfmi_mg* mg, int32_t slot, int32_t level, const double* x, double* y, int32_t k) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    Hierarchy& H = h.slots[slot];
+    if (level < 0 || level >= (int)H.levels.size()) throw std::invalid_argument("level out of range");
+    Level& L = *H.levels[level];
+    ApplyArgs a{};
+    a.x = x;
+    a.y = y;
+    a.k = k;
+    if (level == 0)
+      fine_apply<double>(L.g, h.m64.as<double>(), L.mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream, nullptr);
+    else
+      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, h.stream, nullptr);
+    sync(&h);
+  })
+}
+
+int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs) {
+  DFMI_GUARD({
+    Hierarchy& H = mg->h.slots[slot];
+    if (!H.built) throw std::invalid_argument("slot not built");
+    for (size_t l = 0; l + 1 < H.levels.size(); ++l) lambda_max[l] = H.levels[l]->lambda_max;
+    *coarsest_dofs = H.coarsest_n;
+  })
+}
+
+}  // extern "C"

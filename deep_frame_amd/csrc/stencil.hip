@@ -1,0 +1,417 @@
+// Coarse levels: Galerkin operators stored as assembled 27-point 3x3-block stencils
+// (structure of arrays st[(d*9 + 3c + c') * nodes + n], d = 9(dx+1) + 3(dy+1) + (dz+1)),
+// the stencil form of the per-coarse-element Galerkin blocks of
+// deep_frame/topology_multigrid.py:174-181 (A_c = P^T M A M P), plus the transfer
+// operators (:314-316), the coarsest dense solve (:230-283 uses a sparse direct
+// factorisation; here a dense inverse applied as one product per V-cycle).
+#include "dfmi_internal.h"
+
+namespace dfmi {
+
+__host__ __device__ inline bool live_bit(uint8_t m, int c) { return (m >> c) & 1; }
+__device__ inline double weight1(int t) { return t == 0 ? 1.0 : ((t == 1 || t == -1) ? 0.5 : 0.0); }
+
+template <typename T, int RC, int MODE>
+__global__ __launch_bounds__(256) void k_stencil_apply(Geom g, const T* __restrict__ st,
+                                                        const uint8_t* __restrict__ mask, ApplyArgs a) {
+  const long long N = g.nodes();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r0 = blockIdx.y * RC;
+  if (n >= N) return;
+  const int l = (int)(n % g.nz), j = (int)((n / g.nz) % g.ny), i = (int)(n / ((long long)g.nz * g.ny));
+  const T* X = reinterpret_cast<const T*>(a.x);
+  bool on[RC];
+#pragma unroll
+  for (int rr = 0; rr < RC; ++rr) on[rr] = (r0 + rr < a.k) && (!a.active || a.active[r0 + rr]);
+  T acc[RC][3];
+#pragma unroll
+  for (int rr = 0; rr < RC; ++rr) acc[rr][0] = acc[rr][1] = acc[rr][2] = T(0);
+  for (int d = 0; d < 27; ++d) {
+    const int mi = i + d / 9 - 1, mj = j + (d / 3) % 3 - 1, ml = l + d % 3 - 1;
+    if (mi < 0 || mj < 0 || ml < 0 || mi >= g.nx || mj >= g.ny || ml >= g.nz) continue;
+    const long long m = ((long long)mi * g.ny + mj) * g.nz + ml;
+    const uint8_t mm = mask[m];
+    T s[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) s[e] = st[((long long)d * 9 + e) * N + n];
+#pragma unroll
+    for (int rr = 0; rr < RC; ++rr) {
+      if (!on[rr]) continue;
+      const T* xm = X + ((long long)(r0 + rr) * N + m) * 3;
+      const T x0 = live_bit(mm, 0) ? xm[0] : T(0);
+      const T x1 = live_bit(mm, 1) ? xm[1] : T(0);
+      const T x2 = live_bit(mm, 2) ? xm[2] : T(0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[rr][c] += s[3 * c] * x0 + s[3 * c + 1] * x1 + s[3 * c + 2] * x2;
+    }
+  }
+  const uint8_t mn = mask[n];
+#pragma unroll
+  for (int rr = 0; rr < RC; ++rr) {
+    if (!on[rr]) continue;
+    const int r = r0 + rr;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long o = ((long long)r * N + n) * 3 + c;
+      const bool lv = live_bit(mn, c);
+      T* Y = reinterpret_cast<T*>(a.y);
+      if (MODE == MODE_APPLY) {
+        Y[o] = lv ? acc[rr][c] : T(0);
+      } else if (MODE == MODE_RESID) {
+        Y[o] = lv ? reinterpret_cast<const T*>(a.b)[o] - acc[rr][c] : T(0);
+      } else if (MODE == MODE_CHEB) {
+        const T xv = lv ? X[o] : T(0);
+        const T xpv = a.xp ? reinterpret_cast<const T*>(a.xp)[o] : T(0);
+        const T bv = reinterpret_cast<const T*>(a.b)[o];
+        const T dv = reinterpret_cast<const T*>(a.dinv)[n * 3 + c];
+        Y[o] = lv ? xv + (T)a.c1 * (xv - xpv) + (T)a.c2 * dv * (bv - acc[rr][c]) : T(0);
+      }
+    }
+  }
+}
+
+int stencil_blocks(const Geom& g, int k) { return (int)((g.nodes() + 255) / 256); }
+
+template <typename T>
+void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, hipStream_t s,
+                   int* nblocks_out) {
+  const unsigned bx = (unsigned)((g.nodes() + 255) / 256);
+  if (nblocks_out) *nblocks_out = bx;
+#define DFMI_SLAUNCH(RC, M) \
+  hipLaunchKernelGGL((k_stencil_apply<T, RC, M>), dim3(bx, (a.k + RC - 1) / RC), dim3(256), 0, s, g, st, mask, a)
+#define DFMI_SMODE(RC)                                        \
+  switch (mode) {                                             \
+    case MODE_APPLY: DFMI_SLAUNCH(RC, MODE_APPLY); break;      \
+    case MODE_RESID: DFMI_SLAUNCH(RC, MODE_RESID); break;      \
+    case MODE_CHEB: DFMI_SLAUNCH(RC, MODE_CHEB); break;        \
+    default: throw std::runtime_error("stencil_apply: bad mode"); \
+  }
+  if (a.k <= 1) {
+    DFMI_SMODE(1)
+  } else if (a.k <= 2) {
+    DFMI_SMODE(2)
+  } else if (a.k <= 4) {
+    DFMI_SMODE(4)
+  } else {
+    DFMI_SMODE(8)
+  }
+#undef DFMI_SMODE
+#undef DFMI_SLAUNCH
+  DFMI_HIP(hipGetLastError());
+}
+
+template void stencil_apply<float>(const Geom&, const float*, const uint8_t*, int, const ApplyArgs&, hipStream_t,
+                                   int*);
+template void stencil_apply<double>(const Geom&, const double*, const uint8_t*, int, const ApplyArgs&, hipStream_t,
+                                    int*);
+
+// ---- Galerkin product P^T A P in stencil form -------------------------------------------
+// Coarse node N, offset D: sum over fine f = 2N + e (weight w(e)) and fine neighbours
+// g = f + df inside the support of coarse node N + D (weight w(g - 2(N + D))).
+__global__ void k_rap(Geom gf, const double* __restrict__ fst, Geom gc, double* __restrict__ cst) {
+  const long long Nc = gc.nodes(), Nf = gf.nodes();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int D = blockIdx.y;
+  if (n >= Nc) return;
+  const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
+  const int MI = I + D / 9 - 1, MJ = J + (D / 3) % 3 - 1, MK = K + D % 3 - 1;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (MI >= 0 && MJ >= 0 && MK >= 0 && MI < gc.nx && MJ < gc.ny && MK < gc.nz) {
+    for (int e = 0; e < 27; ++e) {
+      const int ex = e / 9 - 1, ey = (e / 3) % 3 - 1, ez = e % 3 - 1;
+      const int fi = 2 * I + ex, fj = 2 * J + ey, fk = 2 * K + ez;
+      if (fi < 0 || fj < 0 || fk < 0 || fi >= gf.nx || fj >= gf.ny || fk >= gf.nz) continue;
+      const double wf = weight1(ex) * weight1(ey) * weight1(ez);
+      const long long f = ((long long)fi * gf.ny + fj) * gf.nz + fk;
+      for (int df = 0; df < 27; ++df) {
+        const int gi = fi + df / 9 - 1, gj = fj + (df / 3) % 3 - 1, gk = fk + df % 3 - 1;
+        const int tx = gi - 2 * MI, ty = gj - 2 * MJ, tz = gk - 2 * MK;
+        if (tx < -1 || tx > 1 || ty < -1 || ty > 1 || tz < -1 || tz > 1) continue;
+        if (gi < 0 || gj < 0 || gk < 0 || gi >= gf.nx || gj >= gf.ny || gk >= gf.nz) continue;
+        const double w = wf * weight1(tx) * weight1(ty) * weight1(tz);
+        const double* s = fst + (long long)df * 9 * Nf + f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] += w * s[q * Nf];
+      }
+    }
+  }
+  for (int q = 0; q < 9; ++q) cst[((long long)D * 9 + q) * Nc + n] = acc[q];
+}
+
+void stencil_rap(const Geom& fine, const double* fst, const Geom& coarse, double* cst, hipStream_t s) {
+  hipLaunchKernelGGL(k_rap, dim3((unsigned)((coarse.nodes() + 255) / 256), 27), dim3(256), 0, s, fine, fst, coarse,
+                     cst);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_stencil_diag(Geom g, const double* __restrict__ st, double* __restrict__ diag) {
+  const long long N = g.nodes();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  for (int c = 0; c < 3; ++c) diag[n * 3 + c] = st[((long long)13 * 9 + 4 * c) * N + n];
+}
+
+void stencil_diagonal(const Geom& g, const double* st, double* diag, hipStream_t s) {
+  hipLaunchKernelGGL(k_stencil_diag, dim3((unsigned)((g.nodes() + 255) / 256)), dim3(256), 0, s, g, st, diag);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_dead_mask(Geom g, const double* __restrict__ diag, double thr, uint8_t* __restrict__ mask) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nodes()) return;
+  uint8_t m = 0;
+  for (int c = 0; c < 3; ++c)
+    if (diag[n * 3 + c] > thr) m |= (uint8_t)(1 << c);
+  mask[n] = m;
+}
+
+void dead_mask(const Geom& g, const double* diag, double threshold, uint8_t* mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_dead_mask, dim3((unsigned)((g.nodes() + 255) / 256)), dim3(256), 0, s, g, diag, threshold,
+                     mask);
+  DFMI_HIP(hipGetLastError());
+}
+
+// zero rows and columns of masked DOFs: st_nm[c][c'] *= M_nc M_mc'
+__global__ void k_stencil_mask(Geom g, double* __restrict__ st, const uint8_t* __restrict__ mask) {
+  const long long N = g.nodes();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (n >= N) return;
+  const int l = (int)(n % g.nz), j = (int)((n / g.nz) % g.ny), i = (int)(n / ((long long)g.nz * g.ny));
+  const int mi = i + d / 9 - 1, mj = j + (d / 3) % 3 - 1, ml = l + d % 3 - 1;
+  const bool in = mi >= 0 && mj >= 0 && ml >= 0 && mi < g.nx && mj < g.ny && ml < g.nz;
+  const uint8_t mn = mask[n];
+  const uint8_t mm = in ? mask[((long long)mi * g.ny + mj) * g.nz + ml] : 0;
+  for (int c = 0; c < 3; ++c)
+    for (int cc = 0; cc < 3; ++cc)
+      if (!live_bit(mn, c) || !live_bit(mm, cc)) st[((long long)d * 9 + 3 * c + cc) * N + n] = 0.0;
+}
+
+void stencil_mask(const Geom& g, double* st, const uint8_t* mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_stencil_mask, dim3((unsigned)((g.nodes() + 255) / 256), 27), dim3(256), 0, s, g, st, mask);
+  DFMI_HIP(hipGetLastError());
+}
+
+// ---- transfer operators ------------------------------------------------------------------
+template <typename T>
+__global__ void k_restrict(Geom gf, const T* __restrict__ f, Geom gc, const uint8_t* __restrict__ cmask,
+                           T* __restrict__ c, const int* active) {
+  const long long Nc = gc.nodes(), Nf = gf.nodes();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (n >= Nc || (active && !active[r])) return;
+  const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
+  const T* F = f + (long long)r * Nf * 3;
+  double acc[3] = {0, 0, 0};
+  for (int e = 0; e < 27; ++e) {
+    const int ex = e / 9 - 1, ey = (e / 3) % 3 - 1, ez = e % 3 - 1;
+    const int fi = 2 * I + ex, fj = 2 * J + ey, fk = 2 * K + ez;
+    if (fi < 0 || fj < 0 || fk < 0 || fi >= gf.nx || fj >= gf.ny || fk >= gf.nz) continue;
+    const double w = weight1(ex) * weight1(ey) * weight1(ez);
+    const T* v = F + (((long long)fi * gf.ny + fj) * gf.nz + fk) * 3;
+    acc[0] += w * (double)v[0];
+    acc[1] += w * (double)v[1];
+    acc[2] += w * (double)v[2];
+  }
+  const uint8_t m = cmask[n];
+  T* C = c + ((long long)r * Nc + n) * 3;
+  for (int q = 0; q < 3; ++q) C[q] = live_bit(m, q) ? (T)acc[q] : T(0);
+}
+
+template <typename T>
+void restrict_full(const Geom& fine, const T* f, const Geom& coarse, const uint8_t* cmask, T* c, int k,
+                   const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((coarse.nodes() + 255) / 256), k), dim3(256), 0, s, fine, f,
+                     coarse, cmask, c, active);
+  DFMI_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_prolong_add(Geom gc, const T* __restrict__ c, Geom gf, const uint8_t* __restrict__ fmask,
+                              T* __restrict__ f, const int* active) {
+  const long long Nc = gc.nodes(), Nf = gf.nodes();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (n >= Nf || (active && !active[r])) return;
+  const int k = (int)(n % gf.nz), j = (int)((n / gf.nz) % gf.ny), i = (int)(n / ((long long)gf.nz * gf.ny));
+  const T* C = c + (long long)r * Nc * 3;
+  double acc[3] = {0, 0, 0};
+  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+  const double wi = (i & 1) ? 0.5 : 1.0, wj = (j & 1) ? 0.5 : 1.0, wk = (k & 1) ? 0.5 : 1.0;
+  for (int a = 0; a < ni; ++a)
+    for (int b = 0; b < nj; ++b)
+      for (int d = 0; d < nk; ++d) {
+        const T* v = C + (((long long)(i0 + a) * gc.ny + j0 + b) * gc.nz + k0 + d) * 3;
+        const double w = wi * wj * wk;
+        acc[0] += w * (double)v[0];
+        acc[1] += w * (double)v[1];
+        acc[2] += w * (double)v[2];
+      }
+  const uint8_t m = fmask[n];
+  T* F = f + ((long long)r * Nf + n) * 3;
+  for (int q = 0; q < 3; ++q)
+    if (live_bit(m, q)) F[q] = F[q] + (T)acc[q];
+}
+
+template <typename T>
+void prolong_add(const Geom& coarse, const T* c, const Geom& fine, const uint8_t* fmask, T* f, int k,
+                 const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((fine.nodes() + 255) / 256), k), dim3(256), 0, s, coarse, c,
+                     fine, fmask, f, active);
+  DFMI_HIP(hipGetLastError());
+}
+
+template void restrict_full<float>(const Geom&, const float*, const Geom&, const uint8_t*, float*, int, const int*,
+                                   hipStream_t);
+template void restrict_full<double>(const Geom&, const double*, const Geom&, const uint8_t*, double*, int,
+                                    const int*, hipStream_t);
+template void prolong_add<float>(const Geom&, const float*, const Geom&, const uint8_t*, float*, int, const int*,
+                                 hipStream_t);
+template void prolong_add<double>(const Geom&, const double*, const Geom&, const uint8_t*, double*, int, const int*,
+                                  hipStream_t);
+
+// ---- coarsest level: dense inverse of the live-DOF Galerkin matrix -------------------------
+__global__ void k_coarsest_assemble(Geom g, const double* __restrict__ st, const int* __restrict__ live_index, int n,
+                                    double* __restrict__ dense) {
+  const long long N = g.nodes();
+  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = blockIdx.y;
+  if (node >= N) return;
+  const int l = (int)(node % g.nz), j = (int)((node / g.nz) % g.ny), i = (int)(node / ((long long)g.nz * g.ny));
+  const int mi = i + d / 9 - 1, mj = j + (d / 3) % 3 - 1, ml = l + d % 3 - 1;
+  if (mi < 0 || mj < 0 || ml < 0 || mi >= g.nx || mj >= g.ny || ml >= g.nz) return;
+  const long long m = ((long long)mi * g.ny + mj) * g.nz + ml;
+  for (int c = 0; c < 3; ++c) {
+    const int row = live_index[node * 3 + c];
+    if (row < 0) continue;
+    for (int cc = 0; cc < 3; ++cc) {
+      const int col = live_index[m * 3 + cc];
+      if (col < 0) continue;
+      dense[(long long)row * n + col] = st[((long long)d * 9 + 3 * c + cc) * N + node];
+    }
+  }
+}
+
+__global__ void k_symmetrize(double* a, int n) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  if (j <= i) return;
+  const double v = 0.5 * (a[(long long)i * n + j] + a[(long long)j * n + i]);
+  a[(long long)i * n + j] = v;
+  a[(long long)j * n + i] = v;
+}
+
+void coarsest_assemble(const Geom& g, const double* st, const int* live_index, int n, double* dense,
+                       hipStream_t s) {
+  DFMI_HIP(hipMemsetAsync(dense, 0, sizeof(double) * (size_t)n * n, s));
+  hipLaunchKernelGGL(k_coarsest_assemble, dim3((unsigned)((g.nodes() + 255) / 256), 27), dim3(256), 0, s, g, st,
+                     live_index, n, dense);
+  DFMI_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)(((long long)n * n + 255) / 256)), dim3(256), 0, s, dense, n);
+  DFMI_HIP(hipGetLastError());
+}
+
+// Symmetric sweep operator (Goodnight 1979): sweeping every pivot of an SPD matrix
+// leaves -A^-1.  Row k is copied first so the rank-1 update reads unmodified values.
+__global__ void k_sweep_copy(const double* a, int n, int k, double* row) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) row[j] = a[(long long)k * n + j];
+}
+
+__global__ void k_sweep(double* a, int n, int k, const double* row) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  const double d = row[k];
+  double v;
+  if (i == k && j == k) v = -1.0 / d;
+  else if (i == k) v = row[j] / d;
+  else if (j == k) v = row[i] / d;
+  else v = a[idx] - row[i] * row[j] / d;
+  a[idx] = v;
+}
+
+__global__ void k_negate(double* a, long long n) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) a[idx] = -a[idx];
+}
+
+void dense_sweep_inverse(double* a, int n, hipStream_t s) {
+  double* row = nullptr;
+  DFMI_HIP(hipMallocAsync((void**)&row, sizeof(double) * n, s));
+  const unsigned b1 = (unsigned)((n + 255) / 256), b2 = (unsigned)(((long long)n * n + 255) / 256);
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_sweep_copy, dim3(b1), dim3(256), 0, s, a, n, k, row);
+    hipLaunchKernelGGL(k_sweep, dim3(b2), dim3(256), 0, s, a, n, k, row);
+  }
+  hipLaunchKernelGGL(k_negate, dim3(b2), dim3(256), 0, s, a, (long long)n * n);
+  DFMI_HIP(hipGetLastError());
+  DFMI_HIP(hipFreeAsync(row, s));
+}
+
+template <typename T>
+__global__ void k_gather_live(const T* __restrict__ v, long long nodes, const int* __restrict__ live_dofs, int n,
+                              double* __restrict__ out, const int* active) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (idx >= n) return;
+  out[(long long)r * n + idx] = (active && !active[r]) ? 0.0 : (double)v[(long long)r * nodes * 3 + live_dofs[idx]];
+}
+
+// out[r][j] = sum_i in[r][i] ainv[i][j]; 256 columns per block, 8 right-hand sides per block row
+__global__ __launch_bounds__(256) void k_dense_rows(const double* __restrict__ in, const double* __restrict__ ainv,
+                                                    int n, int k, double* __restrict__ out) {
+  __shared__ double tile[8][64];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int r0 = blockIdx.y * 8;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    for (int v = threadIdx.x; v < 8 * 64; v += 256) {
+      const int rr = v / 64, ii = v % 64;
+      tile[rr][ii] = (r0 + rr < k && i0 + ii < n) ? in[(long long)(r0 + rr) * n + i0 + ii] : 0.0;
+    }
+    __syncthreads();
+    if (j < n) {
+      const int lim = min(64, n - i0);
+      for (int ii = 0; ii < lim; ++ii) {
+        const double a = ainv[(long long)(i0 + ii) * n + j];
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) acc[rr] = fma(tile[rr][ii], a, acc[rr]);
+      }
+    }
+    __syncthreads();
+  }
+  if (j < n)
+    for (int rr = 0; rr < 8; ++rr)
+      if (r0 + rr < k) out[(long long)(r0 + rr) * n + j] = acc[rr];
+}
+
+template <typename T>
+__global__ void k_scatter_live(const double* __restrict__ in, long long nodes, const int* __restrict__ live_dofs,
+                               int n, T* __restrict__ v, const int* active) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (idx >= n || (active && !active[r])) return;
+  v[(long long)r * nodes * 3 + live_dofs[idx]] = (T)in[(long long)r * n + idx];
+}
+
+template <typename T>
+void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out,
+                    double* work_in, double* work_out, int k, const int* active, hipStream_t s) {
+  DFMI_HIP(hipMemsetAsync(out, 0, sizeof(T) * (size_t)k * g.nodes() * 3, s));
+  if (n == 0) return;
+  const unsigned b = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_gather_live<T>, dim3(b, k), dim3(256), 0, s, rhs, g.nodes(), live_dofs, n, work_in, active);
+  hipLaunchKernelGGL(k_dense_rows, dim3(b, (k + 7) / 8), dim3(256), 0, s, work_in, ainv, n, k, work_out);
+  hipLaunchKernelGGL(k_scatter_live<T>, dim3(b, k), dim3(256), 0, s, work_out, g.nodes(), live_dofs, n, out, active);
+  DFMI_HIP(hipGetLastError());
+}
+
+template void coarsest_solve<float>(const Geom&, const double*, const int*, int, const float*, float*, double*,
+                                    double*, int, const int*, hipStream_t);
+template void coarsest_solve<double>(const Geom&, const double*, const int*, int, const double*, double*, double*,
+                                     double*, int, const int*, hipStream_t);
+
+}  // namespace dfmi

@@ -1,0 +1,382 @@
+// Column-batched vector kernels for PCG and the smoother: per-column reductions
+// (wavefront shuffle + per-block partials, then a fixed-order second pass, so
+// results are deterministic), AXPY-type updates, precision conversion and the
+// flat <-> padded-grid layout changes.  Column r of a multi-vector is the
+// contiguous slice v[r * n .. (r + 1) * n).
+#include "dfmi_internal.h"
+
+namespace dfmi {
+
+int reduce_blocks(long long n) {
+  long long b = (n + 1023) / 1024;
+  return (int)std::max(1LL, std::min(b, 512LL));
+}
+
+__device__ inline double block_sum(double v) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__device__ inline double block_max(double v) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_column_dot(const T* __restrict__ a, const T* __restrict__ b, long long n,
+                                                     double* partial, const int* active) {
+  const int r = blockIdx.y;
+  double acc = 0.0;
+  if (!active || active[r]) {
+    const T* A = a + (long long)r * n;
+    const T* B = b + (long long)r * n;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+      acc += (double)A[i] * (double)B[i];
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[(long long)r * gridDim.x + blockIdx.x] = acc;
+}
+
+template <typename T>
+void column_dot(const T* a, const T* b, long long n, int k, double* partial, int nblocks, const int* active,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_column_dot<T>, dim3(nblocks, k), dim3(256), 0, s, a, b, n, partial, active);
+  DFMI_HIP(hipGetLastError());
+}
+template void column_dot<float>(const float*, const float*, long long, int, double*, int, const int*, hipStream_t);
+template void column_dot<double>(const double*, const double*, long long, int, double*, int, const int*,
+                                 hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_column_absmax(const T* __restrict__ a, long long n, double* partial,
+                                                        const int* active) {
+  const int r = blockIdx.y;
+  double acc = 0.0;
+  if (!active || active[r]) {
+    const T* A = a + (long long)r * n;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+      acc = fmax(acc, fabs((double)A[i]));
+  }
+  acc = block_max(acc);
+  if (threadIdx.x == 0) partial[(long long)r * gridDim.x + blockIdx.x] = acc;
+}
+
+template <typename T>
+void column_absmax(const T* a, long long n, int k, double* partial, int nblocks, const int* active,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_column_absmax<T>, dim3(nblocks, k), dim3(256), 0, s, a, n, partial, active);
+  DFMI_HIP(hipGetLastError());
+}
+template void column_absmax<float>(const float*, long long, int, double*, int, const int*, hipStream_t);
+template void column_absmax<double>(const double*, long long, int, double*, int, const int*, hipStream_t);
+
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* partial, int nblocks, double* out, int op) {
+  const int r = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += 256) {
+    const double v = partial[(long long)r * nblocks + i];
+    acc = op == 0 ? acc + v : fmax(acc, v);
+  }
+  acc = op == 0 ? block_sum(acc) : block_max(acc);
+  if (threadIdx.x == 0) out[r] = acc;
+}
+
+void reduce_partials(const double* partial, int nblocks, int k, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, s, partial, nblocks, out, 0);
+  DFMI_HIP(hipGetLastError());
+}
+
+void reduce_max(const double* partial, int nblocks, int k, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, s, partial, nblocks, out, 1);
+  DFMI_HIP(hipGetLastError());
+}
+
+// first Chebyshev step from a zero iterate: x = c * Dinv * b
+template <typename T>
+__global__ void k_cheb_start(const T* __restrict__ b, const T* __restrict__ dinv, double c, T* __restrict__ x,
+                             long long nodes, const int* active) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (i >= nodes * 3 || (active && !active[r])) return;
+  const long long o = (long long)r * nodes * 3 + i;
+  x[o] = (T)c * dinv[i] * b[o];
+}
+
+template <typename T>
+void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * 3 + 255) / 256), k), dim3(256), 0, s, b, dinv, c, x,
+                     nodes, active);
+  DFMI_HIP(hipGetLastError());
+}
+template void cheb_start<float>(const float*, const float*, double, float*, long long, int, const int*,
+                                hipStream_t);
+template void cheb_start<double>(const double*, const double*, double, double*, long long, int, const int*,
+                                 hipStream_t);
+
+// deterministic pseudo-random start vector in [-0.5, 0.5) on live DOFs (power iteration)
+__global__ void k_power_init(const uint8_t* __restrict__ mask, double* __restrict__ v, long long nodes, unsigned seed) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nodes * 3) return;
+  unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ULL + seed;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  v[i] = ((mask[i / 3] >> (i % 3)) & 1) ? u : 0.0;
+}
+
+void power_init(const uint8_t* mask, double* v, long long nodes, unsigned seed, hipStream_t s) {
+  hipLaunchKernelGGL(k_power_init, dim3((unsigned)((nodes * 3 + 255) / 256)), dim3(256), 0, s, mask, v, nodes, seed);
+  DFMI_HIP(hipGetLastError());
+}
+
+// v = Dinv y * inv_norm[0]
+__global__ void k_dinv_scale(const double* __restrict__ y, const double* __restrict__ dinv, const double* inv_norm,
+                             double* __restrict__ v, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = dinv[i] * y[i] * inv_norm[0];
+}
+
+void dinv_scale(const double* y, const double* dinv, const double* inv_norm, double* v, long long nodes,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_dinv_scale, dim3((unsigned)((nodes * 3 + 255) / 256)), dim3(256), 0, s, y, dinv, inv_norm, v,
+                     nodes * 3);
+  DFMI_HIP(hipGetLastError());
+}
+
+// Polak-Ribiere: beta = (r.z_new - r.z_old) / rz_old, rz = r.z_new
+__global__ void k_pcg_beta(const double* rzn, const double* rzc, double* rz, double* beta, int k, const int* active) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  if (active && !active[c]) {
+    beta[c] = 0.0;
+    return;
+  }
+  beta[c] = rz[c] != 0.0 ? (rzn[c] - rzc[c]) / rz[c] : 0.0;
+  rz[c] = rzn[c];
+}
+
+void pcg_beta(const double* rzn, const double* rzc, double* rz, double* beta, int k, const int* active,
+              hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_beta, dim3((k + 63) / 64), dim3(64), 0, s, rzn, rzc, rz, beta, k, active);
+  DFMI_HIP(hipGetLastError());
+}
+
+// dst[c] = src[c] for columns with select[c] != 0
+__global__ void k_copy_columns(const double* __restrict__ src, double* __restrict__ dst, long long n,
+                               const int* select) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n || (select && !select[c])) return;
+  dst[(long long)c * n + i] = src[(long long)c * n + i];
+}
+
+void copy_columns(const double* src, double* dst, long long n, int k, const int* select, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_columns, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, src, dst, n, select);
+  DFMI_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_fill_zero(T* x, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = T(0);
+}
+
+template <typename T>
+void fill_zero(T* x, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n);
+  DFMI_HIP(hipGetLastError());
+}
+template void fill_zero<float>(float*, long long, hipStream_t);
+template void fill_zero<double>(double*, long long, hipStream_t);
+
+// x += alpha p, r -= alpha Ap, partial |r|^2 (columns with alpha = 0 are left unchanged)
+__global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ x, double* __restrict__ r,
+                                                     const double* __restrict__ p, const double* __restrict__ ap,
+                                                     const double* alpha, long long n, const int* active,
+                                                     double* partial) {
+  const int c = blockIdx.y;
+  double acc = 0.0;
+  if (!active || active[c]) {
+    const double a = alpha[c];
+    const long long off = (long long)c * n;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+      x[off + i] += a * p[off + i];
+      const double rv = r[off + i] - a * ap[off + i];
+      r[off + i] = rv;
+      acc += rv * rv;
+    }
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partial[(long long)c * gridDim.x + blockIdx.x] = acc;
+}
+
+void pcg_update(double* x, double* r, const double* p, const double* ap, const double* alpha, long long n, int k,
+                const int* active, double* partial_rr, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(nblocks, k), dim3(256), 0, s, x, r, p, ap, alpha, n, active, partial_rr);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_pcg_direction(double* __restrict__ p, const double* __restrict__ z, const double* beta, long long n,
+                                const int* active) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n || (active && !active[c])) return;
+  const long long o = (long long)c * n + i;
+  p[o] = z[o] + beta[c] * p[o];
+}
+
+void pcg_direction(double* p, const double* z, const double* beta, long long n, int k, const int* active,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_direction, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, p, z, beta, n, active);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_alpha(const double* rz, const double* pap, double* alpha, int k, const int* active) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k) return;
+  alpha[c] = (active && !active[c]) || pap[c] == 0.0 ? 0.0 : rz[c] / pap[c];
+}
+
+void pcg_scalars_alpha(const double* rz, const double* pap, double* alpha, int k, const int* active,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_alpha, dim3((k + 63) / 64), dim3(64), 0, s, rz, pap, alpha, k, active);
+  DFMI_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_to_work(const double* __restrict__ r, const double* scale, T* __restrict__ w, long long n,
+                          const int* active) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n || (active && !active[c])) return;
+  const double sc = scale[c] > 1e-300 ? 1.0 / scale[c] : 1.0;
+  w[(long long)c * n + i] = (T)(r[(long long)c * n + i] * sc);
+}
+
+template <typename T>
+void to_work(const double* r, const double* scale, T* w, long long n, int k, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_work<T>, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, r, scale, w, n, active);
+  DFMI_HIP(hipGetLastError());
+}
+template void to_work<float>(const double*, const double*, float*, long long, int, const int*, hipStream_t);
+template void to_work<double>(const double*, const double*, double*, long long, int, const int*, hipStream_t);
+
+template <typename T>
+__global__ void k_from_work(const T* __restrict__ w, const double* scale, double* __restrict__ z, long long n,
+                            const int* active) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n || (active && !active[c])) return;
+  const double sc = scale[c] > 1e-300 ? scale[c] : 1.0;
+  z[(long long)c * n + i] = (double)w[(long long)c * n + i] * sc;
+}
+
+template <typename T>
+void from_work(const T* w, const double* scale, double* z, long long n, int k, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(k_from_work<T>, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, w, scale, z, n, active);
+  DFMI_HIP(hipGetLastError());
+}
+template void from_work<float>(const float*, const double*, double*, long long, int, const int*, hipStream_t);
+template void from_work<double>(const double*, const double*, double*, long long, int, const int*, hipStream_t);
+
+// flat DOF vectors (reference numbering over the unpadded (ux+1, uy+1, uz+1) nodes) <->
+// padded grid vectors over pad.nodes(); padding nodes are zero
+__global__ void k_flat_to_grid(Geom pad, int vx, int vy, int vz, const double* __restrict__ flat,
+                               double* __restrict__ grid) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  const long long N = pad.nodes();
+  if (n >= N) return;
+  const int l = (int)(n % pad.nz), j = (int)((n / pad.nz) % pad.ny), i = (int)(n / ((long long)pad.nz * pad.ny));
+  double* G = grid + ((long long)r * N + n) * 3;
+  if (i < vx && j < vy && l < vz) {
+    const long long u = ((long long)i * vy + j) * vz + l;
+    const double* F = flat + ((long long)r * vx * vy * vz + u) * 3;
+    G[0] = F[0];
+    G[1] = F[1];
+    G[2] = F[2];
+  } else {
+    G[0] = G[1] = G[2] = 0.0;
+  }
+}
+
+void flat_to_grid(const Geom& pad, int ux, int uy, int uz, const double* flat, double* grid, int k, hipStream_t s) {
+  hipLaunchKernelGGL(k_flat_to_grid, dim3((unsigned)((pad.nodes() + 255) / 256), k), dim3(256), 0, s, pad, ux + 1,
+                     uy + 1, uz + 1, flat, grid);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_grid_to_flat(Geom pad, int vx, int vy, int vz, const double* __restrict__ grid,
+                               double* __restrict__ flat) {
+  const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  const long long V = (long long)vx * vy * vz;
+  if (u >= V) return;
+  const int l = (int)(u % vz), j = (int)((u / vz) % vy), i = (int)(u / ((long long)vz * vy));
+  const long long n = ((long long)i * pad.ny + j) * pad.nz + l;
+  const double* G = grid + ((long long)r * pad.nodes() + n) * 3;
+  double* F = flat + ((long long)r * V + u) * 3;
+  F[0] = G[0];
+  F[1] = G[1];
+  F[2] = G[2];
+}
+
+void grid_to_flat(const Geom& pad, int ux, int uy, int uz, const double* grid, double* flat, int k, hipStream_t s) {
+  const long long V = (long long)(ux + 1) * (uy + 1) * (uz + 1);
+  hipLaunchKernelGGL(k_grid_to_flat, dim3((unsigned)((V + 255) / 256), k), dim3(256), 0, s, pad, ux + 1, uy + 1,
+                     uz + 1, grid, flat);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_pad_moduli(Geom pad, int ux, int uy, int uz, const double* __restrict__ m,
+                             double* __restrict__ m64, float* __restrict__ m32) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pad.elems()) return;
+  const int l = (int)(e % pad.ez), j = (int)((e / pad.ez) % pad.ey), i = (int)(e / ((long long)pad.ez * pad.ey));
+  double v = 0.0;
+  if (i < ux && j < uy && l < uz) v = m[((long long)i * uy + j) * uz + l];
+  m64[e] = v;
+  m32[e] = (float)v;
+}
+
+void pad_moduli(const Geom& pad, int ux, int uy, int uz, const double* moduli, double* m64, float* m32,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_pad_moduli, dim3((unsigned)((pad.elems() + 255) / 256)), dim3(256), 0, s, pad, ux, uy, uz,
+                     moduli, m64, m32);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_scale_vector(double* v, double a, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] *= a;
+}
+
+void scale_vector(double* v, double a, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_vector, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, a, n);
+  DFMI_HIP(hipGetLastError());
+}
+
+__global__ void k_mask_vector(const uint8_t* __restrict__ mask, double* __restrict__ v, long long nodes) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (i >= nodes * 3) return;
+  if (!((mask[i / 3] >> (i % 3)) & 1)) v[(long long)r * nodes * 3 + i] = 0.0;
+}
+
+void mask_vector(const uint8_t* mask, double* v, long long nodes, int k, hipStream_t s) {
+  hipLaunchKernelGGL(k_mask_vector, dim3((unsigned)((nodes * 3 + 255) / 256), k), dim3(256), 0, s, mask, v, nodes);
+  DFMI_HIP(hipGetLastError());
+}
+
+}  // namespace dfmi

@@ -1,0 +1,224 @@
+"""GeometricMultigrid with the reference's call surface, backed by libdfmi (HIP).
+
+Mirrors ``deep_frame/topology_multigrid.py`` ``GeometricMultigrid``: constructor
+(:47-79), ``update`` (:97-100), ``hierarchy`` (:145-208), ``solve`` (:336-375),
+``product`` (:455-457), ``element_energy`` (:458-459), ``diagnostics`` (:541-542),
+``close`` (:543-548) and the ``statistics`` list.  PyTorch only carries device
+memory and the current stream; all arithmetic runs in libdfmi.
+"""
+import ctypes
+from ctypes import byref, c_double, c_int32, c_void_p
+from time import perf_counter
+
+import numpy as np
+
+from deep_frame_amd import _native
+
+MULTIGRID = {"coarse": "galerkin", "coarsest_dofs": 80000, "max_levels": 8, "smoother": "chebyshev", "sweeps": 2,
+             "damping": 1.0, "chebyshev_degree": 3, "chebyshev_ratio": 20.0, "growth": 2.0, "cycle": "V",
+             "power_iterations": 20, "precision": "float32", "tolerance": 1e-8, "max_iterations": 2000,
+             "batch": 64, "projection": True, "dead_ratio": 1e-12, "zero_rhs": 1e-12, "levels": 0, "seed": 0,
+             "refine": False, "inner_tolerance": 1e-4, "device": "cuda"}
+PRECISIONS = {"float64": 0, "float32": 1}
+SLOTS = 8
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class GeometricMultigrid:
+    def __init__(self, shape, spacing, ke, settings=None, me=None):
+        torch = self.torch = _torch()
+        self.settings = {**MULTIGRID, **(settings or {})}
+        s = self.settings
+        if (s["precision"] not in PRECISIONS or s["coarse"] != "galerkin" or s["smoother"] != "chebyshev"
+                or s["cycle"] != "V"):
+            raise ValueError("Invalid multigrid settings")
+        if not torch.cuda.is_available():
+            raise RuntimeError("GeometricMultigrid needs a ROCm GPU (libdfmi has no CPU path)")
+        self.device = torch.device(s["device"])
+        self.shape = tuple(int(n) for n in shape)
+        self.spacing = np.asarray(spacing, dtype=float)
+        self.ke64 = np.ascontiguousarray(np.asarray(ke, dtype=float).reshape(24, 24))
+        self.nodes = tuple(n + 1 for n in self.shape)
+        self.ndof = 3 * int(np.prod(self.nodes))
+        self.nelem = int(np.prod(self.shape))
+        self.native = self._settings()
+        handle = c_void_p()
+        _native.call("dfmi_create", (c_int32 * 3)(*self.shape), (c_double * 3)(*self.spacing),
+                     self.ke64.ctypes.data_as(ctypes.POINTER(c_double)), byref(self.native), byref(handle))
+        self.handle = handle
+        levels, padded = c_int32(), (c_int32 * 3)()
+        _native.call("dfmi_levels", self.handle, byref(levels), padded)
+        self.level_count, self.padded = levels.value, tuple(padded)
+        self.moduli = None
+        self.hierarchies, self.statistics = {}, []
+        self._fixed_cache = {}
+
+    def _settings(self):
+        s = self.settings
+        n = _native.default_settings()
+        n.coarsest_dofs, n.max_levels, n.levels = int(s["coarsest_dofs"]), int(s["max_levels"]), int(s["levels"])
+        n.chebyshev_degree, n.chebyshev_ratio, n.growth = int(s["chebyshev_degree"]), float(s["chebyshev_ratio"]), float(s["growth"])
+        n.power_iterations, n.precision = int(s["power_iterations"]), PRECISIONS[s["precision"]]
+        n.tolerance, n.max_iterations, n.dead_ratio = float(s["tolerance"]), int(s["max_iterations"]), float(s["dead_ratio"])
+        n.seed, n.refine, n.inner_tolerance = int(s["seed"]), int(bool(s["refine"])), float(s["inner_tolerance"])
+        return n
+
+    def _stream(self):
+        _native.call("dfmi_set_stream", self.handle, _native.stream_handle(self.torch, self.device))
+
+    def _tensor(self, array, dtype=None):
+        torch = self.torch
+        return torch.as_tensor(np.ascontiguousarray(array), dtype=dtype or torch.float64, device=self.device)
+
+    def update(self, moduli):
+        """E per element (FP64, 0 = inactive); drops cached hierarchies (topology_multigrid.py:97-100)."""
+        torch = self.torch
+        m = moduli if isinstance(moduli, torch.Tensor) else self._tensor(np.asarray(moduli, float).ravel())
+        m = m.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
+        if m.numel() != self.nelem:
+            raise ValueError("moduli must have one value per element")
+        self._stream()
+        _native.call("dfmi_update", self.handle, _native.ptr(m))
+        self.moduli = m
+        self.hierarchies.clear()
+
+    def hierarchy(self, key, fixed):
+        """Build (or reuse) the Galerkin hierarchy for a fixture; returns its slot."""
+        if self.moduli is None:
+            raise RuntimeError("update(moduli) must precede solve")
+        if key in self.hierarchies:
+            return self.hierarchies[key]
+        if len(self.hierarchies) >= SLOTS:
+            self.hierarchies.pop(next(iter(self.hierarchies)))
+        used = set(h["slot"] for h in self.hierarchies.values())
+        slot = next(i for i in range(SLOTS) if i not in used)
+        flags = np.zeros(self.ndof, dtype=np.uint8)
+        fixed = np.asarray(fixed, dtype=np.int64)
+        if fixed.size and (fixed.min() < 0 or fixed.max() >= self.ndof):
+            raise ValueError("fixed DOF index out of range")
+        flags[fixed] = 1
+        dev = self._tensor(flags, self.torch.uint8)
+        self._stream()
+        seconds = c_double()
+        _native.call("dfmi_build_hierarchy", self.handle, slot, _native.ptr(dev), byref(seconds))
+        lam = (c_double * max(1, self.level_count - 1))()
+        coarsest = c_int32()
+        _native.call("dfmi_diagnostics", self.handle, slot, lam, byref(coarsest))
+        entry = {"slot": slot, "setup_s": seconds.value, "lambda_max": list(lam)[:self.level_count - 1],
+                 "coarsest_dofs": coarsest.value, "fixed": fixed}
+        self.hierarchies[key] = entry
+        return entry
+
+    def solve_device(self, key, fixed, rhs):
+        """rhs: (k, ndof) FP64 device tensor -> (x (k, ndof) device tensor, list of per-column reports)."""
+        torch = self.torch
+        entry = self.hierarchy(key, fixed)
+        rhs = rhs.to(device=self.device, dtype=torch.float64).contiguous()
+        k = rhs.shape[0]
+        x = torch.empty_like(rhs)
+        reports = (_native.Report * k)()
+        self._stream()
+        _native.call("dfmi_solve", self.handle, entry["slot"], _native.ptr(rhs), _native.ptr(x), k, reports)
+        return x, [(r.iterations, r.relative_residual, bool(r.converged)) for r in reports]
+
+    def solve(self, key, fixed, forces, support=None):
+        """Reference signature (topology_multigrid.py:336-375); forces (ndof, k) host FP64."""
+        if support is not None:
+            raise NotImplementedError("floating (inertia-relief) supports are not part of this build")
+        started = perf_counter()
+        entry = self.hierarchy(key, fixed)
+        forces = np.array(forces, dtype=float).reshape(self.ndof, -1)
+        norms = np.linalg.norm(forces, axis=0)
+        live = np.flatnonzero(norms > self.settings["zero_rhs"] * max(norms.max(initial=0.0), 1e-300))
+        flat = np.zeros_like(forces)
+        iterations, relative = np.zeros(forces.shape[1], dtype=int), np.zeros(forces.shape[1])
+        batches = 0
+        for start in range(0, len(live), self.settings["batch"]):
+            cols = live[start:start + self.settings["batch"]]
+            x, reps = self.solve_device(key, fixed, self._tensor(forces[:, cols].T))
+            flat[:, cols] = x.cpu().numpy().T
+            iterations[cols] = [r[0] for r in reps]
+            relative[cols] = [r[1] for r in reps]
+            batches += 1
+        converged = bool(np.all(relative[live] < self.settings["tolerance"])) if len(live) else True
+        report = {"iterations": iterations.tolist(), "relative_residual": relative.tolist(), "converged": converged,
+                  "batches": batches, "retried_columns": 0, "zero_rhs_columns": int(forces.shape[1] - len(live)),
+                  "seconds": perf_counter() - started, "setup_s": entry.pop("setup_s", 0.0), "floating": False,
+                  "levels": self.level_count, "coarsest_dofs": entry["coarsest_dofs"], "kernel": 0,
+                  "lambda_max": entry["lambda_max"]}
+        self.statistics.append(report)
+        return flat, report
+
+    def operator_device(self, x, key=None, precision="float64"):
+        """K x (key None) or M K M x with a hierarchy's fine mask; x (k, ndof) FP64 device tensor."""
+        x = x.to(device=self.device, dtype=self.torch.float64).contiguous()
+        y = self.torch.empty_like(x)
+        slot = -1 if key is None else self.hierarchies[key]["slot"]
+        self._stream()
+        _native.call("dfmi_apply", self.handle, slot, PRECISIONS[precision], _native.ptr(x), _native.ptr(y), x.shape[0])
+        return y
+
+    def level_operator(self, key, level, x):
+        """Level operator on the padded node grid of ``level``; x (k, 3 * nodes_l) FP64 device tensor."""
+        x = x.to(device=self.device, dtype=self.torch.float64).contiguous()
+        y = self.torch.empty_like(x)
+        self._stream()
+        _native.call("dfmi_level_apply", self.handle, self.hierarchies[key]["slot"], level, _native.ptr(x),
+                     _native.ptr(y), x.shape[0])
+        return y
+
+    def product(self, flat):
+        """FP64 K u for host columns (ndof, k) (topology_multigrid.py:455-457)."""
+        flat = np.asarray(flat, dtype=float).reshape(self.ndof, -1)
+        return self.operator_device(self._tensor(flat.T)).cpu().numpy().T
+
+    def element_energy_device(self, u):
+        u = u.to(device=self.device, dtype=self.torch.float64).contiguous()
+        out = self.torch.empty((u.shape[0], self.nelem), dtype=self.torch.float64, device=self.device)
+        self._stream()
+        _native.call("dfmi_element_energy", self.handle, _native.ptr(u), u.shape[0], _native.ptr(out))
+        return out
+
+    def element_energy(self, flat):
+        """u_e^T Ke u_e per column and element, (k, Ne) host (topology_multigrid.py:458-459)."""
+        flat = np.asarray(flat, dtype=float).reshape(self.ndof, -1)
+        return self.element_energy_device(self._tensor(flat.T)).cpu().numpy()
+
+    def compliance_sensitivity_device(self, u, f, dmoduli, factor=1.0):
+        """(compliance (k,), sensitivity (Ne,)) device tensors (topology_multigrid.py:568-579)."""
+        torch = self.torch
+        u = u.to(device=self.device, dtype=torch.float64).contiguous()
+        f = f.to(device=self.device, dtype=torch.float64).contiguous()
+        d = dmoduli.to(device=self.device, dtype=torch.float64).contiguous()
+        comp = torch.empty(u.shape[0], dtype=torch.float64, device=self.device)
+        sens = torch.empty(self.nelem, dtype=torch.float64, device=self.device)
+        self._stream()
+        _native.call("dfmi_compliance_sensitivity", self.handle, _native.ptr(u), _native.ptr(f), u.shape[0],
+                     _native.ptr(d), float(factor), _native.ptr(comp), _native.ptr(sens))
+        return comp, sens
+
+    def diagnostics(self):
+        return {"settings": self.settings, "levels": [{"shape": [n >> l for n in self.padded],
+                                                      "spacing_mm": (self.spacing * 2 ** l).tolist()}
+                                                     for l in range(self.level_count)],
+                "padded_shape": list(self.padded), "solves": self.statistics[-64:]}
+
+    def close(self):
+        errors = []
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            try:
+                _native.call("dfmi_destroy", self.handle)
+            except RuntimeError as error:
+                errors.append(str(error))
+            self.handle = None
+        return errors
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,136 @@
+/*
+ * libdfmi — MI355X-native SIMP inner loop (hex8 K.u, GMG-PCG, compliance,
+ * sensitivity, density filter, OC/MMA updates) behind a plain C ABI.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero status on failure;
+ *     dfmi_last_error() then holds a message (thread-local).
+ *   - Pointers named *_dev are device pointers (e.g. torch tensor data_ptr());
+ *     the caller owns them.  The library owns its hierarchy and workspace.
+ *   - Work is queued on the stream set by dfmi_set_stream (default: the null
+ *     stream).  Functions that return host values synchronise that stream.
+ *   - Flat DOF vectors use the reference numbering 3*node + component with nodes
+ *     in C order over (nx+1, ny+1, nz+1); a multi-vector with k columns is
+ *     stored column after column: v[r * ndof + dof].
+ *   - Element fields are (nx, ny, nz) in C order.
+ *   - One handle per thread; calls on one handle must not overlap.
+ */
+#ifndef DFMI_H
+#define DFMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dfmi_mg dfmi_mg;
+typedef struct dfmi_filter dfmi_filter;
+
+/* Settings of deep_frame/topology_multigrid.py:8-10 (MULTIGRID) that the solver uses. */
+typedef struct dfmi_settings {
+  int32_t coarsest_dofs;     /* 80000: add levels while the next-coarser level has more DOFs */
+  int32_t max_levels;        /* 8 */
+  int32_t levels;            /* 0 = choose from coarsest_dofs/max_levels, else force this count */
+  int32_t chebyshev_degree;  /* 3 */
+  double chebyshev_ratio;    /* 20.0 */
+  double growth;             /* 2.0: degree at level l = round(degree * growth^l) */
+  int32_t power_iterations;  /* 20 */
+  int32_t precision;         /* V-cycle arithmetic: 0 = float64, 1 = float32 */
+  double tolerance;          /* 1e-8, relative residual per column */
+  int32_t max_iterations;    /* 2000 */
+  double dead_ratio;         /* 1e-12 */
+  int32_t seed;              /* power-iteration start vector seed */
+  int32_t refine;            /* 0 = flexible PCG with FP64 outer operator (reference scheme);
+                                1 = iterative refinement: FP64 residual, inner PCG in the V-cycle precision */
+  double inner_tolerance;    /* refine = 1: inner relative tolerance per outer step */
+} dfmi_settings;
+
+/* Per-column outcome of a solve. */
+typedef struct dfmi_report {
+  int32_t iterations;        /* PCG iterations (all outer steps for refine = 1) */
+  double relative_residual;  /* true FP64 |b - K x| / |b| at exit */
+  int32_t converged;         /* relative_residual < tolerance */
+} dfmi_report;
+
+const char* dfmi_last_error(void);
+void dfmi_default_settings(dfmi_settings* out);
+
+/* GeometricMultigrid(shape, spacing, ke, settings) — deep_frame/topology_multigrid.py:47-79.
+ * ke: 24x24 row-major FP64 element stiffness at unit modulus, reference DOF order
+ * (3*corner + component, corners as deep_frame/topology_optimization.py:246-249). */
+int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
+                dfmi_mg** out);
+int dfmi_destroy(dfmi_mg* mg);
+int dfmi_set_stream(dfmi_mg* mg, void* hip_stream);
+/* Level count and padded element shape (topology_multigrid.py:58-63). */
+int dfmi_levels(dfmi_mg* mg, int32_t* levels, int32_t padded[3]);
+
+/* GeometricMultigrid.update(moduli) — topology_multigrid.py:97-100.  moduli_dev: FP64 element
+ * field (E_e, 0 = inactive).  Drops every cached hierarchy. */
+int dfmi_update(dfmi_mg* mg, const double* moduli_dev);
+
+/* GeometricMultigrid.hierarchy(key, fixed) — topology_multigrid.py:145-208: masks, Galerkin
+ * levels, lambda_max, coarsest factor.  fixed_dev: uint8 per flat DOF (1 = fixed).  slot < 8
+ * plays the role of the reference's cache key.  setup_seconds may be NULL. */
+int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed_dev, double* setup_seconds);
+
+/* GeometricMultigrid.solve(key, fixed, forces) for a built slot — topology_multigrid.py:336-375
+ * (batched flexible PCG :407-454).  rhs_dev, x_dev: k flat FP64 columns.  report: host array of k. */
+int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs_dev, double* x_dev, int32_t k, dfmi_report* report);
+
+/* GeometricMultigrid.operator / product — topology_multigrid.py:105-109, :455-457.
+ * slot < 0: unmasked K(E) x; slot >= 0: M K M x with that hierarchy's fine mask.
+ * precision: 0 = FP64, 1 = FP32 (inputs and outputs stay FP64). */
+int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x_dev, double* y_dev, int32_t k);
+
+/* GeometricMultigrid.element_energy — topology_multigrid.py:458-459: energy_dev[r*Ne + e] = u_e^T Ke u_e. */
+int dfmi_element_energy(dfmi_mg* mg, const double* u_dev, int32_t k, double* energy_dev);
+
+/* solve_elasticity / HexElasticity._collect — topology_multigrid.py:568-579,
+ * deep_frame/topology_optimization.py:721-734, :707: compliance_dev[r] = factor * f_r . u_r and
+ * sensitivity_dev[e] = -dmoduli[e] * factor * sum_r u_re^T Ke u_re. */
+int dfmi_compliance_sensitivity(dfmi_mg* mg, const double* u_dev, const double* f_dev, int32_t k,
+                                const double* dmoduli_dev, double factor, double* compliance_dev,
+                                double* sensitivity_dev);
+
+/* Operator of level `level` of a built slot on its padded node grid (x, y: k columns of
+ * 3 * nodes_l FP64, node-major); level 0 is M K M, coarser levels the Galerkin P^T A P
+ * (deep_frame/topology_multigrid.py:110-117).  For tests and diagnostics. */
+int dfmi_level_apply(dfmi_mg* mg, int32_t slot, int32_t level, const double* x_dev, double* y_dev, int32_t k);
+
+/* Diagnostics of a built slot: lambda_max per smoothed level and the coarsest live DOF count. */
+int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs);
+
+/* SIMP interpolation E = E0 (eps + (1 - eps) rho^p), dE — topology_optimization.py:657-661;
+ * inactive cells (active_dev[e] == 0) get 0. */
+int dfmi_simp(const double* rho_dev, const uint8_t* active_dev, int64_t n, double young, double penal, double eps,
+              double* moduli_dev, double* dmoduli_dev, void* hip_stream);
+
+/* ConeFilter / DensityMap.filtered / pullback — topology_optimization.py:1082-1092, 1135-1142,
+ * 1171-1178.  allowed/preserve: uint8 per element (host). */
+int dfmi_filter_create(const int32_t shape[3], const double spacing[3], double radius, const uint8_t* allowed,
+                       const uint8_t* preserve, dfmi_filter** out);
+int dfmi_filter_destroy(dfmi_filter* f);
+int dfmi_filter_set_stream(dfmi_filter* f, void* hip_stream);
+int dfmi_filter_forward(dfmi_filter* f, const double* x_dev, double* out_dev);
+int dfmi_filter_pullback(dfmi_filter* f, const double* sens_dev, const double* dproj_dev, double* out_dev);
+/* Sum of the filtered field over elements with weights (NULL = 1): the volume of DensityMap.volume
+ * with the identity projection (topology_optimization.py:1148-1151). */
+int dfmi_filter_volume(dfmi_filter* f, const double* x_dev, double* volume);
+
+/* _oc_update — topology_optimization.py:1197-1233 (volume through the filter). */
+int dfmi_oc_update(dfmi_filter* f, const double* x_dev, const double* dc_dev, const double* dv_dev, double target,
+                   double xmin, double move, double* out_dev);
+
+/* One MMA step (Svanberg's method as called by deep_frame/topology_optimizers.py:55-67), m = 1:
+ * state arrays low/upp/xold1/xold2 (n, device) are updated in place; x_dev receives the new design. */
+int dfmi_mma_step(int64_t n, int32_t iteration, double* x_dev, double* xold1_dev, double* xold2_dev,
+                  double* low_dev, double* upp_dev, double f0, const double* df0_dev, double g,
+                  const double* dg_dev, double move, double scale, void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFMI_H */

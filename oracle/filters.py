@@ -1,0 +1,68 @@
+"""Cone density filter, pullback and Heaviside projection (FP64 oracle).
+
+Restates ``deep_frame/topology_optimization.py``:
+  * cone weights max(r - |o h|, 0) over |o_a| <= floor(r / h_a) (:1082-1092, :1094-1105),
+  * ``DensityMap.filtered`` x~ = (W (x a)) a / sums, sums = (W a) a, 1 on
+    forbidden cells (:1123, :1130, :1135-1142),
+  * ``DensityMap.project`` single tanh projection (:1152-1170),
+  * ``DensityMap.pullback`` (:1171-1178).
+Implemented as an explicit shifted-sum over the offset box (not a convolution
+library call).  Test infrastructure only.
+"""
+import numpy as np
+
+
+def cone_weights(spacing, radius):
+    pad = [int(np.floor(radius / h)) for h in spacing]
+    offs = np.stack(np.meshgrid(*[np.arange(-k, k + 1) for k in pad], indexing="ij"), -1).reshape(-1, 3)
+    dist = np.sqrt(((offs * np.asarray(spacing, float)) ** 2).sum(1))
+    w = np.maximum(radius - dist, 0.0)
+    keep = w > 0
+    return offs[keep], w[keep]
+
+
+def _neighbour_sum(field, offs, w):
+    shape = field.shape
+    out = np.zeros(shape)
+    for (dx, dy, dz), wt in zip(offs, w):
+        src = [slice(max(0, d), n + min(0, d)) for d, n in zip((dx, dy, dz), shape)]
+        dst = [slice(max(0, -d), n + min(0, -d)) for d, n in zip((dx, dy, dz), shape)]
+        out[tuple(dst)] += wt * field[tuple(src)]
+    return out
+
+
+class Cone:
+    def __init__(self, allowed, spacing, radius, preserve=None):
+        self.allowed = np.asarray(allowed, bool)
+        self.shape = self.allowed.shape
+        self.preserve = np.zeros(self.shape, bool) if preserve is None else np.asarray(preserve, bool)
+        self.free = self.allowed & ~self.preserve
+        self.offs, self.w = cone_weights(spacing, radius)
+        a = self.allowed.astype(float)
+        self.sums = _neighbour_sum(a, self.offs, self.w) * a
+        self.sums[~self.allowed] = 1.0
+
+    def apply(self, x):
+        a = self.allowed.astype(float)
+        return _neighbour_sum(np.asarray(x, float).reshape(self.shape) * a, self.offs, self.w) * a
+
+    def filtered(self, design):
+        return (self.apply(design) / self.sums).ravel()
+
+    def pullback(self, sensitivity, dproj=None):
+        s = np.asarray(sensitivity, float).reshape(self.shape)
+        if dproj is not None:
+            s = s * np.asarray(dproj, float).reshape(self.shape)
+        out = self.apply(s / self.sums)
+        out[~self.free] = 0.0
+        return out.ravel()
+
+
+def project(filtered, beta, eta=0.5):
+    """(physical, d physical / d filtered) for the single tanh projection."""
+    x = np.asarray(filtered, float)
+    if not beta:
+        return x.copy(), np.ones_like(x)
+    den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
+    t = np.tanh(beta * (x - eta))
+    return np.clip((np.tanh(beta * eta) + t) / den, 0, 1), beta * (1 - t ** 2) / den

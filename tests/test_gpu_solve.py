@@ -1,0 +1,125 @@
+"""GMG-PCG solve parity on the GPU against the oracle's SuperLU solve and the
+reference's golden vectors (mirrors tests/test_topology_multigrid.py:55-86 and
+tests/test_topology_optimization.py:56-68 of the reference)."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from deep_frame_amd import element, problems
+from oracle import hex8, loads, system
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def make(shape, spacing, ke, moduli, settings=None):
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    mg = GeometricMultigrid(shape, spacing, ke, {"coarsest_dofs": 200, **(settings or {})})
+    mg.update(moduli)
+    return mg
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_euler_bernoulli_tip_deflection(precision, torch_gpu):
+    """Documented known answer 0.0730389 mm (docs/topology_optimization.md, Numerische Pruefungen)."""
+    print(f"eb precision={precision}", flush=True)
+    shape, h = (24, 4, 4), (2.0, 1.5, 1.5)
+    ke = element.stiffness(h, element.isotropic(0.3))
+    pts = element.node_points(shape, h)
+    fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    tip = np.flatnonzero(np.abs(pts[:, 0] - 48.0) < 1e-9)
+    f = np.zeros((3 * len(pts), 1))
+    f[3 * tip + 2, 0] = -1.0 / len(tip)
+    mg = make(shape, h, ke, np.full(96 * 4, 4430.0), {"precision": precision})
+    u, rep = mg.solve(b"eb", fixed, f)
+    assert rep["converged"] and rep["relative_residual"][0] < 1e-8
+    assert -u[3 * tip + 2, 0].mean() == pytest.approx(0.0730389, abs=5e-8)
+    mg.close()
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_pcg_matches_direct_solution(precision, torch_gpu):
+    print(f"pcg-vs-direct precision={precision}", flush=True)
+    shape = (16, 8, 8)
+    ke, _, _ = hex8.hex8((1, 1, 1), hex8.isotropic_d(0.3))
+    rho = np.clip(np.random.default_rng(0).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
+    mod = 1e-6 + (1 - 1e-6) * rho ** 3
+    pts, conn, dofs = hex8.grid(shape)
+    fixed = problems.clamp_dofs(np.flatnonzero(pts[:, 0] == 0))
+    f = np.random.default_rng(1).normal(size=(3 * len(pts), 3))
+    f[fixed] = 0
+    ref, _ = system.solve(dofs, conn, ke, mod, fixed, f, len(pts))
+    mg = make(shape, (1, 1, 1), ke, mod, {"precision": precision})
+    u, rep = mg.solve(b"c", fixed, f)
+    assert rep["converged"]
+    for c in range(3):
+        assert np.linalg.norm(u[:, c] - ref[:, c]) < 1e-6 * np.linalg.norm(ref[:, c])
+    mg.close()
+
+
+def test_covariance_cantilever_dense_flexibility(torch_gpu):
+    """Golden vector docs/validation/formulation_covariance_cantilever.json: F = P^T K^-1 P."""
+    gold = json.loads((GOLDEN / "covariance_cantilever.json").read_text())
+    d = hex8.orthotropic_d(gold["orthotropic"]) / gold["orthotropic"]["e_xy_mpa"]
+    ke, _, _ = hex8.hex8((1, 1, 1), d)
+    shape = (32, 12, 12)
+    pts, conn, dofs = hex8.grid(shape, (1, 1, 1), (0, -6, 0))
+    fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    tol = 1e-6
+    tip = hex8.select_nodes(pts, [32 - tol, -6 - tol, -tol], [32 + tol, 6 + tol, 12 + tol])
+    mid = hex8.select_nodes(pts, [15 - tol, -6 - tol, 12 - tol], [17 + tol, 6 + tol, 12 + tol])
+    nd = 3 * len(pts)
+    wt, wm = loads.wrench_columns(pts, tip, [32, 0, 6], nd), loads.wrench_columns(pts, mid, [16, 0, 12], nd)
+    p = np.column_stack([wt[:, 2], wm[:, 2], wt[:, 1]])
+    dens = np.random.default_rng(11).uniform(0.3, 1.0, (32, 6, 12))
+    rho = np.concatenate([dens[:, ::-1, :], dens], axis=1).ravel()
+    mod = 4430.0 * (1e-6 + (1 - 1e-6) * rho ** 3)
+    mg = make(shape, (1, 1, 1), ke, mod, {"precision": "float32"})
+    u, rep = mg.solve(b"cov", fixed, p)
+    assert rep["converged"]
+    flex = p.T @ u
+    expected = np.asarray(gold["dense_flexibility"])
+    diag = np.diag(expected)
+    assert np.allclose(np.diag(flex), diag, rtol=1e-7)
+    assert abs(flex[0, 1] - expected[0, 1]) < 1e-7 * expected[0, 1]
+    assert np.max(np.abs(flex[2, :2])) < 1e-9 * diag.max()
+    mg.close()
+
+
+def test_cantilever_config_compliance_and_sensitivity(torch_gpu):
+    """48 x 24 x 24 cantilever: compliance 1e-6 and sensitivity 1e-5 against the FP64 oracle."""
+    prob = problems.cantilever(seed=0)
+    shape = prob["shape"]
+    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
+    mod, dmod = problems.moduli(prob["density"])
+    pts, conn, dofs = hex8.grid(shape)
+    ref_u, _ = system.solve(dofs, conn, ke, mod, prob["fixed"], prob["forces"], len(pts))
+    ref_c, ref_s = system.compliance_sensitivity(dofs, ke, ref_u, prob["forces"], dmod)
+    torch = torch_gpu
+    mg = make(shape, (1, 1, 1), ke, mod, {"coarsest_dofs": 5000})
+    rhs = torch.as_tensor(prob["forces"].T.copy(), device="cuda")
+    x, reps = mg.solve_device(b"c1", prob["fixed"], rhs)
+    assert all(r[2] and r[1] < 1e-8 for r in reps)
+    comp, sens = mg.compliance_sensitivity_device(x, rhs, torch.as_tensor(dmod, device="cuda"))
+    assert comp.item() == pytest.approx(ref_c[0], rel=1e-6)
+    s = sens.cpu().numpy()
+    assert np.linalg.norm(s - ref_s[:, 0]) <= 1e-5 * np.linalg.norm(ref_s[:, 0])
+    mg.close()
+
+
+def test_zero_load_columns_skip_pcg(torch_gpu):
+    shape = (8, 4, 4)
+    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
+    pts = element.node_points(shape)
+    fixed = problems.clamp_dofs(np.flatnonzero(pts[:, 0] == 0))
+    f = np.zeros((3 * len(pts), 2))
+    f[3 * (len(pts) - 1) + 2, 0] = -1.0
+    f[:, 1] = 1e-20 * f[:, 0]
+    mg = make(shape, (1, 1, 1), ke, np.ones(128), {"precision": "float64", "tolerance": 1e-12})
+    flat, rep = mg.solve(b"z", fixed, f)
+    single, _ = mg.solve(b"z", fixed, f[:, :1])
+    assert rep["converged"] and rep["zero_rhs_columns"] == 1 and rep["iterations"][1] == 0
+    assert not np.any(flat[:, 1]) and np.linalg.norm(flat[:, 0] - single[:, 0]) <= 1e-12 * np.linalg.norm(single[:, 0])
+    mg.close()
