@@ -27,6 +27,23 @@ static const int kRefToBits[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   void ensure(size_t n) {
     if (n <= bytes) return;
     release();
@@ -795,8 +812,6 @@ int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x, do
       a.x = xf.as<float>();
       a.y = yf.as<float>();
       fine_apply<float>(h.pad, h.m32.as<float>(), mask, h.kb, MODE_APPLY, a, s, nullptr);
-      const double one = 1.0;
-      DFMI_HIP(hipMemcpyAsync(h.scal.as<double>(), &one, sizeof(double), hipMemcpyHostToDevice, s));
       std::vector<double> ones(k, 1.0);
       DFMI_HIP(hipMemcpyAsync(h.scal.as<double>(), ones.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
       from_work<float>(yf.as<float>(), h.scal.as<double>(), h.tmp.as<double>(), n, k, nullptr, s);
@@ -849,6 +864,40 @@ int dfmi_level_apply(dfmi_mg* mg, int32_t slot, int32_t level, const double* x, 
       fine_apply<double>(L.g, h.m64.as<double>(), L.mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream, nullptr);
     else
       stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, h.stream, nullptr);
+    sync(&h);
+  })
+}
+
+int dfmi_precondition(dfmi_mg* mg, int32_t slot, const double* r, double* z, int32_t k) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    Hierarchy& H = h.slots[slot];
+    ensure_workspace(&h, k);
+    std::vector<int> ones(k, 1);
+    DFMI_HIP(hipMemcpyAsync(h.active.p, ones.data(), sizeof(int) * k, hipMemcpyHostToDevice, h.stream));
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, r, h.tmp.as<double>(), k, h.stream);
+    precond_any(&h, H, h.tmp.as<double>(), h.zn.as<double>(), k, h.active.as<int>());
+    grid_to_flat(h.pad, h.ux, h.uy, h.uz, h.zn.as<double>(), z, k, h.stream);
+    sync(&h);
+  })
+}
+
+int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, const double* in, double* out,
+                  int32_t k) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    Hierarchy& H = h.slots[slot];
+    if (level < 0 || level + 1 >= (int)H.levels.size()) throw std::invalid_argument("level out of range");
+    Level& F = *H.levels[level];
+    Level& C = *H.levels[level + 1];
+    if (direction == 0) {
+      restrict_full<double>(F.g, in, C.g, C.mask.as<uint8_t>(), out, k, nullptr, h.stream);
+    } else {
+      fill_zero<double>(out, (long long)k * col(F.g), h.stream);
+      prolong_add<double>(C.g, in, F.g, F.mask.as<uint8_t>(), out, k, nullptr, h.stream);
+    }
     sync(&h);
   })
 }

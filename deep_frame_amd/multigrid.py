@@ -171,6 +171,24 @@ class GeometricMultigrid:
                      _native.ptr(y), x.shape[0])
         return y
 
+    def precondition_device(self, key, r):
+        """One V-cycle z = M^-1 r on (k, ndof) FP64 device columns."""
+        r = r.to(device=self.device, dtype=self.torch.float64).contiguous()
+        z = self.torch.empty_like(r)
+        self._stream()
+        _native.call("dfmi_precondition", self.handle, self.hierarchies[key]["slot"], _native.ptr(r), _native.ptr(z),
+                     r.shape[0])
+        return z
+
+    def transfer(self, key, level, direction, x, out_size):
+        """Restriction (direction 0) or prolongation (1) between padded levels ``level`` and ``level + 1``."""
+        x = x.to(device=self.device, dtype=self.torch.float64).contiguous()
+        y = self.torch.empty((x.shape[0], out_size), dtype=self.torch.float64, device=self.device)
+        self._stream()
+        _native.call("dfmi_transfer", self.handle, self.hierarchies[key]["slot"], level, direction, _native.ptr(x),
+                     _native.ptr(y), x.shape[0])
+        return y
+
     def product(self, flat):
         """FP64 K u for host columns (ndof, k) (topology_multigrid.py:455-457)."""
         flat = np.asarray(flat, dtype=float).reshape(self.ndof, -1)

@@ -99,6 +99,16 @@ int dfmi_compliance_sensitivity(dfmi_mg* mg, const double* u_dev, const double* 
  * (deep_frame/topology_multigrid.py:110-117).  For tests and diagnostics. */
 int dfmi_level_apply(dfmi_mg* mg, int32_t slot, int32_t level, const double* x_dev, double* y_dev, int32_t k);
 
+/* One application of the V-cycle preconditioner (topology_multigrid.py:318-324) to k flat FP64
+ * columns, in the handle's V-cycle precision.  For tests and diagnostics. */
+int dfmi_precondition(dfmi_mg* mg, int32_t slot, const double* r_dev, double* z_dev, int32_t k);
+
+/* Transfer between level `level` and level + 1 of a built slot on padded node grids:
+ * direction 0 = full-weighting restriction masked by the coarse mask, 1 = trilinear prolongation
+ * masked by the fine mask (topology_multigrid.py:314-316).  For tests and diagnostics. */
+int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, const double* in_dev,
+                  double* out_dev, int32_t k);
+
 /* Diagnostics of a built slot: lambda_max per smoothed level and the coarsest live DOF count. */
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs);
 

@@ -19,7 +19,7 @@ import numpy as np
 
 
 def asymptotes(it, x, xold1, xold2, low, upp, xmin, xmax, asyinit=0.5, asydecr=0.7, asyincr=1.2):
-    span = xmax - xmin
+    span = np.broadcast_to(np.asarray(xmax - xmin, float), x.shape)
     if it <= 2:
         return x - asyinit * span, x + asyinit * span
     sign = (x - xold1) * (xold1 - xold2)
@@ -36,7 +36,7 @@ def approximation(x, low, upp, xmin, xmax, df0, g, dg, move, raa0=1e-5, albefa=0
     span = xmax - xmin
     alfa = np.maximum(np.maximum(low + albefa * (x - low), x - move * span), xmin)
     beta = np.minimum(np.minimum(upp - albefa * (upp - x), x + move * span), xmax)
-    inv = 1.0 / np.maximum(span, 1e-5)
+    inv = np.broadcast_to(1.0 / np.maximum(span, 1e-5), x.shape)
     ux2, xl2 = (upp - x) ** 2, (x - low) ** 2
     p0, q0 = np.maximum(df0, 0), np.maximum(-df0, 0)
     pq0 = 0.001 * (p0 + q0) + raa0 * inv

@@ -34,6 +34,7 @@ def test_euler_bernoulli_tip_deflection(precision, torch_gpu):
     f[3 * tip + 2, 0] = -1.0 / len(tip)
     mg = make(shape, h, ke, np.full(96 * 4, 4430.0), {"precision": precision})
     u, rep = mg.solve(b"eb", fixed, f)
+    print(rep["iterations"], rep["relative_residual"], flush=True)
     assert rep["converged"] and rep["relative_residual"][0] < 1e-8
     assert -u[3 * tip + 2, 0].mean() == pytest.approx(0.0730389, abs=5e-8)
     mg.close()
@@ -53,6 +54,7 @@ def test_pcg_matches_direct_solution(precision, torch_gpu):
     ref, _ = system.solve(dofs, conn, ke, mod, fixed, f, len(pts))
     mg = make(shape, (1, 1, 1), ke, mod, {"precision": precision})
     u, rep = mg.solve(b"c", fixed, f)
+    print(rep["iterations"], rep["relative_residual"], flush=True)
     assert rep["converged"]
     for c in range(3):
         assert np.linalg.norm(u[:, c] - ref[:, c]) < 1e-6 * np.linalg.norm(ref[:, c])

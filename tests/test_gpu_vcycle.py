@@ -1,0 +1,53 @@
+"""V-cycle pieces against the oracle: transfer operators (P, P^T) and one full
+V-cycle (deep_frame/topology_multigrid.py:284-324) with the GPU's lambda_max."""
+import numpy as np
+import pytest
+
+from deep_frame_amd import element, problems
+from oracle import gmg
+
+pytestmark = pytest.mark.gpu
+
+
+def case(shape, spacing, seed, precision, torch):
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    ke = element.stiffness(spacing, element.isotropic(0.3))
+    rho = np.clip(np.random.default_rng(seed).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
+    mod = 1e-6 + (1 - 1e-6) * rho ** 3
+    pts = element.node_points(shape, spacing)
+    fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    mg = GeometricMultigrid(shape, spacing, ke, {"coarsest_dofs": 200, "precision": precision})
+    mg.update(mod)
+    entry = mg.hierarchy("v", fixed)
+    levels = gmg.hierarchy(shape, ke, mod, fixed, gmg.level_count(shape, 200))
+    return mg, entry, levels, fixed
+
+
+@pytest.mark.parametrize("shape", [(24, 4, 4), (16, 8, 8)])
+def test_transfer_operators_match_prolongation(shape, torch_gpu):
+    print(f"transfer shape={shape}", flush=True)
+    mg, entry, levels, _ = case(shape, (1.0, 1.0, 1.0), 0, "float64", torch_gpu)
+    for index in range(len(levels) - 1):
+        fine, coarse = levels[index], levels[index + 1]
+        p = coarse["P"]
+        vf = np.random.default_rng(index).normal(size=(p.shape[0], 2)) * fine["mask"][:, None]
+        vc = np.random.default_rng(10 + index).normal(size=(p.shape[1], 2)) * coarse["mask"][:, None]
+        down = mg.transfer("v", index, 0, torch_gpu.as_tensor(vf.T.copy(), device="cuda"), p.shape[1]).cpu().numpy().T
+        up = mg.transfer("v", index, 1, torch_gpu.as_tensor(vc.T.copy(), device="cuda"), p.shape[0]).cpu().numpy().T
+        assert np.allclose(down, (p.T @ vf) * coarse["mask"][:, None], rtol=0, atol=1e-12 * np.abs(vf).max() * 8)
+        assert np.allclose(up, (p @ vc) * fine["mask"][:, None], rtol=0, atol=1e-12 * np.abs(vc).max())
+    mg.close()
+
+
+@pytest.mark.parametrize("shape,spacing", [((24, 4, 4), (2.0, 1.5, 1.5)), ((16, 8, 8), (1.0, 1.0, 1.0))])
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_vcycle_matches_oracle(shape, spacing, precision, torch_gpu):
+    print(f"vcycle shape={shape} precision={precision}", flush=True)
+    mg, entry, levels, fixed = case(shape, spacing, 1, precision, torch_gpu)
+    r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], 2)) * levels[0]["mask"][:, None]
+    ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c]) for c in range(2)])
+    z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
+    tol = 1e-9 if precision == "float64" else 1e-4
+    for c in range(2):
+        assert np.linalg.norm(z[:, c] - ref[:, c]) < tol * np.linalg.norm(ref[:, c]), c
+    mg.close()

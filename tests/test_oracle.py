@@ -1,0 +1,218 @@
+"""The oracle pinned against the reference's golden vectors, known answers and
+closed forms (CPU only).  Mirrors the reference's tests/test_topology_optimization.py:37-68
+(Hex8 rigid modes, patch energy, Euler-Bernoulli) and the covariance golden vector."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.sparse.linalg import spsolve
+
+from deep_frame_amd import element, problems
+from oracle import filters, gmg, hex8, loads, mma, oc, system
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_hex8_rigid_modes_patch_energy_and_mass():
+    spacing = (2.0, 3.0, 4.0)
+    ke, me, b = hex8.hex8(spacing, hex8.isotropic_d(0.3))
+    ev = np.linalg.eigvalsh(ke)
+    assert np.max(np.abs(ev[:6])) < 1e-12 and ev[6] > 0.01
+    pts, conn, _ = hex8.grid((1, 1, 1), spacing)
+    t = np.array([[0.01, 0.02, -0.01], [0.03, -0.02, 0.01], [0.04, 0.02, 0.03]])
+    u = (pts[conn[0]] @ t.T).ravel()
+    strain = np.array([0.01, -0.02, 0.03, 0.05, 0.03, 0.03])
+    assert u @ ke @ u == pytest.approx(np.prod(spacing) * strain @ hex8.isotropic_d(0.3) @ strain, rel=1e-12)
+    assert np.allclose(b @ u, strain)
+    for axis in range(3):
+        r = np.zeros(24)
+        r[axis::3] = 1
+        assert r @ me @ r == pytest.approx(np.prod(spacing), rel=1e-12)
+
+
+def test_product_element_matches_oracle_element():
+    """The package's host Ke (einsum form) equals the oracle's loop form."""
+    for spacing in [(1, 1, 1), (2.0, 1.5, 1.5), (0.747, 0.753, 0.667)]:
+        for d in [hex8.isotropic_d(0.3), hex8.orthotropic_d(json.loads((GOLDEN / "covariance_cantilever.json").read_text())["orthotropic"])]:
+            a = element.stiffness(spacing, d)
+            b, _, _ = hex8.hex8(spacing, d)
+            assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(b))
+
+
+def test_euler_bernoulli_known_answer():
+    gold = json.loads((GOLDEN / "euler_bernoulli.json").read_text())
+    shape, h = tuple(gold["shape"]), tuple(gold["spacing_mm"])
+    ke, _, _ = hex8.hex8(h, hex8.isotropic_d(gold["poisson"]))
+    pts, conn, dofs = hex8.grid(shape, h)
+    fixn = hex8.select_nodes(pts, [-0.01, -0.01, -0.01], [0.01, 6.01, 6.01])
+    tip = hex8.select_nodes(pts, [47.99, -0.01, -0.01], [48.01, 6.01, 6.01])
+    f = np.zeros(3 * len(pts))
+    f[3 * tip + 2] = gold["force_n"] / len(tip)
+    mod, _ = hex8.simp(np.ones(int(np.prod(shape))), gold["young_mpa"])
+    u, res = system.solve(dofs, conn, ke, mod, problems.clamp_dofs(fixn), f, len(pts))
+    assert res[0] < 1e-10
+    assert -u[3 * tip + 2, 0].mean() == pytest.approx(gold["tip_deflection_mm"], abs=5e-8)
+
+
+def test_covariance_cantilever_golden_flexibility():
+    gold = json.loads((GOLDEN / "covariance_cantilever.json").read_text())
+    d = hex8.orthotropic_d(gold["orthotropic"]) / gold["orthotropic"]["e_xy_mpa"]
+    ke, _, _ = hex8.hex8((1, 1, 1), d)
+    pts, conn, dofs = hex8.grid(gold["shape_full"], (1, 1, 1), gold["origin_mm"])
+    tol = 1e-6
+    fixn = hex8.select_nodes(pts, [-tol, -6 - tol, -tol], [tol, 6 + tol, 12 + tol])
+    cols = {}
+    for name, spec in gold["interfaces"].items():
+        lo, hi = np.asarray(spec["box"][0]) - tol, np.asarray(spec["box"][1]) + tol
+        cols[name] = loads.wrench_columns(pts, hex8.select_nodes(pts, lo, hi), spec["reference_mm"], 3 * len(pts))
+    index = {"Fx": 0, "Fy": 1, "Fz": 2}
+    p = np.column_stack([cols[name][:, index[dof]] for name, dof in gold["loads"]])
+    dens = np.random.default_rng(gold["density_seed"]).uniform(0.3, 1.0, (32, 6, 12))
+    rho = np.concatenate([dens[:, ::-1, :], dens], axis=1).ravel()
+    mod, _ = hex8.simp(rho, gold["orthotropic"]["e_xy_mpa"], 3.0, gold["simp"]["eps"])
+    u, res = system.solve(dofs, conn, ke, mod, problems.clamp_dofs(fixn), p, len(pts))
+    flex = p.T @ u
+    expected = np.asarray(gold["dense_flexibility"])
+    assert np.max(np.abs(flex - expected)) < 1e-11 * np.max(np.abs(expected))
+    sigma = np.asarray(gold["sigma"])
+    assert np.trace(flex @ sigma) == pytest.approx(gold["dense_full_model"]["load_mean"], rel=1e-11)
+
+
+def test_unit_wrenches_reproduce_the_wrench_with_minimum_norm():
+    pts = element.node_points((6, 5, 4), (1.0, 1.3, 0.7))
+    nodes = np.arange(10, 40, 3)
+    ref = np.array([2.0, 1.0, 1.5])
+    blocks = loads.unit_wrenches(pts, nodes, ref)
+    arms = pts[nodes] - ref
+    for j in range(6):
+        f = blocks[j].reshape(-1, 3)
+        w = np.concatenate([f.sum(0), np.cross(arms, f).sum(0)])
+        assert np.allclose(w, np.eye(6)[j], atol=1e-12)
+    # minimum norm: orthogonal to the null space of the wrench map
+    b = np.vstack([np.hstack([np.eye(3)] * len(nodes)),
+                   np.hstack([[[0, -z, y], [z, 0, -x], [-y, x, 0]] for x, y, z in arms])])
+    null = np.linalg.svd(b)[2][6:]
+    assert np.max(np.abs(null @ blocks.T)) < 1e-12
+    # the package builds the same loads independently
+    assert np.allclose(problems.min_norm_wrenches(pts, nodes, ref).T, blocks, atol=1e-12)
+
+
+def test_gmg_restriction_is_prolongation_transpose_and_galerkin_is_rap():
+    shape = (4, 4, 4)
+    ke, _, _ = hex8.hex8((1, 1, 1), hex8.isotropic_d(0.3))
+    mod = np.random.default_rng(0).uniform(0.1, 1.0, 64)
+    lv = gmg.hierarchy(shape, ke, mod, np.arange(3 * 25), (2, shape))
+    p = lv[1]["P"].toarray()
+    # brute force: trilinear interpolation of each coarse basis vector evaluated at fine nodes
+    fine = element.node_points(shape)
+    coarse = element.node_points(tuple(n // 2 for n in shape), (2.0, 2.0, 2.0))
+    for ci, cp in enumerate(coarse):
+        w = np.prod(np.clip(1 - np.abs(fine - cp) / 2.0, 0, None), axis=1)
+        assert np.allclose(p[0::3, 3 * ci], w)
+    a = lv[0]["A"].toarray()
+    ac = p.T @ a @ p
+    m = lv[1]["mask"]
+    assert np.allclose(lv[1]["A"].toarray(), (ac + ac.T) / 2 * np.outer(m, m), atol=1e-12 * np.abs(ac).max())
+
+
+def test_gmg_pcg_matches_direct_solve():
+    shape = (8, 4, 4)
+    ke, _, _ = hex8.hex8((1, 1, 1), hex8.isotropic_d(0.3))
+    rho = np.random.default_rng(1).uniform(0.3, 1.0, 128)
+    mod = 1e-6 + (1 - 1e-6) * rho ** 3
+    pts, conn, dofs = hex8.grid(shape)
+    fixed = problems.clamp_dofs(np.flatnonzero(pts[:, 0] == 0))
+    f = np.zeros(3 * len(pts))
+    f[-1] = -1.0
+    levels = gmg.hierarchy(shape, ke, mod, fixed, gmg.level_count(shape, 100))
+    lam = [gmg.spectrum(l) for l in levels[:-1]]
+    x, it, rel = gmg.pcg(levels, lam, f)
+    ref, _ = system.solve(dofs, conn, ke, mod, fixed, f, len(pts))
+    assert rel < 1e-8 and it < 40
+    assert np.linalg.norm(x - ref[:, 0]) < 1e-7 * np.linalg.norm(ref)
+
+
+def test_sensitivity_matches_central_difference():
+    shape = (4, 3, 3)
+    ke, _, _ = hex8.hex8((1, 1, 1), hex8.isotropic_d(0.3))
+    pts, conn, dofs = hex8.grid(shape)
+    fixed = problems.clamp_dofs(np.flatnonzero(pts[:, 0] == 0))
+    f = np.zeros(3 * len(pts))
+    f[3 * (len(pts) - 1) + 2] = -1
+    rho = np.random.default_rng(2).uniform(0.3, 0.9, 36)
+
+    def compliance(r):
+        mod, dmod = hex8.simp(r, 1.0, 3.0, 1e-9)
+        u, _ = system.solve(dofs, conn, ke, mod, fixed, f, len(pts))
+        c, s = system.compliance_sensitivity(dofs, ke, u, f, dmod)
+        return c[0], s[:, 0]
+
+    _, sens = compliance(rho)
+    d = np.random.default_rng(3).normal(size=36)
+    h = 1e-6
+    fd = (compliance(rho + h * d)[0] - compliance(rho - h * d)[0]) / (2 * h)
+    assert sens @ d == pytest.approx(fd, rel=1e-6)
+
+
+def test_cone_filter_brute_force_and_adjoint():
+    shape = (7, 5, 4)
+    rng = np.random.default_rng(4)
+    allowed = rng.uniform(size=shape) > 0.2
+    preserve = np.zeros(shape, bool)
+    preserve[0, 0, 0] = allowed[0, 0, 0] = True
+    cone = filters.Cone(allowed, (1.0, 1.0, 1.0), 1.5, preserve)
+    x = rng.uniform(size=allowed.size)
+    out = cone.filtered(x).reshape(shape)
+    idx = np.argwhere(np.ones(shape, bool))
+    for e in idx[::7]:
+        if not allowed[tuple(e)]:
+            assert out[tuple(e)] == 0
+            continue
+        num = den = 0.0
+        for o in idx:
+            w = max(1.5 - np.linalg.norm(o - e), 0.0)
+            if w > 0 and allowed[tuple(o)]:
+                num += w * x.reshape(shape)[tuple(o)]
+                den += w
+        assert out[tuple(e)] == pytest.approx(num / den, rel=1e-13)
+    # pullback is the adjoint of the filter on free cells
+    s = rng.normal(size=allowed.size)
+    dx = rng.normal(size=allowed.size) * cone.free.ravel()
+    assert s @ cone.filtered(dx) == pytest.approx(cone.pullback(s) @ dx, rel=1e-12)
+
+
+def test_oc_update_hits_volume_and_respects_move_limits():
+    shape = (6, 4, 3)
+    allowed = np.ones(shape, bool)
+    cone = filters.Cone(allowed, (1, 1, 1), 1.5)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.3, 0.7, allowed.size)
+    dc = -rng.uniform(0.1, 2.0, allowed.size)
+    dv = cone.pullback(np.ones(allowed.size))
+    free = cone.free.ravel()
+    vol = lambda d: cone.filtered(d).sum()
+    target = 0.45 * allowed.size
+    new = oc.oc_update(x, dc, dv, free, vol, target, 1e-3, 0.2)
+    assert vol(new) <= target + 1e-9 and vol(new) == pytest.approx(target, rel=1e-6)
+    assert np.all(new >= np.maximum(1e-3, x - 0.2) - 1e-15) and np.all(new <= np.minimum(1, x + 0.2) + 1e-15)
+
+
+def test_mma_subproblem_kkt_and_dual_agree():
+    rng = np.random.default_rng(6)
+    n = 40
+    x = rng.uniform(0.2, 0.8, n)
+    df0 = rng.uniform(-1.0, 0.3, n)
+    dg = rng.uniform(0.5, 1.5, (1, n)) / n
+    g = np.array([x @ dg[0] - 0.4])
+    xn, low, upp, lam, (alfa, beta, p0, q0, p, q, b) = mma.mma_step(1, x, x, x, np.zeros(n), np.ones(n), 0.0, 1.0,
+                                                                    1.0, df0, g, dg, move=0.5)
+    xd, lamd = mma.dual_subsolve(low, upp, alfa, beta, p0, q0, p, q, b, np.array([1e4]), np.array([1.0]))
+    assert np.max(np.abs(xn - xd)) < 1e-6
+    assert np.all(xn >= alfa - 1e-12) and np.all(xn <= beta + 1e-12)
+    # KKT: stationarity of the separable Lagrangian on interior variables
+    grad = (p0 + lam[0] * p[0]) / (upp - xn) ** 2 - (q0 + lam[0] * q[0]) / (xn - low) ** 2
+    inner = (xn > alfa + 1e-6) & (xn < beta - 1e-6)
+    assert np.max(np.abs(grad[inner])) < 1e-5 * np.max(np.abs(p0 / (upp - xn) ** 2))
+    # first two iterations place the asymptotes at x -+ 0.5 (asyinit)
+    assert np.allclose(low, x - 0.5) and np.allclose(upp, x + 0.5)
