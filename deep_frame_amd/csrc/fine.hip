@@ -110,6 +110,15 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
   for (int i = i0 - 1; i < i1; ++i) {
     load_plane(i + 1, U + ((i + 1) & 1) * plane * 3);
     __syncthreads();
+    // own operand value of node plane i, taken now: the next step's plane load reuses buffer i & 1
+    // before the epilogue of this step has finished on every thread
+    T xs[3] = {T(0), T(0), T(0)};
+    if (own) {
+      const T* src = U + (i & 1) * plane * 3 + (ty * (TZ + 1) + tz) * 3;
+      xs[0] = src[0];
+      xs[1] = src[1];
+      xs[2] = src[2];
+    }
     T y[8][3];
     if (elem_in && i >= 0 && i < g.ex) {
       const T e = moduli[((long long)i * g.ey + ej) * g.ez + el];
@@ -167,7 +176,6 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
       if (i >= i0) {
         const long long n = ((long long)i * g.ny + ej) * g.nz + el;
         const uint8_t mk = mask[n];
-        const T* xs = U + (i & 1) * plane * 3 + (ty * (TZ + 1) + tz) * 3;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const bool on = bit_live(mk, c);

@@ -902,6 +902,47 @@ int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, c
   })
 }
 
+int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t precision, int32_t k, int32_t reps, double* ms_per_launch) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    if (k < 1 || reps < 1) throw std::invalid_argument("k and reps must be positive");
+    Hierarchy& H = h.slots[slot];
+    const long long n = (long long)k * col(h.pad);
+    const size_t el = precision == 0 ? sizeof(double) : sizeof(float);
+    DevBuf xb, yb;
+    xb.ensure(el * n);
+    yb.ensure(el * n);
+    if (precision == 0) fill_zero<double>(xb.as<double>(), n, h.stream);
+    else fill_zero<float>(xb.as<float>(), n, h.stream);
+    ApplyArgs a{};
+    a.x = xb.p;
+    a.y = yb.p;
+    a.k = k;
+    auto launch = [&]() {
+      if (precision == 0)
+        fine_apply<double>(h.pad, h.m64.as<double>(), H.levels[0]->mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream,
+                           nullptr);
+      else
+        fine_apply<float>(h.pad, h.m32.as<float>(), H.levels[0]->mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream,
+                          nullptr);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    DFMI_HIP(hipEventCreate(&e0));
+    DFMI_HIP(hipEventCreate(&e1));
+    DFMI_HIP(hipEventRecord(e0, h.stream));
+    for (int i = 0; i < reps; ++i) launch();
+    DFMI_HIP(hipEventRecord(e1, h.stream));
+    DFMI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DFMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_launch = ms / reps;
+  })
+}
+
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs) {
   DFMI_GUARD({
     Hierarchy& H = mg->h.slots[slot];
