@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--apply-reps", type=int, default=20)
+    ap.add_argument("--apply-only", action="store_true",
+                    help="build the hierarchy once and only time the fine operator (for counter profiles)")
     return ap.parse_args()
 
 
@@ -99,6 +101,26 @@ def main():
                 times.append(dt)
         return mg, times, reports, x
 
+    if args.apply_only:
+        mg = GeometricMultigrid(cfg["shape"], (1, 1, 1), ke, {"precision": "float32", "levels": cfg["levels"]})
+        mg.update(moduli)
+        slot = mg.hierarchy("apply", fixed)["slot"]
+        for precision in (1, 0):
+            ms = c_double()
+            mg._stream()
+            _native.call("dfmi_time_apply", mg.handle, slot, precision, rhs.shape[0], args.apply_reps, byref(ms))
+            size = 4 if precision else 8
+            gbs = (2.0 * ndof * rhs.shape[0] * size + nel * size) / (ms.value * 1e-3) / 1e9
+            print(json.dumps({"kernel": "k_fine_apply", "precision": "fp32" if precision else "fp64",
+                              "rhs": rhs.shape[0], "ms_per_launch": ms.value, "algorithmic_GBs": gbs}))
+        mg.close()
+        # plain copy of a known size: calibrates the FETCH_SIZE / WRITE_SIZE counters
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": src.numel() * 4}))
+        return
     mg, times, reports, x = run("float32", args.steps, args.warmup, "mixed")
     t_mixed = float(np.mean(times))
     # dominant kernel: fine-level FP32 operator on all RHS columns, HIP events on the library stream

@@ -15,6 +15,8 @@
 // corner results go through LDS, and each thread sums the four elements of layer
 // i and carries the four of layer i+1 in registers.  Sums are in a fixed order,
 // so results are deterministic.
+#include <cmath>
+
 #include "dfmi_internal.h"
 
 namespace dfmi {
@@ -60,14 +62,80 @@ __device__ __forceinline__ void blocks_mul(const Blocks<T>& kb, const T (&v)[8][
 
 __host__ __device__ inline bool bit_live(uint8_t m, int c) { return (m >> c) & 1; }
 
-template <typename T, int MODE>
+// The eight parity blocks of an axis-aligned brick (isotropic or orthotropic, any spacing) share one
+// structural pattern: classes 0 and 7 are full symmetric 3x3 blocks, classes 1, 2, 4 are 2x2 blocks
+// (their third row and column belong to a rigid translation and vanish), classes 3, 5, 6 are a 2x2
+// block plus one diagonal entry.  33 distinct constants, 45 FMA per element and right-hand side.
+template <typename T>
+struct KeSym {
+  T k[33];
+};
+
+template <typename T>
+__device__ __forceinline__ void sym_mul(const KeSym<T>& s, const T (&v)[8][3], T (&w)[8][3]) {
+  {  // class 0: (c0, m1) (c1, m2) (c2, m4)
+    const T x = v[1][0], y = v[2][1], z = v[4][2];
+    w[1][0] = fma(s.k[0], x, fma(s.k[1], y, s.k[2] * z));
+    w[2][1] = fma(s.k[1], x, fma(s.k[3], y, s.k[4] * z));
+    w[4][2] = fma(s.k[2], x, fma(s.k[4], y, s.k[5] * z));
+  }
+  {  // class 7: (c0, m6) (c1, m5) (c2, m3)
+    const T x = v[6][0], y = v[5][1], z = v[3][2];
+    w[6][0] = fma(s.k[6], x, fma(s.k[7], y, s.k[8] * z));
+    w[5][1] = fma(s.k[7], x, fma(s.k[9], y, s.k[10] * z));
+    w[3][2] = fma(s.k[8], x, fma(s.k[10], y, s.k[11] * z));
+  }
+  {  // class 1: (c1, m3) (c2, m5)
+    const T y = v[3][1], z = v[5][2];
+    w[3][1] = fma(s.k[12], y, s.k[13] * z);
+    w[5][2] = fma(s.k[13], y, s.k[14] * z);
+  }
+  {  // class 2: (c0, m3) (c2, m6)
+    const T x = v[3][0], z = v[6][2];
+    w[3][0] = fma(s.k[15], x, s.k[16] * z);
+    w[6][2] = fma(s.k[16], x, s.k[17] * z);
+  }
+  {  // class 4: (c0, m5) (c1, m6)
+    const T x = v[5][0], y = v[6][1];
+    w[5][0] = fma(s.k[18], x, s.k[19] * y);
+    w[6][1] = fma(s.k[19], x, s.k[20] * y);
+  }
+  {  // class 3: (c0, m2) (c1, m1) block, (c2, m7) diagonal
+    const T x = v[2][0], y = v[1][1];
+    w[2][0] = fma(s.k[21], x, s.k[22] * y);
+    w[1][1] = fma(s.k[22], x, s.k[23] * y);
+    w[7][2] = s.k[24] * v[7][2];
+  }
+  {  // class 5: (c0, m4) (c2, m1) block, (c1, m7) diagonal
+    const T x = v[4][0], z = v[1][2];
+    w[4][0] = fma(s.k[25], x, s.k[26] * z);
+    w[1][2] = fma(s.k[26], x, s.k[27] * z);
+    w[7][1] = s.k[28] * v[7][1];
+  }
+  {  // class 6: (c1, m4) (c2, m2) block, (c0, m7) diagonal
+    const T y = v[4][1], z = v[2][2];
+    w[4][1] = fma(s.k[29], y, s.k[30] * z);
+    w[2][2] = fma(s.k[30], y, s.k[31] * z);
+    w[7][0] = s.k[32] * v[7][0];
+  }
+  w[0][0] = w[0][1] = w[0][2] = T(0);
+}
+
+// LDS slot per node / corner value: float4 (one ds_read_b128) or three doubles
+template <typename T>
+struct Slot {
+  static constexpr int S = sizeof(T) == 4 ? 4 : 3;
+};
+
+template <typename T, int MODE, int TZ>
 __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict__ moduli,
-                                                     const uint8_t* __restrict__ mask, Blocks<T> kb, ApplyArgs a,
-                                                     int TY, int TZ, int xchunk, int tiles_y, int tiles_z) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* U = reinterpret_cast<T*>(smem_raw);                   // [2][(TY+1)*(TZ+1)][3]
-  const int plane = (TY + 1) * (TZ + 1);
-  T* OUT = U + 2 * plane * 3;                               // [8][256][3]
+                                                     const uint8_t* __restrict__ mask, KeSym<T> ks, ApplyArgs a,
+                                                     int xchunk, int tiles_y, int tiles_z) {
+  constexpr int TY = 256 / TZ;
+  constexpr int NP = (TY + 1) * (TZ + 1);
+  constexpr int S = Slot<T>::S;
+  __shared__ __align__(16) T U[2][NP][S];     // node planes i and i+1 of the operand (masked)
+  __shared__ __align__(16) T OUT[6][256][S];  // element corner results q = 2..7
   __shared__ double red[4];
 
   const int r = blockIdx.y;
@@ -81,58 +149,98 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
   const int i0 = cx * xchunk;
   const int i1 = min(i0 + xchunk, g.nx);
   const int t = threadIdx.x;
-  const bool live = t < TY * TZ;
-  const int ty = live ? t / TZ : 0, tz = live ? t % TZ : 0;
-  const int ej = j0 - 1 + ty, el = l0 - 1 + tz;  // element coords of this thread == node coords of its output
-  const long long nodes = g.nodes();
-  const T* X = reinterpret_cast<const T*>(a.x) + (long long)r * nodes * 3;
-  T* Y = reinterpret_cast<T*>(a.y) + (long long)r * nodes * 3;
-  const bool own = live && ty >= 1 && tz >= 1 && ej < g.ny && el < g.nz;
-  const bool elem_in = live && ej >= 0 && el >= 0 && ej < g.ey && el < g.ez;
+  const int ty = t / TZ, tz = t % TZ;
+  const int ej = j0 - 1 + ty, el = l0 - 1 + tz;  // element of this thread; also the node it outputs
+  const int plane_nodes = g.ny * g.nz;
+  const long long col = (long long)g.nodes() * 3;
+  const T* X = reinterpret_cast<const T*>(a.x) + (long long)r * col;
+  T* Y = reinterpret_cast<T*>(a.y) + (long long)r * col;
+  const bool own = ty >= 1 && tz >= 1 && ej < g.ny && el < g.nz;
+  const bool elem_in = ej >= 0 && el >= 0 && ej < g.ey && el < g.ez;
 
-  auto load_plane = [&](int ip, T* dst) {
-    for (int v = t; v < plane * 3; v += 256) {
-      const int nl = v / 3, c = v - 3 * (v / 3);
-      const int ly = nl / (TZ + 1), lz = nl - ly * (TZ + 1);
-      const int gj = j0 - 1 + ly, gl = l0 - 1 + lz;
-      T val = T(0);
-      if (ip >= 0 && ip < g.nx && gj >= 0 && gj < g.ny && gl >= 0 && gl < g.nz) {
-        const long long n = ((long long)ip * g.ny + gj) * g.nz + gl;
-        if (bit_live(mask[n], c)) val = X[n * 3 + c];
+  // nodes staged by this thread: (ty, tz) and, on the high edges of the tile, the halo row/column.
+  // Fixed slots (no dynamic indexing, so the arrays stay in registers).
+  const int ly[4] = {ty, ty, TY, TY}, lz[4] = {tz, TZ, tz, TZ};
+  const bool has[4] = {true, tz == TZ - 1, ty == TY - 1, ty == TY - 1 && tz == TZ - 1};
+  int nl[4], no[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int gj = j0 - 1 + ly[k], gl = l0 - 1 + lz[k];
+    nl[k] = ly[k] * (TZ + 1) + lz[k];
+    no[k] = (has[k] && gj >= 0 && gl >= 0 && gj < g.ny && gl < g.nz) ? gj * g.nz + gl : -1;
+  }
+  T nv[4][3];
+  uint8_t nm[4];
+  auto fetch = [&](int ip) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      nv[k][0] = nv[k][1] = nv[k][2] = T(0);
+      nm[k] = 0;
+      if (no[k] >= 0 && ip >= 0 && ip < g.nx) {
+        const int n = ip * plane_nodes + no[k];
+        const uint8_t m = mask[n];
+        nm[k] = m;
+        const T* src = X + 3LL * n;
+        const T v0 = src[0], v1 = src[1], v2 = src[2];
+        nv[k][0] = (m & 1) ? v0 : T(0);
+        nv[k][1] = (m & 2) ? v1 : T(0);
+        nv[k][2] = (m & 4) ? v2 : T(0);
       }
-      dst[v] = val;
     }
+  };
+  auto stage = [&](int buf) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (has[k]) {
+        T* d = &U[buf][nl[k]][0];
+        d[0] = nv[k][0];
+        d[1] = nv[k][1];
+        d[2] = nv[k][2];
+      }
+    }
+  };
+  auto modulus = [&](int i) -> T {
+    return (elem_in && i >= 0 && i < g.ex) ? moduli[((long long)i * g.ey + ej) * g.ez + el] : T(0);
   };
 
   T carry[3] = {T(0), T(0), T(0)};
   double dot = 0.0;
-  load_plane(i0 - 1, U + ((i0 - 1) & 1) * plane * 3);
+  fetch(i0 - 1);
+  stage((i0 - 1) & 1);
+  uint8_t mask_cur = nm[0];
+  fetch(i0);
+  T e_cur = modulus(i0 - 1);
   for (int i = i0 - 1; i < i1; ++i) {
-    load_plane(i + 1, U + ((i + 1) & 1) * plane * 3);
+    stage((i + 1) & 1);
+    const uint8_t mask_next = nm[0];
+    if (i + 2 <= i1) fetch(i + 2);
+    const T e_next = modulus(i + 1);
     __syncthreads();
-    // own operand value of node plane i, taken now: the next step's plane load reuses buffer i & 1
-    // before the epilogue of this step has finished on every thread
-    T xs[3] = {T(0), T(0), T(0)};
-    if (own) {
-      const T* src = U + (i & 1) * plane * 3 + (ty * (TZ + 1) + tz) * 3;
+    T xs[3];
+    {
+      const T* src = &U[i & 1][ty * (TZ + 1) + tz][0];
       xs[0] = src[0];
       xs[1] = src[1];
       xs[2] = src[2];
     }
     T y[8][3];
-    if (elem_in && i >= 0 && i < g.ex) {
-      const T e = moduli[((long long)i * g.ey + ej) * g.ez + el];
+    if (e_cur != T(0)) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int qa = q & 1, qb = (q >> 1) & 1, qc = (q >> 2) & 1;
-        const T* src = U + ((i + qa) & 1) * plane * 3 + ((ty + qb) * (TZ + 1) + tz + qc) * 3;
+        const T* src = &U[(i + (q & 1)) & 1][(ty + ((q >> 1) & 1)) * (TZ + 1) + tz + ((q >> 2) & 1)][0];
         y[q][0] = src[0];
         y[q][1] = src[1];
         y[q][2] = src[2];
       }
       butterfly(y);
+#pragma unroll
+      for (int q = 1; q < 8; ++q) {
+        y[q][0] *= e_cur;
+        y[q][1] *= e_cur;
+        y[q][2] *= e_cur;
+      }
       T w[8][3];
-      blocks_mul(kb, y, w, e);
+      sym_mul(ks, y, w);
       butterfly(w);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -144,65 +252,83 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
 #pragma unroll
       for (int q = 0; q < 8; ++q) y[q][0] = y[q][1] = y[q][2] = T(0);
     }
-    if (live) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        T* d = OUT + (q * 256 + t) * 3;
-        d[0] = y[q][0];
-        d[1] = y[q][1];
-        d[2] = y[q][2];
+    for (int q = 2; q < 8; ++q) {
+      T* d = &OUT[q - 2][t][0];
+      d[0] = y[q][0];
+      d[1] = y[q][1];
+      d[2] = y[q][2];
+    }
+    // epilogue operands of node (i, ej, el), fetched before the barrier
+    const int nn = i * plane_nodes + ej * g.nz + el;
+    T bv[3] = {T(0), T(0), T(0)}, xpv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)};
+    if (own && i >= i0) {
+      if (MODE == MODE_RESID || MODE == MODE_CHEB) {
+        const T* bp = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * nn;
+        bv[0] = bp[0];
+        bv[1] = bp[1];
+        bv[2] = bp[2];
+      }
+      if (MODE == MODE_CHEB) {
+        if (a.xp) {
+          const T* pp = reinterpret_cast<const T*>(a.xp) + (long long)r * col + 3LL * nn;
+          xpv[0] = pp[0];
+          xpv[1] = pp[1];
+          xpv[2] = pp[2];
+        }
+        const T* dp = reinterpret_cast<const T*>(a.dinv) + 3LL * nn;
+        dv[0] = dp[0];
+        dv[1] = dp[1];
+        dv[2] = dp[2];
       }
     }
     __syncthreads();
     if (own) {
-      T s0[3] = {carry[0], carry[1], carry[2]};
-      T s1[3] = {T(0), T(0), T(0)};
+      T s0[3], s1[3];
 #pragma unroll
-      for (int bc = 0; bc < 4; ++bc) {
-        const int qb = bc & 1, qc = bc >> 1;
-        const int src = (ty - qb) * TZ + (tz - qc);
-        const T* o0 = OUT + ((2 * qb + 4 * qc) * 256 + src) * 3;
-        const T* o1 = OUT + ((1 + 2 * qb + 4 * qc) * 256 + src) * 3;
-        s0[0] += o0[0];
-        s0[1] += o0[1];
-        s0[2] += o0[2];
-        s1[0] += o1[0];
-        s1[1] += o1[1];
-        s1[2] += o1[2];
+      for (int c = 0; c < 3; ++c) {
+        s0[c] = carry[c] + y[0][c];
+        s1[c] = y[1][c];
+      }
+      {
+        const T* o = &OUT[0][t - TZ][0];  // element (ty-1, tz): corners q = 2 (plane i), 3 (plane i+1)
+        const T* o1 = &OUT[1][t - TZ][0];
+        const T* p = &OUT[2][t - 1][0];  // element (ty, tz-1): corners q = 4, 5
+        const T* p1 = &OUT[3][t - 1][0];
+        const T* d = &OUT[4][t - TZ - 1][0];  // element (ty-1, tz-1): corners q = 6, 7
+        const T* d1 = &OUT[5][t - TZ - 1][0];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          s0[c] += o[c] + p[c] + d[c];
+          s1[c] += o1[c] + p1[c] + d1[c];
+        }
       }
       carry[0] = s1[0];
       carry[1] = s1[1];
       carry[2] = s1[2];
       if (i >= i0) {
-        const long long n = ((long long)i * g.ny + ej) * g.nz + el;
-        const uint8_t mk = mask[n];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const bool on = bit_live(mk, c);
-          const long long o = (long long)r * nodes * 3 + n * 3 + c;
+          const bool on = bit_live(mask_cur, c);
           if (MODE == MODE_APPLY) {
-            Y[n * 3 + c] = on ? s0[c] : T(0);
+            Y[3LL * nn + c] = on ? s0[c] : T(0);
           } else if (MODE == MODE_DOT) {
             const T yv = on ? s0[c] : T(0);
-            Y[n * 3 + c] = yv;
+            Y[3LL * nn + c] = yv;
             dot += (double)xs[c] * (double)yv;
           } else if (MODE == MODE_RESID) {
-            const T bv = reinterpret_cast<const T*>(a.b)[o];
-            Y[n * 3 + c] = on ? bv - s0[c] : T(0);
+            Y[3LL * nn + c] = on ? bv[c] - s0[c] : T(0);
           } else if (MODE == MODE_CHEB) {
-            const T bv = reinterpret_cast<const T*>(a.b)[o];
-            const T xpv = a.xp ? reinterpret_cast<const T*>(a.xp)[o] : T(0);
-            const T dv = reinterpret_cast<const T*>(a.dinv)[n * 3 + c];
             const T c1 = (T)a.c1, c2 = (T)a.c2;
-            const T xv = xs[c];
-            Y[n * 3 + c] = on ? xv + c1 * (xv - xpv) + c2 * dv * (bv - s0[c]) : T(0);
+            Y[3LL * nn + c] = on ? xs[c] + c1 * (xs[c] - xpv[c]) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
           }
         }
       }
     }
+    mask_cur = mask_next;
+    e_cur = e_next;
   }
   if (MODE == MODE_DOT) {
-    // wavefront reduction (64 lanes) then across the 4 wavefronts of the block
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((t & 63) == 0) red[t >> 6] = dot;
     __syncthreads();
@@ -211,58 +337,94 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
 }
 
 struct Tiling {
-  int TY, TZ, tiles_y, tiles_z, xchunk, chunks;
+  int TZ, tiles_y, tiles_z, xchunk, chunks;
 };
 
 static Tiling choose_tiling(const Geom& g, int k) {
   Tiling best{};
-  double best_cost = 1e300;
-  for (int TZ = 8; TZ <= 64; ++TZ) {
-    int TY = 256 / TZ;
-    if (TY < 3) continue;
+  long long best_cost = -1;
+  for (int TZ : {8, 16, 32, 64}) {
+    const int TY = 256 / TZ;
     const int ty = (g.ny + TY - 2) / (TY - 1), tz = (g.nz + TZ - 2) / (TZ - 1);
-    const int waves = (TY * TZ + 63) / 64;
-    double cost = (double)ty * tz * waves * (1.0 + 0.15 * (TZ < 16) + 0.3 * (TZ < 12));
-    if (cost < best_cost) {
+    const long long cost = (long long)ty * tz;
+    if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
-      best = Tiling{TY, TZ, ty, tz, 0, 0};
+      best = Tiling{TZ, ty, tz, 0, 0};
     }
   }
+  // enough workgroups to fill 256 CUs several times over; each x-chunk recomputes one element layer
   const long long tiles = (long long)best.tiles_y * best.tiles_z * (k > 0 ? k : 1);
-  const long long target = 4096;
-  int chunks = (int)((target + tiles - 1) / tiles);
-  chunks = std::max(1, std::min(chunks, (g.nx + 7) / 8));
+  int chunks = (int)((8192 + tiles - 1) / tiles);
+  chunks = std::max(1, std::min(chunks, std::max(1, g.nx / 32)));
   best.xchunk = (g.nx + chunks - 1) / chunks;
   best.chunks = (g.nx + best.xchunk - 1) / best.xchunk;
   return best;
 }
 
 int fine_blocks(const Geom& g) {
-  // upper bound for the partial-sum buffer (tiling for k = 1 has the most chunks)
   Tiling tl = choose_tiling(g, 1);
   return tl.tiles_y * tl.tiles_z * tl.chunks;
 }
 
 template <typename T>
+static KeSym<T> sym_blocks(const KeBlocks& kb) {
+  KeSym<T> s;
+  auto b = [&](int p, int i, int j) { return (T)kb.b[p][i][j]; };
+  const double big = [&] {
+    double m = 0;
+    for (int p = 0; p < 8; ++p)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) m = std::max(m, std::fabs(kb.b[p][i][j]));
+    return m;
+  }();
+  // structural zeros of the axis-aligned brick (rigid-translation rows and the parity-forbidden couplings)
+  const int zeros[][3] = {{1, 0, 0}, {1, 0, 1}, {1, 0, 2}, {1, 1, 0}, {1, 2, 0}, {2, 1, 0}, {2, 1, 1}, {2, 1, 2},
+                          {2, 0, 1}, {2, 2, 1}, {4, 2, 0}, {4, 2, 1}, {4, 2, 2}, {4, 0, 2}, {4, 1, 2}, {3, 0, 2},
+                          {3, 1, 2}, {3, 2, 0}, {3, 2, 1}, {5, 0, 1}, {5, 1, 0}, {5, 1, 2}, {5, 2, 1}, {6, 0, 1},
+                          {6, 0, 2}, {6, 1, 0}, {6, 2, 0}};
+  for (auto& z : zeros)
+    if (std::fabs(kb.b[z[0]][z[1]][z[2]]) > 1e-10 * big)
+      throw std::runtime_error("element stiffness does not have the block pattern of an axis-aligned brick");
+  for (int p = 0; p < 8; ++p)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        if (std::fabs(kb.b[p][i][j] - kb.b[p][j][i]) > 1e-10 * big)
+          throw std::runtime_error("element stiffness is not symmetric");
+  const T v[33] = {b(0, 0, 0), b(0, 0, 1), b(0, 0, 2), b(0, 1, 1), b(0, 1, 2), b(0, 2, 2),
+                   b(7, 0, 0), b(7, 0, 1), b(7, 0, 2), b(7, 1, 1), b(7, 1, 2), b(7, 2, 2),
+                   b(1, 1, 1), b(1, 1, 2), b(1, 2, 2), b(2, 0, 0), b(2, 0, 2), b(2, 2, 2),
+                   b(4, 0, 0), b(4, 0, 1), b(4, 1, 1), b(3, 0, 0), b(3, 0, 1), b(3, 1, 1), b(3, 2, 2),
+                   b(5, 0, 0), b(5, 0, 2), b(5, 2, 2), b(5, 1, 1), b(6, 1, 1), b(6, 1, 2), b(6, 2, 2), b(6, 0, 0)};
+  for (int i = 0; i < 33; ++i) s.k[i] = v[i];
+  return s;
+}
+
+template <typename T>
 void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlocks& kbd, int mode,
                 const ApplyArgs& a, hipStream_t s, int* nblocks_out) {
+  if (g.nodes() * 3 > (long long)INT32_MAX) throw std::runtime_error("grid too large for 32-bit node indexing");
   Tiling tl = choose_tiling(g, mode == MODE_DOT ? 1 : a.k);
-  Blocks<T> kb;
-  for (int p = 0; p < 8; ++p)
-    for (int e = 0; e < 9; ++e) kb.b[p][e] = (T)kbd.b[p][e / 3][e % 3];
+  const KeSym<T> ks = sym_blocks<T>(kbd);
   dim3 grid(tl.tiles_y * tl.tiles_z * tl.chunks, a.k);
-  const size_t lds = (size_t)(2 * (tl.TY + 1) * (tl.TZ + 1) * 3 + 8 * 256 * 3) * sizeof(T);
   if (nblocks_out) *nblocks_out = grid.x;
-#define DFMI_LAUNCH(M)                                                                                   \
-  hipLaunchKernelGGL((k_fine_apply<T, M>), grid, dim3(256), lds, s, g, moduli, mask, kb, a, tl.TY, tl.TZ, \
-                     tl.xchunk, tl.tiles_y, tl.tiles_z)
+#define DFMI_LAUNCH(M, TZV)                                                                                   \
+  hipLaunchKernelGGL((k_fine_apply<T, M, TZV>), grid, dim3(256), 0, s, g, moduli, mask, ks, a, tl.xchunk, \
+                     tl.tiles_y, tl.tiles_z)
+#define DFMI_TILE(M)                          \
+  switch (tl.TZ) {                            \
+    case 8: DFMI_LAUNCH(M, 8); break;         \
+    case 16: DFMI_LAUNCH(M, 16); break;       \
+    case 32: DFMI_LAUNCH(M, 32); break;       \
+    default: DFMI_LAUNCH(M, 64); break;       \
+  }
   switch (mode) {
-    case MODE_APPLY: DFMI_LAUNCH(MODE_APPLY); break;
-    case MODE_RESID: DFMI_LAUNCH(MODE_RESID); break;
-    case MODE_CHEB: DFMI_LAUNCH(MODE_CHEB); break;
-    case MODE_DOT: DFMI_LAUNCH(MODE_DOT); break;
+    case MODE_APPLY: DFMI_TILE(MODE_APPLY); break;
+    case MODE_RESID: DFMI_TILE(MODE_RESID); break;
+    case MODE_CHEB: DFMI_TILE(MODE_CHEB); break;
+    case MODE_DOT: DFMI_TILE(MODE_DOT); break;
     default: throw std::runtime_error("fine_apply: bad mode");
   }
+#undef DFMI_TILE
 #undef DFMI_LAUNCH
   DFMI_HIP(hipGetLastError());
 }

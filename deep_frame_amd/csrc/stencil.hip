@@ -86,14 +86,25 @@ void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, co
     case MODE_CHEB: DFMI_SLAUNCH(RC, MODE_CHEB); break;        \
     default: throw std::runtime_error("stencil_apply: bad mode"); \
   }
-  if (a.k <= 1) {
+  // right-hand sides per thread: each stencil block is read once per chunk of rc columns
+  const int chunks = (a.k + 15) / 16;
+  const int want = (a.k + chunks - 1) / chunks;
+  if (want <= 1) {
     DFMI_SMODE(1)
-  } else if (a.k <= 2) {
+  } else if (want <= 2) {
     DFMI_SMODE(2)
-  } else if (a.k <= 4) {
+  } else if (want <= 4) {
     DFMI_SMODE(4)
-  } else {
+  } else if (want <= 6) {
+    DFMI_SMODE(6)
+  } else if (want <= 8) {
     DFMI_SMODE(8)
+  } else if (want <= 11) {
+    DFMI_SMODE(11)
+  } else if (want <= 14) {
+    DFMI_SMODE(14)
+  } else {
+    DFMI_SMODE(16)
   }
 #undef DFMI_SMODE
 #undef DFMI_SLAUNCH
@@ -313,24 +324,101 @@ void coarsest_assemble(const Geom& g, const double* st, const int* live_index, i
   DFMI_HIP(hipGetLastError());
 }
 
-// Symmetric sweep operator (Goodnight 1979): sweeping every pivot of an SPD matrix
-// leaves -A^-1.  Row k is copied first so the rank-1 update reads unmodified values.
-__global__ void k_sweep_copy(const double* a, int n, int k, double* row) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) row[j] = a[(long long)k * n + j];
+// Symmetric sweep operator (Goodnight 1979), blocked: sweeping the pivot block P of a symmetric
+// matrix maps [A_PP A_PQ; A_QP A_QQ] to [-A_PP^-1, A_PP^-1 A_PQ; A_QP A_PP^-1, A_QQ - A_QP A_PP^-1 A_PQ];
+// sweeps on disjoint blocks commute, and sweeping every pivot of an SPD matrix leaves -A^-1.
+// Per block of NB pivots: copy the panel rows R = A[P, :], invert the NB x NB diagonal block in
+// LDS (Gauss-Jordan, no pivoting: SPD), form W = A_PP^-1 R, then one rank-NB update of A.
+constexpr int NB = 32;
+
+__global__ void k_panel_copy(const double* __restrict__ a, int n, int k0, int b, double* __restrict__ r) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)b * n) return;
+  const int p = (int)(idx / n), j = (int)(idx % n);
+  r[idx] = a[(long long)(k0 + p) * n + j];
 }
 
-__global__ void k_sweep(double* a, int n, int k, const double* row) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)n * n) return;
-  const int i = (int)(idx / n), j = (int)(idx % n);
-  const double d = row[k];
-  double v;
-  if (i == k && j == k) v = -1.0 / d;
-  else if (i == k) v = row[j] / d;
-  else if (j == k) v = row[i] / d;
-  else v = a[idx] - row[i] * row[j] / d;
-  a[idx] = v;
+// in-place Gauss-Jordan inverse of the b x b SPD block R[:, k0:k0+b] -> dinv (one workgroup)
+__global__ __launch_bounds__(256) void k_block_inverse(const double* __restrict__ r, int n, int k0, int b,
+                                                       double* __restrict__ dinv) {
+  __shared__ double m[NB][NB + 1];
+  for (int v = threadIdx.x; v < b * b; v += blockDim.x) m[v / b][v % b] = r[(long long)(v / b) * n + k0 + v % b];
+  __syncthreads();
+  for (int k = 0; k < b; ++k) {
+    const double d = m[k][k];
+    __syncthreads();
+    // sweep pivot k inside the block (same rule as the scalar sweep)
+    double nv[(NB * NB + 255) / 256];
+    int cnt = 0;
+    for (int v = threadIdx.x; v < b * b; v += blockDim.x, ++cnt) {
+      const int i = v / b, j = v % b;
+      double x;
+      if (i == k && j == k) x = -1.0 / d;
+      else if (i == k) x = m[k][j] / d;
+      else if (j == k) x = m[i][k] / d;
+      else x = m[i][j] - m[i][k] * m[k][j] / d;
+      nv[cnt] = x;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int v = threadIdx.x; v < b * b; v += blockDim.x, ++cnt) m[v / b][v % b] = nv[cnt];
+    __syncthreads();
+  }
+  // the block now holds -D^-1
+  for (int v = threadIdx.x; v < b * b; v += blockDim.x) dinv[v] = -m[v / b][v % b];
+}
+
+// W[p][j] = sum_q dinv[p][q] R[q][j]
+__global__ void k_panel_w(const double* __restrict__ dinv, const double* __restrict__ r, int n, int b,
+                          double* __restrict__ w) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (j >= n) return;
+  double acc = 0.0;
+  for (int q = 0; q < b; ++q) acc = fma(dinv[p * b + q], r[(long long)q * n + j], acc);
+  w[(long long)p * n + j] = acc;
+}
+
+// rank-b update; 64 x 64 output tile per workgroup, 4 x 4 outputs per thread
+__global__ __launch_bounds__(256) void k_block_update(double* __restrict__ a, int n, int k0, int b,
+                                                      const double* __restrict__ r, const double* __restrict__ w,
+                                                      const double* __restrict__ dinv) {
+  __shared__ double ri[NB][64];
+  __shared__ double wj[NB][64];
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  for (int v = threadIdx.x; v < b * 64; v += 256) {
+    const int p = v / 64, c = v % 64;
+    ri[p][c] = (i0 + c < n) ? r[(long long)p * n + i0 + c] : 0.0;
+    wj[p][c] = (j0 + c < n) ? w[(long long)p * n + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % 16, tyy = threadIdx.x / 16;
+  double acc[4][4] = {};
+  for (int p = 0; p < b; ++p) {
+    double x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = ri[p][tyy + 16 * u];
+      y[u] = wj[p][tx + 16 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + tyy + 16 * u, j = j0 + tx + 16 * v;
+      if (i >= n || j >= n) continue;
+      const bool pi = i >= k0 && i < k0 + b, pj = j >= k0 && j < k0 + b;
+      double& dst = a[(long long)i * n + j];
+      if (pi && pj) dst = -dinv[(i - k0) * b + (j - k0)];
+      else if (pi) dst = w[(long long)(i - k0) * n + j];
+      else if (pj) dst = w[(long long)(j - k0) * n + i];
+      else dst -= acc[u][v];
+    }
 }
 
 __global__ void k_negate(double* a, long long n) {
@@ -339,16 +427,25 @@ __global__ void k_negate(double* a, long long n) {
 }
 
 void dense_sweep_inverse(double* a, int n, hipStream_t s) {
-  double* row = nullptr;
-  DFMI_HIP(hipMallocAsync((void**)&row, sizeof(double) * n, s));
-  const unsigned b1 = (unsigned)((n + 255) / 256), b2 = (unsigned)(((long long)n * n + 255) / 256);
-  for (int k = 0; k < n; ++k) {
-    hipLaunchKernelGGL(k_sweep_copy, dim3(b1), dim3(256), 0, s, a, n, k, row);
-    hipLaunchKernelGGL(k_sweep, dim3(b2), dim3(256), 0, s, a, n, k, row);
+  double *r = nullptr, *w = nullptr, *dinv = nullptr;
+  DFMI_HIP(hipMallocAsync((void**)&r, sizeof(double) * NB * (size_t)n, s));
+  DFMI_HIP(hipMallocAsync((void**)&w, sizeof(double) * NB * (size_t)n, s));
+  DFMI_HIP(hipMallocAsync((void**)&dinv, sizeof(double) * NB * NB, s));
+  const unsigned tiles = (unsigned)((n + 63) / 64);
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int b = std::min(NB, n - k0);
+    hipLaunchKernelGGL(k_panel_copy, dim3((unsigned)(((long long)b * n + 255) / 256)), dim3(256), 0, s, a, n, k0, b,
+                       r);
+    hipLaunchKernelGGL(k_block_inverse, dim3(1), dim3(256), 0, s, r, n, k0, b, dinv);
+    hipLaunchKernelGGL(k_panel_w, dim3((unsigned)((n + 255) / 256), b), dim3(256), 0, s, dinv, r, n, b, w);
+    hipLaunchKernelGGL(k_block_update, dim3(tiles, tiles), dim3(256), 0, s, a, n, k0, b, r, w, dinv);
   }
+  const unsigned b2 = (unsigned)(((long long)n * n + 255) / 256);
   hipLaunchKernelGGL(k_negate, dim3(b2), dim3(256), 0, s, a, (long long)n * n);
   DFMI_HIP(hipGetLastError());
-  DFMI_HIP(hipFreeAsync(row, s));
+  DFMI_HIP(hipFreeAsync(r, s));
+  DFMI_HIP(hipFreeAsync(w, s));
+  DFMI_HIP(hipFreeAsync(dinv, s));
 }
 
 template <typename T>
