@@ -121,22 +121,26 @@ __device__ __forceinline__ void sym_mul(const KeSym<T>& s, const T (&v)[8][3], T
   w[0][0] = w[0][1] = w[0][2] = T(0);
 }
 
-// LDS slot per node / corner value: float4 (one ds_read_b128) or three doubles
+// Threads per workgroup: one element column per thread.
 template <typename T>
-struct Slot {
-  static constexpr int S = sizeof(T) == 4 ? 4 : 3;
+struct FineCfg {
+  static constexpr int NT = sizeof(T) == 4 ? 512 : 256;
+  static constexpr int MAXNP = NT + 136;  // (TY+1)(TZ+1) = TY*TZ + TY + TZ + 1 with TZ <= 128, TY >= 4
 };
 
-template <typename T, int MODE, int TZ>
-__global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict__ moduli,
-                                                     const uint8_t* __restrict__ mask, KeSym<T> ks, ApplyArgs a,
-                                                     int xchunk, int tiles_y, int tiles_z) {
-  constexpr int TY = 256 / TZ;
-  constexpr int NP = (TY + 1) * (TZ + 1);
-  constexpr int S = Slot<T>::S;
-  __shared__ __align__(16) T U[2][NP][S];     // node planes i and i+1 of the operand (masked)
-  __shared__ __align__(16) T OUT[6][256][S];  // element corner results q = 2..7
-  __shared__ double red[4];
+// One workgroup owns a (TY-1) x (TZ-1) tile of node columns (TY * TZ <= NT element columns including
+// a one-element halo at the low side) and marches along x.  LDS is structure-of-arrays (component
+// planes), so consecutive lanes touch consecutive 4- or 8-byte words: no bank conflicts.
+template <typename T, int MODE>
+__global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* __restrict__ moduli,
+                                                               const uint8_t* __restrict__ mask, KeSym<T> ks,
+                                                               ApplyArgs a, int TY, int TZ, int xchunk, int tiles_y,
+                                                               int tiles_z) {
+  constexpr int NT = FineCfg<T>::NT;
+  constexpr int MAXNP = FineCfg<T>::MAXNP;
+  __shared__ T U[2][3][MAXNP];  // node planes i and i+1 of the operand (masked)
+  __shared__ T OUT[6][3][NT];   // element corner results q = 2..7
+  __shared__ double red[NT / 64];
 
   const int r = blockIdx.y;
   if (a.active && !a.active[r]) return;
@@ -149,37 +153,39 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
   const int i0 = cx * xchunk;
   const int i1 = min(i0 + xchunk, g.nx);
   const int t = threadIdx.x;
-  const int ty = t / TZ, tz = t % TZ;
+  const int NP = (TY + 1) * (TZ + 1);
+  const bool live = t < TY * TZ;
+  const int ty = live ? t / TZ : 0, tz = live ? t - (t / TZ) * TZ : 0;
   const int ej = j0 - 1 + ty, el = l0 - 1 + tz;  // element of this thread; also the node it outputs
   const int plane_nodes = g.ny * g.nz;
   const long long col = (long long)g.nodes() * 3;
   const T* X = reinterpret_cast<const T*>(a.x) + (long long)r * col;
   T* Y = reinterpret_cast<T*>(a.y) + (long long)r * col;
-  const bool own = ty >= 1 && tz >= 1 && ej < g.ny && el < g.nz;
-  const bool elem_in = ej >= 0 && el >= 0 && ej < g.ey && el < g.ez;
+  const bool own = live && ty >= 1 && tz >= 1 && ej < g.ny && el < g.nz;
+  const bool elem_in = live && ej >= 0 && el >= 0 && ej < g.ey && el < g.ez;
+  const int self = ty * (TZ + 1) + tz;  // this thread's node in the staged plane
 
-  // nodes staged by this thread: (ty, tz) and, on the high edges of the tile, the halo row/column.
-  // Fixed slots (no dynamic indexing, so the arrays stay in registers).
-  const int ly[4] = {ty, ty, TY, TY}, lz[4] = {tz, TZ, tz, TZ};
-  const bool has[4] = {true, tz == TZ - 1, ty == TY - 1, ty == TY - 1 && tz == TZ - 1};
-  int nl[4], no[4];
+  // plane staging: node slots v = t and v = t + NT of the (TY+1) x (TZ+1) tile (offsets precomputed)
+  int so[2];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int gj = j0 - 1 + ly[k], gl = l0 - 1 + lz[k];
-    nl[k] = ly[k] * (TZ + 1) + lz[k];
-    no[k] = (has[k] && gj >= 0 && gl >= 0 && gj < g.ny && gl < g.nz) ? gj * g.nz + gl : -1;
+  for (int k = 0; k < 2; ++k) {
+    const int v = t + k * NT;
+    so[k] = -2;  // -2: no slot, -1: slot outside the grid
+    if (v < NP) {
+      const int ly = v / (TZ + 1), lz = v - ly * (TZ + 1);
+      const int gj = j0 - 1 + ly, gl = l0 - 1 + lz;
+      so[k] = (gj >= 0 && gl >= 0 && gj < g.ny && gl < g.nz) ? gj * g.nz + gl : -1;
+    }
   }
-  T nv[4][3];
-  uint8_t nm[4];
+  T nv[2][3];
   auto fetch = [&](int ip) {
+    const bool in = ip >= 0 && ip < g.nx;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 2; ++k) {
       nv[k][0] = nv[k][1] = nv[k][2] = T(0);
-      nm[k] = 0;
-      if (no[k] >= 0 && ip >= 0 && ip < g.nx) {
-        const int n = ip * plane_nodes + no[k];
+      if (in && so[k] >= 0) {
+        const int n = ip * plane_nodes + so[k];
         const uint8_t m = mask[n];
-        nm[k] = m;
         const T* src = X + 3LL * n;
         const T v0 = src[0], v1 = src[1], v2 = src[2];
         nv[k][0] = (m & 1) ? v0 : T(0);
@@ -190,12 +196,12 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
   };
   auto stage = [&](int buf) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (has[k]) {
-        T* d = &U[buf][nl[k]][0];
-        d[0] = nv[k][0];
-        d[1] = nv[k][1];
-        d[2] = nv[k][2];
+    for (int k = 0; k < 2; ++k) {
+      if (so[k] != -2) {
+        const int v = t + k * NT;
+        U[buf][0][v] = nv[k][0];
+        U[buf][1][v] = nv[k][1];
+        U[buf][2][v] = nv[k][2];
       }
     }
   };
@@ -207,30 +213,53 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
   double dot = 0.0;
   fetch(i0 - 1);
   stage((i0 - 1) & 1);
-  uint8_t mask_cur = nm[0];
   fetch(i0);
   T e_cur = modulus(i0 - 1);
+  uint8_t mk_next = 0;
   for (int i = i0 - 1; i < i1; ++i) {
     stage((i + 1) & 1);
-    const uint8_t mask_next = nm[0];
     if (i + 2 <= i1) fetch(i + 2);
     const T e_next = modulus(i + 1);
-    __syncthreads();
-    T xs[3];
-    {
-      const T* src = &U[i & 1][ty * (TZ + 1) + tz][0];
-      xs[0] = src[0];
-      xs[1] = src[1];
-      xs[2] = src[2];
+    const int nn = i * plane_nodes + ej * g.nz + el;
+    const bool out = own && i >= i0;
+    const uint8_t mk = out ? mask[nn] : 0;
+    // epilogue operands of node (i, ej, el), requested early
+    T bv[3] = {T(0), T(0), T(0)}, xpv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)};
+    if (out) {
+      if (MODE == MODE_RESID || MODE == MODE_CHEB) {
+        const T* bp = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * nn;
+        bv[0] = bp[0];
+        bv[1] = bp[1];
+        bv[2] = bp[2];
+      }
+      if (MODE == MODE_CHEB) {
+        if (a.xp) {
+          const T* pp = reinterpret_cast<const T*>(a.xp) + (long long)r * col + 3LL * nn;
+          xpv[0] = pp[0];
+          xpv[1] = pp[1];
+          xpv[2] = pp[2];
+        }
+        const T* dp = reinterpret_cast<const T*>(a.dinv) + 3LL * nn;
+        dv[0] = dp[0];
+        dv[1] = dp[1];
+        dv[2] = dp[2];
+      }
     }
+    __syncthreads();
+    const int b0 = i & 1, b1 = (i + 1) & 1;
+    T xs[3];
+    xs[0] = U[b0][0][self];
+    xs[1] = U[b0][1][self];
+    xs[2] = U[b0][2][self];
     T y[8][3];
     if (e_cur != T(0)) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const T* src = &U[(i + (q & 1)) & 1][(ty + ((q >> 1) & 1)) * (TZ + 1) + tz + ((q >> 2) & 1)][0];
-        y[q][0] = src[0];
-        y[q][1] = src[1];
-        y[q][2] = src[2];
+        const int buf = (q & 1) ? b1 : b0;
+        const int v = self + ((q >> 1) & 1) * (TZ + 1) + ((q >> 2) & 1);
+        y[q][0] = U[buf][0][v];
+        y[q][1] = U[buf][1][v];
+        y[q][2] = U[buf][2][v];
       }
       butterfly(y);
 #pragma unroll
@@ -254,62 +283,24 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
     }
 #pragma unroll
     for (int q = 2; q < 8; ++q) {
-      T* d = &OUT[q - 2][t][0];
-      d[0] = y[q][0];
-      d[1] = y[q][1];
-      d[2] = y[q][2];
-    }
-    // epilogue operands of node (i, ej, el), fetched before the barrier
-    const int nn = i * plane_nodes + ej * g.nz + el;
-    T bv[3] = {T(0), T(0), T(0)}, xpv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)};
-    if (own && i >= i0) {
-      if (MODE == MODE_RESID || MODE == MODE_CHEB) {
-        const T* bp = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * nn;
-        bv[0] = bp[0];
-        bv[1] = bp[1];
-        bv[2] = bp[2];
-      }
-      if (MODE == MODE_CHEB) {
-        if (a.xp) {
-          const T* pp = reinterpret_cast<const T*>(a.xp) + (long long)r * col + 3LL * nn;
-          xpv[0] = pp[0];
-          xpv[1] = pp[1];
-          xpv[2] = pp[2];
-        }
-        const T* dp = reinterpret_cast<const T*>(a.dinv) + 3LL * nn;
-        dv[0] = dp[0];
-        dv[1] = dp[1];
-        dv[2] = dp[2];
-      }
+      OUT[q - 2][0][t] = y[q][0];
+      OUT[q - 2][1][t] = y[q][1];
+      OUT[q - 2][2][t] = y[q][2];
     }
     __syncthreads();
     if (own) {
+      const int tb = t - TZ, tc = t - 1, td = t - TZ - 1;  // elements (ty-1,tz), (ty,tz-1), (ty-1,tz-1)
       T s0[3], s1[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        s0[c] = carry[c] + y[0][c];
-        s1[c] = y[1][c];
+        s0[c] = carry[c] + y[0][c] + OUT[0][c][tb] + OUT[2][c][tc] + OUT[4][c][td];
+        s1[c] = y[1][c] + OUT[1][c][tb] + OUT[3][c][tc] + OUT[5][c][td];
+        carry[c] = s1[c];
       }
-      {
-        const T* o = &OUT[0][t - TZ][0];  // element (ty-1, tz): corners q = 2 (plane i), 3 (plane i+1)
-        const T* o1 = &OUT[1][t - TZ][0];
-        const T* p = &OUT[2][t - 1][0];  // element (ty, tz-1): corners q = 4, 5
-        const T* p1 = &OUT[3][t - 1][0];
-        const T* d = &OUT[4][t - TZ - 1][0];  // element (ty-1, tz-1): corners q = 6, 7
-        const T* d1 = &OUT[5][t - TZ - 1][0];
+      if (out) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          s0[c] += o[c] + p[c] + d[c];
-          s1[c] += o1[c] + p1[c] + d1[c];
-        }
-      }
-      carry[0] = s1[0];
-      carry[1] = s1[1];
-      carry[2] = s1[2];
-      if (i >= i0) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const bool on = bit_live(mask_cur, c);
+          const bool on = bit_live(mk, c);
           if (MODE == MODE_APPLY) {
             Y[3LL * nn + c] = on ? s0[c] : T(0);
           } else if (MODE == MODE_DOT) {
@@ -325,36 +316,42 @@ __global__ __launch_bounds__(256) void k_fine_apply(Geom g, const T* __restrict_
         }
       }
     }
-    mask_cur = mask_next;
     e_cur = e_next;
+    (void)mk_next;
   }
   if (MODE == MODE_DOT) {
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((t & 63) == 0) red[t >> 6] = dot;
     __syncthreads();
-    if (t == 0) a.partial[(long long)r * gridDim.x + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (t == 0) {
+      double s = 0.0;
+      for (int w = 0; w < NT / 64; ++w) s += red[w];
+      a.partial[(long long)r * gridDim.x + blockIdx.x] = s;
+    }
   }
 }
 
 struct Tiling {
-  int TZ, tiles_y, tiles_z, xchunk, chunks;
+  int TY, TZ, tiles_y, tiles_z, xchunk, chunks;
 };
 
-static Tiling choose_tiling(const Geom& g, int k) {
+static Tiling choose_tiling(const Geom& g, int k, int nt) {
+  // z tile spans the whole node column when it fits (contiguous rows, full cache lines)
   Tiling best{};
   long long best_cost = -1;
-  for (int TZ : {8, 16, 32, 64}) {
-    const int TY = 256 / TZ;
+  for (int TZ = 8; TZ <= std::min(nt / 4, 128); ++TZ) {
+    const int TY = nt / TZ;
+    if ((TY + 1) * (TZ + 1) > nt + 136 || TY < 4) continue;
     const int ty = (g.ny + TY - 2) / (TY - 1), tz = (g.nz + TZ - 2) / (TZ - 1);
-    const long long cost = (long long)ty * tz;
-    if (best_cost < 0 || cost < best_cost) {
+    const long long cost = (long long)ty * tz * ((TY * TZ + 63) / 64) * 64;
+    if (best_cost < 0 || cost < best_cost || (cost == best_cost && TZ > best.TZ)) {
       best_cost = cost;
-      best = Tiling{TZ, ty, tz, 0, 0};
+      best = Tiling{TY, TZ, ty, tz, 0, 0};
     }
   }
   // enough workgroups to fill 256 CUs several times over; each x-chunk recomputes one element layer
   const long long tiles = (long long)best.tiles_y * best.tiles_z * (k > 0 ? k : 1);
-  int chunks = (int)((8192 + tiles - 1) / tiles);
+  int chunks = (int)((4096 + tiles - 1) / tiles);
   chunks = std::max(1, std::min(chunks, std::max(1, g.nx / 32)));
   best.xchunk = (g.nx + chunks - 1) / chunks;
   best.chunks = (g.nx + best.xchunk - 1) / best.xchunk;
@@ -362,8 +359,8 @@ static Tiling choose_tiling(const Geom& g, int k) {
 }
 
 int fine_blocks(const Geom& g) {
-  Tiling tl = choose_tiling(g, 1);
-  return tl.tiles_y * tl.tiles_z * tl.chunks;
+  Tiling a = choose_tiling(g, 1, FineCfg<float>::NT), b = choose_tiling(g, 1, FineCfg<double>::NT);
+  return std::max(a.tiles_y * a.tiles_z * a.chunks, b.tiles_y * b.tiles_z * b.chunks);
 }
 
 template <typename T>
@@ -403,28 +400,21 @@ template <typename T>
 void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlocks& kbd, int mode,
                 const ApplyArgs& a, hipStream_t s, int* nblocks_out) {
   if (g.nodes() * 3 > (long long)INT32_MAX) throw std::runtime_error("grid too large for 32-bit node indexing");
-  Tiling tl = choose_tiling(g, mode == MODE_DOT ? 1 : a.k);
+  constexpr int NT = FineCfg<T>::NT;
+  Tiling tl = choose_tiling(g, mode == MODE_DOT ? 1 : a.k, NT);
   const KeSym<T> ks = sym_blocks<T>(kbd);
   dim3 grid(tl.tiles_y * tl.tiles_z * tl.chunks, a.k);
   if (nblocks_out) *nblocks_out = grid.x;
-#define DFMI_LAUNCH(M, TZV)                                                                                   \
-  hipLaunchKernelGGL((k_fine_apply<T, M, TZV>), grid, dim3(256), 0, s, g, moduli, mask, ks, a, tl.xchunk, \
-                     tl.tiles_y, tl.tiles_z)
-#define DFMI_TILE(M)                          \
-  switch (tl.TZ) {                            \
-    case 8: DFMI_LAUNCH(M, 8); break;         \
-    case 16: DFMI_LAUNCH(M, 16); break;       \
-    case 32: DFMI_LAUNCH(M, 32); break;       \
-    default: DFMI_LAUNCH(M, 64); break;       \
-  }
+#define DFMI_LAUNCH(M)                                                                                      \
+  hipLaunchKernelGGL((k_fine_apply<T, M>), grid, dim3(NT), 0, s, g, moduli, mask, ks, a, tl.TY, tl.TZ, \
+                     tl.xchunk, tl.tiles_y, tl.tiles_z)
   switch (mode) {
-    case MODE_APPLY: DFMI_TILE(MODE_APPLY); break;
-    case MODE_RESID: DFMI_TILE(MODE_RESID); break;
-    case MODE_CHEB: DFMI_TILE(MODE_CHEB); break;
-    case MODE_DOT: DFMI_TILE(MODE_DOT); break;
+    case MODE_APPLY: DFMI_LAUNCH(MODE_APPLY); break;
+    case MODE_RESID: DFMI_LAUNCH(MODE_RESID); break;
+    case MODE_CHEB: DFMI_LAUNCH(MODE_CHEB); break;
+    case MODE_DOT: DFMI_LAUNCH(MODE_DOT); break;
     default: throw std::runtime_error("fine_apply: bad mode");
   }
-#undef DFMI_TILE
 #undef DFMI_LAUNCH
   DFMI_HIP(hipGetLastError());
 }
