@@ -33,6 +33,16 @@ struct Geom {
 
 inline Geom make_geom(int ex, int ey, int ez) { return Geom{ex, ey, ez, ex + 1, ey + 1, ez + 1}; }
 
+// Multi-vector layout: entry (column r, node n, component c) lives at n * ns + r * rs + c.
+// The fine level is column-major (one contiguous column per right-hand side, as the flat DOF
+// vectors of the interface); coarse levels interleave the right-hand sides per node so that a
+// stencil gather of one neighbour is a single contiguous run for all columns of a thread.
+struct VLay {
+  long long ns, rs;
+};
+inline VLay column_major(const Geom& g) { return VLay{3, g.nodes() * 3}; }
+inline VLay interleaved(int k) { return VLay{3LL * k, 3}; }
+
 // Hex8 stiffness in the corner-Hadamard basis: 8 parity classes of 3x3 blocks,
 // pre-scaled by 1/64 (see hex8_blocks() in the host code).
 struct KeBlocks {
@@ -78,23 +88,23 @@ void active_mask_flat(const Geom& pad, int ux, int uy, int uz, const double* mod
 
 // ---- coarse levels (27-point block stencils, SoA [27*9][nodes]) -----------------------
 template <typename T>
-void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, hipStream_t s,
-                   int* nblocks_out);
+void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, VLay lay,
+                   hipStream_t s, int* nblocks_out);
 int stencil_blocks(const Geom& g, int k);
 void stencil_rap(const Geom& fine, const double* fst, const Geom& coarse, double* cst, hipStream_t s);
 void stencil_diagonal(const Geom& g, const double* st, double* diag, hipStream_t s);
 void stencil_mask(const Geom& g, double* st, const uint8_t* mask, hipStream_t s);
 void dead_mask(const Geom& g, const double* diag, double threshold, uint8_t* mask, hipStream_t s);
 template <typename T>
-void restrict_full(const Geom& fine, const T* f, const Geom& coarse, const uint8_t* cmask, T* c, int k,
-                   const int* active, hipStream_t s);
+void restrict_full(const Geom& fine, const T* f, VLay flay, const Geom& coarse, const uint8_t* cmask, T* c,
+                   VLay clay, int k, const int* active, hipStream_t s);
 template <typename T>
-void prolong_add(const Geom& coarse, const T* c, const Geom& fine, const uint8_t* fmask, T* f, int k,
-                 const int* active, hipStream_t s);
+void prolong_add(const Geom& coarse, const T* c, VLay clay, const Geom& fine, const uint8_t* fmask, T* f,
+                 VLay flay, int k, const int* active, hipStream_t s);
 void coarsest_assemble(const Geom& g, const double* st, const int* live_index, int n, double* dense, hipStream_t s);
 void dense_sweep_inverse(double* a, int n, hipStream_t s);
 template <typename T>
-void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out,
+void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out, VLay lay,
                     double* work_in, double* work_out, int k, const int* active, hipStream_t s);
 
 // ---- vectors ----------------------------------------------------------------------------
@@ -106,7 +116,7 @@ template <typename T>
 void column_absmax(const T* a, long long n, int k, double* partial, int nblocks, const int* active, hipStream_t s);
 void reduce_max(const double* partial, int nblocks, int k, double* out, hipStream_t s);
 template <typename T>
-void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active,
+void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active, VLay lay,
                 hipStream_t s);
 void power_init(const uint8_t* mask, double* v, long long nodes, unsigned seed, hipStream_t s);
 void dinv_scale(const double* y, const double* dinv, const double* inv_norm, double* v, long long nodes,

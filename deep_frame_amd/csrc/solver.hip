@@ -210,8 +210,11 @@ static void level_apply(VArgs& v, int idx, int mode, ApplyArgs& a) {
   if (idx == 0)
     fine_apply<T>(L.g, level_moduli<T>(v.h), L.mask.as<uint8_t>(), v.h->kb, mode, a, v.h->stream, nullptr);
   else
-    stencil_apply<T>(L.g, level_stencil<T>(L), L.mask.as<uint8_t>(), mode, a, v.h->stream, nullptr);
+    stencil_apply<T>(L.g, level_stencil<T>(L), L.mask.as<uint8_t>(), mode, a, interleaved(v.k), v.h->stream, nullptr);
 }
+
+// vector layout of a level inside the V-cycle: column-major on the fine level, interleaved below
+static VLay level_layout(const Level& L, int idx, int k) { return idx == 0 ? column_major(L.g) : interleaved(k); }
 
 // Chebyshev smoothing of A x = b (topology_multigrid.py:284-305) in the three-term form
 // x_{j+1} = x_j + rho_j rho_{j-1} (x_j - x_{j-1}) + (2 rho_j / delta) Dinv (b - A x_j).
@@ -228,7 +231,7 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
   const T* dinv = level_dinv<T>(L);
   bool prev_zero;
   if (from_zero) {
-    cheb_start<T>(b, dinv, 1.0 / theta, cur, L.g.nodes(), v.k, v.active, v.h->stream);
+    cheb_start<T>(b, dinv, 1.0 / theta, cur, L.g.nodes(), v.k, v.active, level_layout(L, idx, v.k), v.h->stream);
     prev_zero = true;
   } else {
     ApplyArgs a{};
@@ -269,7 +272,7 @@ static void vcycle(VArgs& v, int idx, const T* b) {
   hipStream_t s = v.h->stream;
   if (idx == (int)H.levels.size() - 1) {
     coarsest_solve<T>(L.g, H.ainv.as<double>(), H.live_dofs.as<int>(), H.coarsest_n, b, L.x.as<T>(),
-                      H.work_in.as<double>(), H.work_out.as<double>(), v.k, v.active, s);
+                      level_layout(L, idx, v.k), H.work_in.as<double>(), H.work_out.as<double>(), v.k, v.active, s);
     return;
   }
   Level& C = *H.levels[idx + 1];
@@ -281,9 +284,11 @@ static void vcycle(VArgs& v, int idx, const T* b) {
   a.y = L.res.as<T>();
   a.b = b;
   level_apply<T>(v, idx, MODE_RESID, a);
-  restrict_full<T>(L.g, L.res.as<T>(), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(), v.k, v.active, s);
+  restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
+                   level_layout(C, idx + 1, v.k), v.k, v.active, s);
   vcycle<T>(v, idx + 1, C.rhs.as<T>());
-  prolong_add<T>(C.g, C.x.as<T>(), L.g, L.mask.as<uint8_t>(), L.x.as<T>(), v.k, v.active, s);
+  prolong_add<T>(C.g, C.x.as<T>(), level_layout(C, idx + 1, v.k), L.g, L.mask.as<uint8_t>(), L.x.as<T>(),
+                 level_layout(L, idx, v.k), v.k, v.active, s);
   chebyshev<T>(v, idx, b, false, count);
 }
 
@@ -374,7 +379,8 @@ static double spectrum(Handle* h, Hierarchy& H, int idx) {
     if (idx == 0)
       fine_apply<double>(L.g, h->m64.as<double>(), L.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, h->stream, nullptr);
     else
-      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, h->stream, nullptr);
+      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, column_major(L.g),
+                            h->stream, nullptr);
     const double one = 1.0;
     DFMI_HIP(hipMemcpyAsync(inv.as<double>(), &one, sizeof(double), hipMemcpyHostToDevice, h->stream));
     dinv_scale(y.as<double>(), L.dinv64.as<double>(), inv.as<double>(), v.as<double>(), L.g.nodes(), h->stream);
@@ -863,7 +869,8 @@ int dfmi_level_apply(dfmi_mg* mg, int32_t slot, int32_t level, const double* x, 
     if (level == 0)
       fine_apply<double>(L.g, h.m64.as<double>(), L.mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream, nullptr);
     else
-      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, h.stream, nullptr);
+      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, column_major(L.g),
+                            h.stream, nullptr);
     sync(&h);
   })
 }
@@ -893,10 +900,12 @@ int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, c
     Level& F = *H.levels[level];
     Level& C = *H.levels[level + 1];
     if (direction == 0) {
-      restrict_full<double>(F.g, in, C.g, C.mask.as<uint8_t>(), out, k, nullptr, h.stream);
+      restrict_full<double>(F.g, in, column_major(F.g), C.g, C.mask.as<uint8_t>(), out, column_major(C.g), k,
+                            nullptr, h.stream);
     } else {
       fill_zero<double>(out, (long long)k * col(F.g), h.stream);
-      prolong_add<double>(C.g, in, F.g, F.mask.as<uint8_t>(), out, k, nullptr, h.stream);
+      prolong_add<double>(C.g, in, column_major(C.g), F.g, F.mask.as<uint8_t>(), out, column_major(F.g), k,
+                          nullptr, h.stream);
     }
     sync(&h);
   })

@@ -13,7 +13,7 @@ __device__ inline double weight1(int t) { return t == 0 ? 1.0 : ((t == 1 || t ==
 
 template <typename T, int RC, int MODE>
 __global__ __launch_bounds__(256) void k_stencil_apply(Geom g, const T* __restrict__ st,
-                                                        const uint8_t* __restrict__ mask, ApplyArgs a) {
+                                                        const uint8_t* __restrict__ mask, ApplyArgs a, VLay lay) {
   const long long N = g.nodes();
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int r0 = blockIdx.y * RC;
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_stencil_apply(Geom g, const T* __restri
 #pragma unroll
     for (int rr = 0; rr < RC; ++rr) {
       if (!on[rr]) continue;
-      const T* xm = X + ((long long)(r0 + rr) * N + m) * 3;
+      const T* xm = X + m * lay.ns + (long long)(r0 + rr) * lay.rs;
       const T x0 = live_bit(mm, 0) ? xm[0] : T(0);
       const T x1 = live_bit(mm, 1) ? xm[1] : T(0);
       const T x2 = live_bit(mm, 2) ? xm[2] : T(0);
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void k_stencil_apply(Geom g, const T* __restri
     const int r = r0 + rr;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const long long o = ((long long)r * N + n) * 3 + c;
+      const long long o = n * lay.ns + (long long)r * lay.rs + c;
       const bool lv = live_bit(mn, c);
       T* Y = reinterpret_cast<T*>(a.y);
       if (MODE == MODE_APPLY) {
@@ -73,12 +73,12 @@ __global__ __launch_bounds__(256) void k_stencil_apply(Geom g, const T* __restri
 int stencil_blocks(const Geom& g, int k) { return (int)((g.nodes() + 255) / 256); }
 
 template <typename T>
-void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, hipStream_t s,
-                   int* nblocks_out) {
+void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, VLay lay,
+                   hipStream_t s, int* nblocks_out) {
   const unsigned bx = (unsigned)((g.nodes() + 255) / 256);
   if (nblocks_out) *nblocks_out = bx;
 #define DFMI_SLAUNCH(RC, M) \
-  hipLaunchKernelGGL((k_stencil_apply<T, RC, M>), dim3(bx, (a.k + RC - 1) / RC), dim3(256), 0, s, g, st, mask, a)
+  hipLaunchKernelGGL((k_stencil_apply<T, RC, M>), dim3(bx, (a.k + RC - 1) / RC), dim3(256), 0, s, g, st, mask, a, lay)
 #define DFMI_SMODE(RC)                                        \
   switch (mode) {                                             \
     case MODE_APPLY: DFMI_SLAUNCH(RC, MODE_APPLY); break;      \
@@ -111,10 +111,10 @@ void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, co
   DFMI_HIP(hipGetLastError());
 }
 
-template void stencil_apply<float>(const Geom&, const float*, const uint8_t*, int, const ApplyArgs&, hipStream_t,
-                                   int*);
-template void stencil_apply<double>(const Geom&, const double*, const uint8_t*, int, const ApplyArgs&, hipStream_t,
-                                    int*);
+template void stencil_apply<float>(const Geom&, const float*, const uint8_t*, int, const ApplyArgs&, VLay,
+                                   hipStream_t, int*);
+template void stencil_apply<double>(const Geom&, const double*, const uint8_t*, int, const ApplyArgs&, VLay,
+                                    hipStream_t, int*);
 
 // ---- Galerkin product P^T A P in stencil form -------------------------------------------
 // Coarse node N, offset D: sum over fine f = 2N + e (weight w(e)) and fine neighbours
@@ -205,47 +205,47 @@ void stencil_mask(const Geom& g, double* st, const uint8_t* mask, hipStream_t s)
 
 // ---- transfer operators ------------------------------------------------------------------
 template <typename T>
-__global__ void k_restrict(Geom gf, const T* __restrict__ f, Geom gc, const uint8_t* __restrict__ cmask,
-                           T* __restrict__ c, const int* active) {
+__global__ void k_restrict(Geom gf, const T* __restrict__ f, VLay fl, Geom gc, const uint8_t* __restrict__ cmask,
+                           T* __restrict__ c, VLay cl, const int* active) {
   const long long Nc = gc.nodes(), Nf = gf.nodes();
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
   if (n >= Nc || (active && !active[r])) return;
   const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
-  const T* F = f + (long long)r * Nf * 3;
+  const T* F = f + (long long)r * fl.rs;
   double acc[3] = {0, 0, 0};
   for (int e = 0; e < 27; ++e) {
     const int ex = e / 9 - 1, ey = (e / 3) % 3 - 1, ez = e % 3 - 1;
     const int fi = 2 * I + ex, fj = 2 * J + ey, fk = 2 * K + ez;
     if (fi < 0 || fj < 0 || fk < 0 || fi >= gf.nx || fj >= gf.ny || fk >= gf.nz) continue;
     const double w = weight1(ex) * weight1(ey) * weight1(ez);
-    const T* v = F + (((long long)fi * gf.ny + fj) * gf.nz + fk) * 3;
+    const T* v = F + (((long long)fi * gf.ny + fj) * gf.nz + fk) * fl.ns;
     acc[0] += w * (double)v[0];
     acc[1] += w * (double)v[1];
     acc[2] += w * (double)v[2];
   }
   const uint8_t m = cmask[n];
-  T* C = c + ((long long)r * Nc + n) * 3;
+  T* C = c + n * cl.ns + (long long)r * cl.rs;
   for (int q = 0; q < 3; ++q) C[q] = live_bit(m, q) ? (T)acc[q] : T(0);
 }
 
 template <typename T>
-void restrict_full(const Geom& fine, const T* f, const Geom& coarse, const uint8_t* cmask, T* c, int k,
-                   const int* active, hipStream_t s) {
+void restrict_full(const Geom& fine, const T* f, VLay flay, const Geom& coarse, const uint8_t* cmask, T* c,
+                   VLay clay, int k, const int* active, hipStream_t s) {
   hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((coarse.nodes() + 255) / 256), k), dim3(256), 0, s, fine, f,
-                     coarse, cmask, c, active);
+                     flay, coarse, cmask, c, clay, active);
   DFMI_HIP(hipGetLastError());
 }
 
 template <typename T>
-__global__ void k_prolong_add(Geom gc, const T* __restrict__ c, Geom gf, const uint8_t* __restrict__ fmask,
-                              T* __restrict__ f, const int* active) {
+__global__ void k_prolong_add(Geom gc, const T* __restrict__ c, VLay cl, Geom gf, const uint8_t* __restrict__ fmask,
+                              T* __restrict__ f, VLay fl, const int* active) {
   const long long Nc = gc.nodes(), Nf = gf.nodes();
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
   if (n >= Nf || (active && !active[r])) return;
   const int k = (int)(n % gf.nz), j = (int)((n / gf.nz) % gf.ny), i = (int)(n / ((long long)gf.nz * gf.ny));
-  const T* C = c + (long long)r * Nc * 3;
+  const T* C = c + (long long)r * cl.rs;
   double acc[3] = {0, 0, 0};
   const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
   const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
@@ -253,34 +253,34 @@ __global__ void k_prolong_add(Geom gc, const T* __restrict__ c, Geom gf, const u
   for (int a = 0; a < ni; ++a)
     for (int b = 0; b < nj; ++b)
       for (int d = 0; d < nk; ++d) {
-        const T* v = C + (((long long)(i0 + a) * gc.ny + j0 + b) * gc.nz + k0 + d) * 3;
+        const T* v = C + (((long long)(i0 + a) * gc.ny + j0 + b) * gc.nz + k0 + d) * cl.ns;
         const double w = wi * wj * wk;
         acc[0] += w * (double)v[0];
         acc[1] += w * (double)v[1];
         acc[2] += w * (double)v[2];
       }
   const uint8_t m = fmask[n];
-  T* F = f + ((long long)r * Nf + n) * 3;
+  T* F = f + n * fl.ns + (long long)r * fl.rs;
   for (int q = 0; q < 3; ++q)
     if (live_bit(m, q)) F[q] = F[q] + (T)acc[q];
 }
 
 template <typename T>
-void prolong_add(const Geom& coarse, const T* c, const Geom& fine, const uint8_t* fmask, T* f, int k,
-                 const int* active, hipStream_t s) {
+void prolong_add(const Geom& coarse, const T* c, VLay clay, const Geom& fine, const uint8_t* fmask, T* f,
+                 VLay flay, int k, const int* active, hipStream_t s) {
   hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((fine.nodes() + 255) / 256), k), dim3(256), 0, s, coarse, c,
-                     fine, fmask, f, active);
+                     clay, fine, fmask, f, flay, active);
   DFMI_HIP(hipGetLastError());
 }
 
-template void restrict_full<float>(const Geom&, const float*, const Geom&, const uint8_t*, float*, int, const int*,
-                                   hipStream_t);
-template void restrict_full<double>(const Geom&, const double*, const Geom&, const uint8_t*, double*, int,
+template void restrict_full<float>(const Geom&, const float*, VLay, const Geom&, const uint8_t*, float*, VLay, int,
+                                   const int*, hipStream_t);
+template void restrict_full<double>(const Geom&, const double*, VLay, const Geom&, const uint8_t*, double*, VLay, int,
                                     const int*, hipStream_t);
-template void prolong_add<float>(const Geom&, const float*, const Geom&, const uint8_t*, float*, int, const int*,
-                                 hipStream_t);
-template void prolong_add<double>(const Geom&, const double*, const Geom&, const uint8_t*, double*, int, const int*,
-                                  hipStream_t);
+template void prolong_add<float>(const Geom&, const float*, VLay, const Geom&, const uint8_t*, float*, VLay, int,
+                                 const int*, hipStream_t);
+template void prolong_add<double>(const Geom&, const double*, VLay, const Geom&, const uint8_t*, double*, VLay, int,
+                                  const int*, hipStream_t);
 
 // ---- coarsest level: dense inverse of the live-DOF Galerkin matrix -------------------------
 __global__ void k_coarsest_assemble(Geom g, const double* __restrict__ st, const int* __restrict__ live_index, int n,
@@ -449,12 +449,14 @@ void dense_sweep_inverse(double* a, int n, hipStream_t s) {
 }
 
 template <typename T>
-__global__ void k_gather_live(const T* __restrict__ v, long long nodes, const int* __restrict__ live_dofs, int n,
+__global__ void k_gather_live(const T* __restrict__ v, VLay lay, const int* __restrict__ live_dofs, int n,
                               double* __restrict__ out, const int* active) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
   if (idx >= n) return;
-  out[(long long)r * n + idx] = (active && !active[r]) ? 0.0 : (double)v[(long long)r * nodes * 3 + live_dofs[idx]];
+  const int d = live_dofs[idx];
+  out[(long long)r * n + idx] =
+      (active && !active[r]) ? 0.0 : (double)v[(d / 3) * lay.ns + (long long)r * lay.rs + d % 3];
 }
 
 // out[r][j] = sum_i in[r][i] ainv[i][j]; 256 columns per block, 8 right-hand sides per block row
@@ -486,29 +488,30 @@ __global__ __launch_bounds__(256) void k_dense_rows(const double* __restrict__ i
 }
 
 template <typename T>
-__global__ void k_scatter_live(const double* __restrict__ in, long long nodes, const int* __restrict__ live_dofs,
-                               int n, T* __restrict__ v, const int* active) {
+__global__ void k_scatter_live(const double* __restrict__ in, VLay lay, const int* __restrict__ live_dofs, int n,
+                               T* __restrict__ v, const int* active) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
   if (idx >= n || (active && !active[r])) return;
-  v[(long long)r * nodes * 3 + live_dofs[idx]] = (T)in[(long long)r * n + idx];
+  const int d = live_dofs[idx];
+  v[(d / 3) * lay.ns + (long long)r * lay.rs + d % 3] = (T)in[(long long)r * n + idx];
 }
 
 template <typename T>
-void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out,
+void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out, VLay lay,
                     double* work_in, double* work_out, int k, const int* active, hipStream_t s) {
   DFMI_HIP(hipMemsetAsync(out, 0, sizeof(T) * (size_t)k * g.nodes() * 3, s));
   if (n == 0) return;
   const unsigned b = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_gather_live<T>, dim3(b, k), dim3(256), 0, s, rhs, g.nodes(), live_dofs, n, work_in, active);
+  hipLaunchKernelGGL(k_gather_live<T>, dim3(b, k), dim3(256), 0, s, rhs, lay, live_dofs, n, work_in, active);
   hipLaunchKernelGGL(k_dense_rows, dim3(b, (k + 7) / 8), dim3(256), 0, s, work_in, ainv, n, k, work_out);
-  hipLaunchKernelGGL(k_scatter_live<T>, dim3(b, k), dim3(256), 0, s, work_out, g.nodes(), live_dofs, n, out, active);
+  hipLaunchKernelGGL(k_scatter_live<T>, dim3(b, k), dim3(256), 0, s, work_out, lay, live_dofs, n, out, active);
   DFMI_HIP(hipGetLastError());
 }
 
-template void coarsest_solve<float>(const Geom&, const double*, const int*, int, const float*, float*, double*,
+template void coarsest_solve<float>(const Geom&, const double*, const int*, int, const float*, float*, VLay, double*,
                                     double*, int, const int*, hipStream_t);
-template void coarsest_solve<double>(const Geom&, const double*, const int*, int, const double*, double*, double*,
-                                     double*, int, const int*, hipStream_t);
+template void coarsest_solve<double>(const Geom&, const double*, const int*, int, const double*, double*, VLay,
+                                     double*, double*, int, const int*, hipStream_t);
 
 }  // namespace dfmi

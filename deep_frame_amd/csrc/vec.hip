@@ -102,24 +102,24 @@ void reduce_max(const double* partial, int nblocks, int k, double* out, hipStrea
 // first Chebyshev step from a zero iterate: x = c * Dinv * b
 template <typename T>
 __global__ void k_cheb_start(const T* __restrict__ b, const T* __restrict__ dinv, double c, T* __restrict__ x,
-                             long long nodes, const int* active) {
+                             long long nodes, const int* active, VLay lay) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
   if (i >= nodes * 3 || (active && !active[r])) return;
-  const long long o = (long long)r * nodes * 3 + i;
+  const long long o = (i / 3) * lay.ns + (long long)r * lay.rs + i % 3;
   x[o] = (T)c * dinv[i] * b[o];
 }
 
 template <typename T>
-void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active,
+void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active, VLay lay,
                 hipStream_t s) {
   hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * 3 + 255) / 256), k), dim3(256), 0, s, b, dinv, c, x,
-                     nodes, active);
+                     nodes, active, lay);
   DFMI_HIP(hipGetLastError());
 }
-template void cheb_start<float>(const float*, const float*, double, float*, long long, int, const int*,
+template void cheb_start<float>(const float*, const float*, double, float*, long long, int, const int*, VLay,
                                 hipStream_t);
-template void cheb_start<double>(const double*, const double*, double, double*, long long, int, const int*,
+template void cheb_start<double>(const double*, const double*, double, double*, long long, int, const int*, VLay,
                                  hipStream_t);
 
 // deterministic pseudo-random start vector in [-0.5, 0.5) on live DOFs (power iteration)
