@@ -11,102 +11,99 @@ namespace dfmi {
 __host__ __device__ inline bool live_bit(uint8_t m, int c) { return (m >> c) & 1; }
 __device__ inline double weight1(int t) { return t == 0 ? 1.0 : ((t == 1 || t == -1) ? 0.5 : 0.0); }
 
-template <typename T, int RC, int MODE>
-__global__ __launch_bounds__(256) void k_stencil_apply(Geom g, const T* __restrict__ st,
-                                                        const uint8_t* __restrict__ mask, ApplyArgs a, VLay lay) {
+// Thread per (node, right-hand side) with the right-hand side fastest: with the interleaved layout the
+// k threads of one node read one contiguous run per neighbour.  The node's 27 x 9 stencil entries are
+// staged once per workgroup in LDS and broadcast to its k threads.
+constexpr int SNT = 256;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restrict__ st,
+                                                        const uint8_t* __restrict__ mask, ApplyArgs a, VLay lay,
+                                                        int nodes_per_block) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sst = reinterpret_cast<T*>(smem_raw);  // [nodes_per_block][244]
+  constexpr int SP = 244;                    // padded row (243 entries)
   const long long N = g.nodes();
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r0 = blockIdx.y * RC;
-  if (n >= N) return;
-  const int l = (int)(n % g.nz), j = (int)((n / g.nz) % g.ny), i = (int)(n / ((long long)g.nz * g.ny));
-  const T* X = reinterpret_cast<const T*>(a.x);
-  bool on[RC];
-#pragma unroll
-  for (int rr = 0; rr < RC; ++rr) on[rr] = (r0 + rr < a.k) && (!a.active || a.active[r0 + rr]);
-  T acc[RC][3];
-#pragma unroll
-  for (int rr = 0; rr < RC; ++rr) acc[rr][0] = acc[rr][1] = acc[rr][2] = T(0);
+  const int k = a.k;
+  const long long n0 = (long long)blockIdx.x * nodes_per_block;
+  const int t = threadIdx.x;
+  for (int v = t; v < 243 * nodes_per_block; v += SNT) {
+    const int e = v / nodes_per_block, j = v - e * nodes_per_block;
+    const long long n = n0 + j;
+    sst[j * SP + e] = n < N ? st[(long long)e * N + n] : T(0);
+  }
+  __syncthreads();
+  const int j = t / k, r = t - (t / k) * k;
+  const long long n = n0 + j;
+  if (j >= nodes_per_block || n >= N) return;
+  if (a.active && !a.active[r]) return;
+  const int l = (int)(n % g.nz), jj = (int)((n / g.nz) % g.ny), i = (int)(n / ((long long)g.nz * g.ny));
+  const T* X = reinterpret_cast<const T*>(a.x) + (long long)r * lay.rs;
+  const T* s = sst + j * SP;
+  T acc0 = T(0), acc1 = T(0), acc2 = T(0);
   for (int d = 0; d < 27; ++d) {
-    const int mi = i + d / 9 - 1, mj = j + (d / 3) % 3 - 1, ml = l + d % 3 - 1;
+    const int mi = i + d / 9 - 1, mj = jj + (d / 3) % 3 - 1, ml = l + d % 3 - 1;
     if (mi < 0 || mj < 0 || ml < 0 || mi >= g.nx || mj >= g.ny || ml >= g.nz) continue;
     const long long m = ((long long)mi * g.ny + mj) * g.nz + ml;
     const uint8_t mm = mask[m];
-    T s[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) s[e] = st[((long long)d * 9 + e) * N + n];
-#pragma unroll
-    for (int rr = 0; rr < RC; ++rr) {
-      if (!on[rr]) continue;
-      const T* xm = X + m * lay.ns + (long long)(r0 + rr) * lay.rs;
-      const T x0 = live_bit(mm, 0) ? xm[0] : T(0);
-      const T x1 = live_bit(mm, 1) ? xm[1] : T(0);
-      const T x2 = live_bit(mm, 2) ? xm[2] : T(0);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) acc[rr][c] += s[3 * c] * x0 + s[3 * c + 1] * x1 + s[3 * c + 2] * x2;
-    }
+    const T* xm = X + m * lay.ns;
+    const T x0 = live_bit(mm, 0) ? xm[0] : T(0);
+    const T x1 = live_bit(mm, 1) ? xm[1] : T(0);
+    const T x2 = live_bit(mm, 2) ? xm[2] : T(0);
+    const T* sd = s + d * 9;
+    acc0 = fma(sd[0], x0, fma(sd[1], x1, fma(sd[2], x2, acc0)));
+    acc1 = fma(sd[3], x0, fma(sd[4], x1, fma(sd[5], x2, acc1)));
+    acc2 = fma(sd[6], x0, fma(sd[7], x1, fma(sd[8], x2, acc2)));
   }
+  const T acc[3] = {acc0, acc1, acc2};
   const uint8_t mn = mask[n];
+  T* Y = reinterpret_cast<T*>(a.y);
 #pragma unroll
-  for (int rr = 0; rr < RC; ++rr) {
-    if (!on[rr]) continue;
-    const int r = r0 + rr;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const long long o = n * lay.ns + (long long)r * lay.rs + c;
-      const bool lv = live_bit(mn, c);
-      T* Y = reinterpret_cast<T*>(a.y);
-      if (MODE == MODE_APPLY) {
-        Y[o] = lv ? acc[rr][c] : T(0);
-      } else if (MODE == MODE_RESID) {
-        Y[o] = lv ? reinterpret_cast<const T*>(a.b)[o] - acc[rr][c] : T(0);
-      } else if (MODE == MODE_CHEB) {
-        const T xv = lv ? X[o] : T(0);
-        const T xpv = a.xp ? reinterpret_cast<const T*>(a.xp)[o] : T(0);
-        const T bv = reinterpret_cast<const T*>(a.b)[o];
-        const T dv = reinterpret_cast<const T*>(a.dinv)[n * 3 + c];
-        Y[o] = lv ? xv + (T)a.c1 * (xv - xpv) + (T)a.c2 * dv * (bv - acc[rr][c]) : T(0);
-      }
+  for (int c = 0; c < 3; ++c) {
+    const long long o = n * lay.ns + (long long)r * lay.rs + c;
+    const bool lv = live_bit(mn, c);
+    if (MODE == MODE_APPLY) {
+      Y[o] = lv ? acc[c] : T(0);
+    } else if (MODE == MODE_RESID) {
+      Y[o] = lv ? reinterpret_cast<const T*>(a.b)[o] - acc[c] : T(0);
+    } else if (MODE == MODE_CHEB) {
+      const T xv = lv ? X[n * lay.ns + c] : T(0);
+      const T xpv = a.xp ? reinterpret_cast<const T*>(a.xp)[o] : T(0);
+      const T bv = reinterpret_cast<const T*>(a.b)[o];
+      const T dv = reinterpret_cast<const T*>(a.dinv)[n * 3 + c];
+      Y[o] = lv ? xv + (T)a.c1 * (xv - xpv) + (T)a.c2 * dv * (bv - acc[c]) : T(0);
     }
   }
 }
 
-int stencil_blocks(const Geom& g, int k) { return (int)((g.nodes() + 255) / 256); }
+// nodes per workgroup: fill the 256 threads, with the staged stencils kept within 48 KB of LDS
+template <typename T>
+static int stencil_npb(int k) {
+  const int cap = (48 * 1024) / (int)(244 * sizeof(T));
+  return std::max(1, std::min(SNT / std::max(1, k), cap));
+}
+
+int stencil_blocks(const Geom& g, int k) {
+  const int npb = stencil_npb<float>(k);
+  return (int)((g.nodes() + npb - 1) / npb);
+}
 
 template <typename T>
 void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, VLay lay,
                    hipStream_t s, int* nblocks_out) {
-  const unsigned bx = (unsigned)((g.nodes() + 255) / 256);
+  if (a.k > SNT) throw std::runtime_error("stencil_apply: at most 256 right-hand sides per call");
+  const int npb = stencil_npb<T>(a.k);
+  const unsigned bx = (unsigned)((g.nodes() + npb - 1) / npb);
+  const size_t lds = sizeof(T) * 244 * (size_t)npb;
   if (nblocks_out) *nblocks_out = bx;
-#define DFMI_SLAUNCH(RC, M) \
-  hipLaunchKernelGGL((k_stencil_apply<T, RC, M>), dim3(bx, (a.k + RC - 1) / RC), dim3(256), 0, s, g, st, mask, a, lay)
-#define DFMI_SMODE(RC)                                        \
-  switch (mode) {                                             \
-    case MODE_APPLY: DFMI_SLAUNCH(RC, MODE_APPLY); break;      \
-    case MODE_RESID: DFMI_SLAUNCH(RC, MODE_RESID); break;      \
-    case MODE_CHEB: DFMI_SLAUNCH(RC, MODE_CHEB); break;        \
-    default: throw std::runtime_error("stencil_apply: bad mode"); \
+#define DFMI_SLAUNCH(M) \
+  hipLaunchKernelGGL((k_stencil_apply<T, M>), dim3(bx), dim3(SNT), lds, s, g, st, mask, a, lay, npb)
+  switch (mode) {
+    case MODE_APPLY: DFMI_SLAUNCH(MODE_APPLY); break;
+    case MODE_RESID: DFMI_SLAUNCH(MODE_RESID); break;
+    case MODE_CHEB: DFMI_SLAUNCH(MODE_CHEB); break;
+    default: throw std::runtime_error("stencil_apply: bad mode");
   }
-  // right-hand sides per thread: each stencil block is read once per chunk of rc columns
-  const int chunks = (a.k + 15) / 16;
-  const int want = (a.k + chunks - 1) / chunks;
-  if (want <= 1) {
-    DFMI_SMODE(1)
-  } else if (want <= 2) {
-    DFMI_SMODE(2)
-  } else if (want <= 4) {
-    DFMI_SMODE(4)
-  } else if (want <= 6) {
-    DFMI_SMODE(6)
-  } else if (want <= 8) {
-    DFMI_SMODE(8)
-  } else if (want <= 11) {
-    DFMI_SMODE(11)
-  } else if (want <= 14) {
-    DFMI_SMODE(14)
-  } else {
-    DFMI_SMODE(16)
-  }
-#undef DFMI_SMODE
 #undef DFMI_SLAUNCH
   DFMI_HIP(hipGetLastError());
 }
@@ -204,13 +201,27 @@ void stencil_mask(const Geom& g, double* st, const uint8_t* mask, hipStream_t s)
 }
 
 // ---- transfer operators ------------------------------------------------------------------
+// thread id -> (node, column): column fastest for interleaved outputs, node fastest otherwise
+__device__ inline bool node_col(long long tid, long long nodes, int k, bool col_fast, long long& n, int& r) {
+  if (tid >= nodes * k) return false;
+  if (col_fast) {
+    n = tid / k;
+    r = (int)(tid - n * k);
+  } else {
+    r = (int)(tid / nodes);
+    n = tid - (long long)r * nodes;
+  }
+  return true;
+}
+
 template <typename T>
 __global__ void k_restrict(Geom gf, const T* __restrict__ f, VLay fl, Geom gc, const uint8_t* __restrict__ cmask,
-                           T* __restrict__ c, VLay cl, const int* active) {
-  const long long Nc = gc.nodes(), Nf = gf.nodes();
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (n >= Nc || (active && !active[r])) return;
+                           T* __restrict__ c, VLay cl, int k, const int* active) {
+  const long long Nc = gc.nodes();
+  long long n;
+  int r;
+  if (!node_col((long long)blockIdx.x * blockDim.x + threadIdx.x, Nc, k, cl.rs == 3 && k > 1, n, r)) return;
+  if (active && !active[r]) return;
   const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
   const T* F = f + (long long)r * fl.rs;
   double acc[3] = {0, 0, 0};
@@ -232,24 +243,25 @@ __global__ void k_restrict(Geom gf, const T* __restrict__ f, VLay fl, Geom gc, c
 template <typename T>
 void restrict_full(const Geom& fine, const T* f, VLay flay, const Geom& coarse, const uint8_t* cmask, T* c,
                    VLay clay, int k, const int* active, hipStream_t s) {
-  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((coarse.nodes() + 255) / 256), k), dim3(256), 0, s, fine, f,
-                     flay, coarse, cmask, c, clay, active);
+  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((coarse.nodes() * k + 255) / 256)), dim3(256), 0, s, fine, f,
+                     flay, coarse, cmask, c, clay, k, active);
   DFMI_HIP(hipGetLastError());
 }
 
 template <typename T>
 __global__ void k_prolong_add(Geom gc, const T* __restrict__ c, VLay cl, Geom gf, const uint8_t* __restrict__ fmask,
-                              T* __restrict__ f, VLay fl, const int* active) {
-  const long long Nc = gc.nodes(), Nf = gf.nodes();
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (n >= Nf || (active && !active[r])) return;
-  const int k = (int)(n % gf.nz), j = (int)((n / gf.nz) % gf.ny), i = (int)(n / ((long long)gf.nz * gf.ny));
+                              T* __restrict__ f, VLay fl, int k, const int* active) {
+  const long long Nf = gf.nodes();
+  long long n;
+  int r;
+  if (!node_col((long long)blockIdx.x * blockDim.x + threadIdx.x, Nf, k, fl.rs == 3 && k > 1, n, r)) return;
+  if (active && !active[r]) return;
+  const int kz = (int)(n % gf.nz), j = (int)((n / gf.nz) % gf.ny), i = (int)(n / ((long long)gf.nz * gf.ny));
   const T* C = c + (long long)r * cl.rs;
   double acc[3] = {0, 0, 0};
-  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-  const double wi = (i & 1) ? 0.5 : 1.0, wj = (j & 1) ? 0.5 : 1.0, wk = (k & 1) ? 0.5 : 1.0;
+  const int i0 = i >> 1, j0 = j >> 1, k0 = kz >> 1;
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (kz & 1) ? 2 : 1;
+  const double wi = (i & 1) ? 0.5 : 1.0, wj = (j & 1) ? 0.5 : 1.0, wk = (kz & 1) ? 0.5 : 1.0;
   for (int a = 0; a < ni; ++a)
     for (int b = 0; b < nj; ++b)
       for (int d = 0; d < nk; ++d) {
@@ -268,8 +280,8 @@ __global__ void k_prolong_add(Geom gc, const T* __restrict__ c, VLay cl, Geom gf
 template <typename T>
 void prolong_add(const Geom& coarse, const T* c, VLay clay, const Geom& fine, const uint8_t* fmask, T* f,
                  VLay flay, int k, const int* active, hipStream_t s) {
-  hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((fine.nodes() + 255) / 256), k), dim3(256), 0, s, coarse, c,
-                     clay, fine, fmask, f, flay, active);
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((fine.nodes() * k + 255) / 256)), dim3(256), 0, s, coarse, c,
+                     clay, fine, fmask, f, flay, k, active);
   DFMI_HIP(hipGetLastError());
 }
 

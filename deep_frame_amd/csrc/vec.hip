@@ -102,19 +102,29 @@ void reduce_max(const double* partial, int nblocks, int k, double* out, hipStrea
 // first Chebyshev step from a zero iterate: x = c * Dinv * b
 template <typename T>
 __global__ void k_cheb_start(const T* __restrict__ b, const T* __restrict__ dinv, double c, T* __restrict__ x,
-                             long long nodes, const int* active, VLay lay) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (i >= nodes * 3 || (active && !active[r])) return;
-  const long long o = (i / 3) * lay.ns + (long long)r * lay.rs + i % 3;
-  x[o] = (T)c * dinv[i] * b[o];
+                             long long nodes, int k, const int* active, VLay lay) {
+  // one thread per stored value, in memory order; recover (node, column, component) from the layout
+  const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nodes * 3 * k) return;
+  long long n;
+  int r;
+  if (lay.rs == 3) {  // interleaved: o = n * 3k + 3r + c
+    n = o / lay.ns;
+    r = (int)((o - n * lay.ns) / 3);
+  } else {  // column-major: o = r * 3N + 3n + c
+    r = (int)(o / lay.rs);
+    n = (o - (long long)r * lay.rs) / 3;
+  }
+  if (active && !active[r]) return;
+  const int comp = (int)(o % 3);
+  x[o] = (T)c * dinv[n * 3 + comp] * b[o];
 }
 
 template <typename T>
 void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active, VLay lay,
                 hipStream_t s) {
-  hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * 3 + 255) / 256), k), dim3(256), 0, s, b, dinv, c, x,
-                     nodes, active, lay);
+  hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * 3 * k + 255) / 256)), dim3(256), 0, s, b, dinv, c, x,
+                     nodes, k, active, lay);
   DFMI_HIP(hipGetLastError());
 }
 template void cheb_start<float>(const float*, const float*, double, float*, long long, int, const int*, VLay,
