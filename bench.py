@@ -105,14 +105,21 @@ def main():
         mg = GeometricMultigrid(cfg["shape"], (1, 1, 1), ke, {"precision": "float32", "levels": cfg["levels"]})
         mg.update(moduli)
         slot = mg.hierarchy("apply", fixed)["slot"]
-        for precision in (1, 0):
-            ms = c_double()
-            mg._stream()
-            _native.call("dfmi_time_apply", mg.handle, slot, precision, rhs.shape[0], args.apply_reps, byref(ms))
-            size = 4 if precision else 8
-            gbs = (2.0 * ndof * rhs.shape[0] * size + nel * size) / (ms.value * 1e-3) / 1e9
-            print(json.dumps({"kernel": "k_fine_apply", "precision": "fp32" if precision else "fp64",
-                              "rhs": rhs.shape[0], "ms_per_launch": ms.value, "algorithmic_GBs": gbs}))
+        for level in (0, 1):
+            for precision in (1, 0):
+                ms = c_double()
+                mg._stream()
+                _native.call("dfmi_time_apply", mg.handle, slot, level, precision, rhs.shape[0], args.apply_reps,
+                             byref(ms))
+                size = 4 if precision else 8
+                nodes = int(np.prod([n // 2 ** level + 1 for n in mg.padded]))
+                if level == 0:
+                    alg = 2.0 * ndof * rhs.shape[0] * size + nel * size
+                else:  # vectors in and out plus the 243-entry stencil per node
+                    alg = 2.0 * 3 * nodes * rhs.shape[0] * size + 243.0 * nodes * size
+                print(json.dumps({"kernel": "k_fine_apply" if level == 0 else "k_stencil_apply", "level": level,
+                                  "precision": "fp32" if precision else "fp64", "rhs": rhs.shape[0],
+                                  "ms_per_launch": ms.value, "algorithmic_GBs": alg / (ms.value * 1e-3) / 1e9}))
         mg.close()
         # plain copy of a known size: calibrates the FETCH_SIZE / WRITE_SIZE counters
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
@@ -127,7 +134,7 @@ def main():
     slot = mg.hierarchies[("bench", args.warmup + args.steps - 1)]["slot"]
     ms = c_double()
     mg._stream()
-    _native.call("dfmi_time_apply", mg.handle, slot, 1, rhs.shape[0], args.apply_reps, byref(ms))
+    _native.call("dfmi_time_apply", mg.handle, slot, 0, 1, rhs.shape[0], args.apply_reps, byref(ms))
     k = rhs.shape[0]
     alg_bytes = 2.0 * ndof * k * 4 + nel * 4
     achieved = alg_bytes / (ms.value * 1e-3) / 1e9

@@ -43,7 +43,7 @@ SIGNATURES = {
     "dfmi_level_apply": ([c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32], c_int32),
     "dfmi_precondition": ([c_void_p, c_int32, c_void_p, c_void_p, c_int32], c_int32),
     "dfmi_transfer": ([c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int32], c_int32),
-    "dfmi_time_apply": ([c_void_p, c_int32, c_int32, c_int32, c_int32, POINTER(c_double)], c_int32),
+    "dfmi_time_apply": ([c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(c_double)], c_int32),
     "dfmi_diagnostics": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
     "dfmi_simp": ([c_void_p, c_void_p, c_int64, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p], c_int32),
     "dfmi_filter_create": ([POINTER(c_int32), POINTER(c_double), c_double, POINTER(c_uint8), POINTER(c_uint8),

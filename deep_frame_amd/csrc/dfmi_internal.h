@@ -37,6 +37,18 @@ inline Geom make_geom(int ex, int ey, int ez) { return Geom{ex, ey, ez, ex + 1, 
 // The fine level is column-major (one contiguous column per right-hand side, as the flat DOF
 // vectors of the interface); coarse levels interleave the right-hand sides per node so that a
 // stencil gather of one neighbour is a single contiguous run for all columns of a thread.
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Remap the hardware
+// block index so that every XCD works on one contiguous range of logical blocks (neighbouring
+// tiles share halo data in the same L2).
+__device__ inline unsigned xcd_block(unsigned b, unsigned nblocks) {
+  constexpr unsigned XCD = 8;
+  const unsigned per = (nblocks + XCD - 1) / XCD;
+  const unsigned x = b % XCD, i = b / XCD;
+  const unsigned full = nblocks % XCD == 0 ? XCD : nblocks % XCD;  // XCDs that get `per` blocks
+  // XCDs [0, full) own `per` blocks each, the others own per - 1
+  return x < full ? x * per + i : full * per + (x - full) * (per - 1) + i;
+}
+
 struct VLay {
   long long ns, rs;
 };

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
 
   const int r = blockIdx.y;
   if (a.active && !a.active[r]) return;
-  int tile = blockIdx.x;
+  int tile = (int)xcd_block(blockIdx.x, gridDim.x);
   const int tzi = tile % tiles_z;
   tile /= tiles_z;
   const int tyi = tile % tiles_y;

@@ -911,13 +911,16 @@ int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, c
   })
 }
 
-int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t precision, int32_t k, int32_t reps, double* ms_per_launch) {
+int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision, int32_t k, int32_t reps,
+                    double* ms_per_launch) {
   DFMI_GUARD({
     Handle& h = mg->h;
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
     if (k < 1 || reps < 1) throw std::invalid_argument("k and reps must be positive");
     Hierarchy& H = h.slots[slot];
-    const long long n = (long long)k * col(h.pad);
+    if (level < 0 || level >= (int)H.levels.size() - 1) throw std::invalid_argument("level out of range");
+    Level& L = *H.levels[level];
+    const long long n = (long long)k * col(L.g);
     const size_t el = precision == 0 ? sizeof(double) : sizeof(float);
     DevBuf xb, yb;
     xb.ensure(el * n);
@@ -929,7 +932,13 @@ int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t precision, int32_t k, int
     a.y = yb.p;
     a.k = k;
     auto launch = [&]() {
-      if (precision == 0)
+      if (level > 0 && precision == 0)
+        stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, interleaved(k), h.stream,
+                              nullptr);
+      else if (level > 0)
+        stencil_apply<float>(L.g, L.st32.as<float>(), L.mask.as<uint8_t>(), MODE_APPLY, a, interleaved(k), h.stream,
+                             nullptr);
+      else if (precision == 0)
         fine_apply<double>(h.pad, h.m64.as<double>(), H.levels[0]->mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream,
                            nullptr);
       else

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
   constexpr int SP = 244;                    // padded row (243 entries)
   const long long N = g.nodes();
   const int k = a.k;
-  const long long n0 = (long long)blockIdx.x * nodes_per_block;
+  const long long n0 = (long long)xcd_block(blockIdx.x, gridDim.x) * nodes_per_block;
   const int t = threadIdx.x;
   for (int v = t; v < 243 * nodes_per_block; v += SNT) {
     const int e = v / nodes_per_block, j = v - e * nodes_per_block;
@@ -220,7 +220,9 @@ __global__ void k_restrict(Geom gf, const T* __restrict__ f, VLay fl, Geom gc, c
   const long long Nc = gc.nodes();
   long long n;
   int r;
-  if (!node_col((long long)blockIdx.x * blockDim.x + threadIdx.x, Nc, k, cl.rs == 3 && k > 1, n, r)) return;
+  if (!node_col((long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, Nc, k, cl.rs == 3 && k > 1,
+                n, r))
+    return;
   if (active && !active[r]) return;
   const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
   const T* F = f + (long long)r * fl.rs;
@@ -254,7 +256,9 @@ __global__ void k_prolong_add(Geom gc, const T* __restrict__ c, VLay cl, Geom gf
   const long long Nf = gf.nodes();
   long long n;
   int r;
-  if (!node_col((long long)blockIdx.x * blockDim.x + threadIdx.x, Nf, k, fl.rs == 3 && k > 1, n, r)) return;
+  if (!node_col((long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, Nf, k, fl.rs == 3 && k > 1,
+                n, r))
+    return;
   if (active && !active[r]) return;
   const int kz = (int)(n % gf.nz), j = (int)((n / gf.nz) % gf.ny), i = (int)(n / ((long long)gf.nz * gf.ny));
   const T* C = c + (long long)r * cl.rs;

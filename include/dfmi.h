@@ -109,9 +109,11 @@ int dfmi_precondition(dfmi_mg* mg, int32_t slot, const double* r_dev, double* z_
 int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, const double* in_dev,
                   double* out_dev, int32_t k);
 
-/* Average duration (ms, HIP events on the handle's stream) of `reps` launches of the fine-level
- * operator kernel on k columns of a built slot, precision 0 = FP64, 1 = FP32.  For benchmarks. */
-int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t precision, int32_t k, int32_t reps, double* ms_per_launch);
+/* Average duration (ms, HIP events on the handle's stream) of `reps` launches of the operator kernel of
+ * `level` (0 = matrix-free fine level, >= 1 = Galerkin stencil level, not the coarsest) on k columns of a
+ * built slot, precision 0 = FP64, 1 = FP32.  For benchmarks. */
+int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision, int32_t k, int32_t reps,
+                    double* ms_per_launch);
 
 /* Diagnostics of a built slot: lambda_max per smoothed level and the coarsest live DOF count. */
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs);
