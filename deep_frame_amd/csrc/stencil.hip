@@ -11,76 +11,96 @@ namespace dfmi {
 __host__ __device__ inline bool live_bit(uint8_t m, int c) { return (m >> c) & 1; }
 __device__ inline double weight1(int t) { return t == 0 ? 1.0 : ((t == 1 || t == -1) ? 0.5 : 0.0); }
 
-// Thread per (node, right-hand side) with the right-hand side fastest: with the interleaved layout the
-// k threads of one node read one contiguous run per neighbour.  The node's 27 x 9 stencil entries are
-// staged once per workgroup in LDS and broadcast to its k threads.
+// Thread per (node, pair of right-hand sides), right-hand sides fastest: with the interleaved layout the
+// threads of one node read one contiguous run per neighbour.  The node's 27 x 9 stencil entries are
+// staged once per workgroup in LDS and broadcast to its threads.  No bounds or mask tests in the
+// gather: stencil entries towards missing or dead neighbours are zero (Galerkin product and
+// stencil_mask) and level vectors are zero on dead DOFs, so a neighbour index that runs off a grid
+// line only multiplies a zero; it is clamped into the array.
 constexpr int SNT = 256;
+constexpr int SP = 244;  // padded LDS row of 243 stencil entries
 
-template <typename T, int MODE>
+template <typename T, int RP, int MODE>
 __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restrict__ st,
                                                         const uint8_t* __restrict__ mask, ApplyArgs a, VLay lay,
-                                                        int nodes_per_block) {
+                                                        int nodes_per_block, int groups) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sst = reinterpret_cast<T*>(smem_raw);  // [nodes_per_block][244]
-  constexpr int SP = 244;                    // padded row (243 entries)
-  const long long N = g.nodes();
-  const int k = a.k;
-  const long long n0 = (long long)xcd_block(blockIdx.x, gridDim.x) * nodes_per_block;
+  T* sst = reinterpret_cast<T*>(smem_raw);  // [nodes_per_block][SP]
+  const int N = (int)g.nodes();
+  const int n0 = (int)xcd_block(blockIdx.x, gridDim.x) * nodes_per_block;
   const int t = threadIdx.x;
   for (int v = t; v < 243 * nodes_per_block; v += SNT) {
     const int e = v / nodes_per_block, j = v - e * nodes_per_block;
-    const long long n = n0 + j;
+    const int n = n0 + j;
     sst[j * SP + e] = n < N ? st[(long long)e * N + n] : T(0);
   }
   __syncthreads();
-  const int j = t / k, r = t - (t / k) * k;
-  const long long n = n0 + j;
+  const int j = t / groups, grp = t - (t / groups) * groups;
+  const int n = n0 + j;
   if (j >= nodes_per_block || n >= N) return;
-  if (a.active && !a.active[r]) return;
-  const int l = (int)(n % g.nz), jj = (int)((n / g.nz) % g.ny), i = (int)(n / ((long long)g.nz * g.ny));
-  const T* X = reinterpret_cast<const T*>(a.x) + (long long)r * lay.rs;
+  const int r0 = grp * RP;
+  bool on[RP];
+#pragma unroll
+  for (int q = 0; q < RP; ++q) on[q] = r0 + q < a.k && (!a.active || a.active[r0 + q]);
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < RP; ++q) any = any || on[q];
+  if (!any) return;
+  const int ns = (int)lay.ns, rs = (int)lay.rs;
+  const T* X = reinterpret_cast<const T*>(a.x);
   const T* s = sst + j * SP;
-  T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+  T acc[RP][3];
+#pragma unroll
+  for (int q = 0; q < RP; ++q) acc[q][0] = acc[q][1] = acc[q][2] = T(0);
+  const int pz = 1, py = g.nz, px = g.ny * g.nz;
+#pragma unroll 3
   for (int d = 0; d < 27; ++d) {
-    const int mi = i + d / 9 - 1, mj = jj + (d / 3) % 3 - 1, ml = l + d % 3 - 1;
-    if (mi < 0 || mj < 0 || ml < 0 || mi >= g.nx || mj >= g.ny || ml >= g.nz) continue;
-    const long long m = ((long long)mi * g.ny + mj) * g.nz + ml;
-    const uint8_t mm = mask[m];
-    const T* xm = X + m * lay.ns;
-    const T x0 = live_bit(mm, 0) ? xm[0] : T(0);
-    const T x1 = live_bit(mm, 1) ? xm[1] : T(0);
-    const T x2 = live_bit(mm, 2) ? xm[2] : T(0);
+    int m = n + (d / 9 - 1) * px + ((d / 3) % 3 - 1) * py + (d % 3 - 1) * pz;
+    m = min(max(m, 0), N - 1);
     const T* sd = s + d * 9;
-    acc0 = fma(sd[0], x0, fma(sd[1], x1, fma(sd[2], x2, acc0)));
-    acc1 = fma(sd[3], x0, fma(sd[4], x1, fma(sd[5], x2, acc1)));
-    acc2 = fma(sd[6], x0, fma(sd[7], x1, fma(sd[8], x2, acc2)));
+    const T s0 = sd[0], s1 = sd[1], s2 = sd[2], s3 = sd[3], s4 = sd[4], s5 = sd[5], s6 = sd[6], s7 = sd[7], s8 = sd[8];
+    const T* xm = X + m * ns;
+#pragma unroll
+    for (int q = 0; q < RP; ++q) {
+      const int col = min(r0 + q, a.k - 1) * rs;  // a missing odd column re-reads the last one
+      const T x0 = xm[col], x1 = xm[col + 1], x2 = xm[col + 2];
+      acc[q][0] = fma(s0, x0, fma(s1, x1, fma(s2, x2, acc[q][0])));
+      acc[q][1] = fma(s3, x0, fma(s4, x1, fma(s5, x2, acc[q][1])));
+      acc[q][2] = fma(s6, x0, fma(s7, x1, fma(s8, x2, acc[q][2])));
+    }
   }
-  const T acc[3] = {acc0, acc1, acc2};
   const uint8_t mn = mask[n];
   T* Y = reinterpret_cast<T*>(a.y);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const long long o = n * lay.ns + (long long)r * lay.rs + c;
-    const bool lv = live_bit(mn, c);
-    if (MODE == MODE_APPLY) {
-      Y[o] = lv ? acc[c] : T(0);
-    } else if (MODE == MODE_RESID) {
-      Y[o] = lv ? reinterpret_cast<const T*>(a.b)[o] - acc[c] : T(0);
-    } else if (MODE == MODE_CHEB) {
-      const T xv = lv ? X[n * lay.ns + c] : T(0);
-      const T xpv = a.xp ? reinterpret_cast<const T*>(a.xp)[o] : T(0);
-      const T bv = reinterpret_cast<const T*>(a.b)[o];
-      const T dv = reinterpret_cast<const T*>(a.dinv)[n * 3 + c];
-      Y[o] = lv ? xv + (T)a.c1 * (xv - xpv) + (T)a.c2 * dv * (bv - acc[c]) : T(0);
+  for (int q = 0; q < RP; ++q) {
+    if (!on[q]) continue;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int o = n * ns + (r0 + q) * rs + c;
+      const bool lv = live_bit(mn, c);
+      if (MODE == MODE_APPLY) {
+        Y[o] = lv ? acc[q][c] : T(0);
+      } else if (MODE == MODE_RESID) {
+        Y[o] = lv ? reinterpret_cast<const T*>(a.b)[o] - acc[q][c] : T(0);
+      } else if (MODE == MODE_CHEB) {
+        const T xv = X[o];
+        const T xpv = a.xp ? reinterpret_cast<const T*>(a.xp)[o] : T(0);
+        const T bv = reinterpret_cast<const T*>(a.b)[o];
+        const T dv = reinterpret_cast<const T*>(a.dinv)[n * 3 + c];
+        Y[o] = lv ? xv + (T)a.c1 * (xv - xpv) + (T)a.c2 * dv * (bv - acc[q][c]) : T(0);
+      }
     }
   }
 }
 
-// nodes per workgroup: fill the 256 threads, with the staged stencils kept within 48 KB of LDS
+// right-hand sides per thread and nodes per workgroup (staged stencils kept within 48 KB of LDS)
+static int stencil_rp(int k) { return k >= 4 ? 2 : 1; }
+
 template <typename T>
 static int stencil_npb(int k) {
-  const int cap = (48 * 1024) / (int)(244 * sizeof(T));
-  return std::max(1, std::min(SNT / std::max(1, k), cap));
+  const int groups = (k + stencil_rp(k) - 1) / stencil_rp(k);
+  const int cap = (48 * 1024) / (int)(SP * sizeof(T));
+  return std::max(1, std::min(SNT / std::max(1, groups), cap));
 }
 
 int stencil_blocks(const Geom& g, int k) {
@@ -91,19 +111,29 @@ int stencil_blocks(const Geom& g, int k) {
 template <typename T>
 void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, const ApplyArgs& a, VLay lay,
                    hipStream_t s, int* nblocks_out) {
-  if (a.k > SNT) throw std::runtime_error("stencil_apply: at most 256 right-hand sides per call");
+  const int rp = stencil_rp(a.k), groups = (a.k + rp - 1) / rp;
+  if (groups > SNT) throw std::runtime_error("stencil_apply: too many right-hand sides per call");
+  if (g.nodes() * 3 * (long long)a.k > (long long)INT32_MAX)
+    throw std::runtime_error("stencil_apply: level too large for 32-bit offsets");
   const int npb = stencil_npb<T>(a.k);
   const unsigned bx = (unsigned)((g.nodes() + npb - 1) / npb);
-  const size_t lds = sizeof(T) * 244 * (size_t)npb;
+  const size_t lds = sizeof(T) * SP * (size_t)npb;
   if (nblocks_out) *nblocks_out = bx;
-#define DFMI_SLAUNCH(M) \
-  hipLaunchKernelGGL((k_stencil_apply<T, M>), dim3(bx), dim3(SNT), lds, s, g, st, mask, a, lay, npb)
-  switch (mode) {
-    case MODE_APPLY: DFMI_SLAUNCH(MODE_APPLY); break;
-    case MODE_RESID: DFMI_SLAUNCH(MODE_RESID); break;
-    case MODE_CHEB: DFMI_SLAUNCH(MODE_CHEB); break;
-    default: throw std::runtime_error("stencil_apply: bad mode");
+#define DFMI_SLAUNCH(RP, M) \
+  hipLaunchKernelGGL((k_stencil_apply<T, RP, M>), dim3(bx), dim3(SNT), lds, s, g, st, mask, a, lay, npb, groups)
+#define DFMI_SMODE(RP)                                                \
+  switch (mode) {                                                     \
+    case MODE_APPLY: DFMI_SLAUNCH(RP, MODE_APPLY); break;              \
+    case MODE_RESID: DFMI_SLAUNCH(RP, MODE_RESID); break;              \
+    case MODE_CHEB: DFMI_SLAUNCH(RP, MODE_CHEB); break;                \
+    default: throw std::runtime_error("stencil_apply: bad mode");     \
   }
+  if (rp == 2) {
+    DFMI_SMODE(2)
+  } else {
+    DFMI_SMODE(1)
+  }
+#undef DFMI_SMODE
 #undef DFMI_SLAUNCH
   DFMI_HIP(hipGetLastError());
 }
