@@ -93,7 +93,9 @@ def default_settings():
 
 
 def ptr(tensor):
-    """Device pointer of a contiguous torch tensor."""
+    """Device pointer of a contiguous torch tensor that lives on the GPU (host memory is refused)."""
+    if tensor.device.type != "cuda":
+        raise ValueError(f"expected a GPU tensor, got one on {tensor.device}")
     if not tensor.is_contiguous():
         raise ValueError("tensor must be contiguous")
     return c_void_p(tensor.data_ptr())

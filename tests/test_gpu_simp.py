@@ -26,19 +26,19 @@ def test_filter_and_pullback_match_oracle(shape, spacing, radius, torch_gpu):
     x = rng.uniform(size=allowed.size)
     s = rng.normal(size=allowed.size)
     d = rng.uniform(0.5, 1.5, size=allowed.size)
-    out = f.filtered(torch.as_tensor(x)).cpu().numpy()
+    out = f.filtered(torch.as_tensor(x, device="cuda")).cpu().numpy()
     assert np.max(np.abs(out - ref.filtered(x))) < 1e-12
-    back = f.pullback(torch.as_tensor(s), torch.as_tensor(d)).cpu().numpy()
+    back = f.pullback(torch.as_tensor(s, device="cuda"), torch.as_tensor(d, device="cuda")).cpu().numpy()
     exp = ref.pullback(s, d)
     assert np.max(np.abs(back - exp)) < 1e-12 * np.max(np.abs(exp))
-    assert f.volume(torch.as_tensor(x)) == pytest.approx(ref.filtered(x).sum(), rel=1e-12)
+    assert f.volume(torch.as_tensor(x, device="cuda")) == pytest.approx(ref.filtered(x).sum(), rel=1e-12)
     f.close()
 
 
 def test_simp_interpolation(torch_gpu):
     from deep_frame_amd.simp import simp
     rho = np.random.default_rng(0).uniform(1e-3, 1, 1000)
-    e, de = simp(torch_gpu.as_tensor(rho), 2.0, 3.0, 1e-9, active=rho > 0.01)
+    e, de = simp(torch_gpu.as_tensor(rho, device="cuda"), 2.0, 3.0, 1e-9, active=rho > 0.01)
     ref_e, ref_de = hex8.simp(rho, 2.0, 3.0, 1e-9)
     ref_e[rho <= 0.01] = 0
     ref_de[rho <= 0.01] = 0
@@ -59,9 +59,9 @@ def test_oc_update_matches_oracle(torch_gpu):
     target = 0.4 * allowed.size
     exp = oc.oc_update(x, dc, dv, ref.free.ravel(), lambda z: ref.filtered(z).sum(), target, 1e-3, 0.2)
     t = torch_gpu
-    out = f.oc_update(t.as_tensor(x), t.as_tensor(dc), t.as_tensor(dv), target, 1e-3, 0.2).cpu().numpy()
+    out = f.oc_update(t.as_tensor(x, device="cuda"), t.as_tensor(dc, device="cuda"), t.as_tensor(dv, device="cuda"), target, 1e-3, 0.2).cpu().numpy()
     assert np.max(np.abs(out - exp)) < 1e-6
-    assert f.volume(t.as_tensor(out)) <= target + 1e-9
+    assert f.volume(t.as_tensor(out, device="cuda")) <= target + 1e-9
     f.close()
 
 
@@ -80,11 +80,12 @@ def test_mma_step_matches_oracle(iteration, torch_gpu):
     exp, elow, eupp, _, _ = mma.mma_step(iteration, x, old1, old2, low, upp, 0.0, 1.0, 1.0, df0, g, dg[None, :],
                                          move=0.1, scale=100.0)
     t = torch_gpu
-    st = MMAState(t.as_tensor(x))
+    dev = lambda a: t.as_tensor(a, device="cuda")
+    st = MMAState(dev(x))
     st.iteration = iteration
-    st.old1, st.old2 = t.as_tensor(old1), t.as_tensor(old2)
-    st.low, st.upp = t.as_tensor(low), t.as_tensor(upp)
-    out = st.step(t.as_tensor(x), 1.0, t.as_tensor(df0), g, t.as_tensor(dg), move=0.1, scale=100.0).cpu().numpy()
+    st.old1, st.old2 = dev(old1), dev(old2)
+    st.low, st.upp = dev(low), dev(upp)
+    out = st.step(dev(x), 1.0, dev(df0), g, dev(dg), move=0.1, scale=100.0).cpu().numpy()
     assert np.max(np.abs(out - np.clip(exp, 0, 1))) < 1e-6
     assert np.allclose(st.low.cpu().numpy(), elow, atol=1e-14) and np.allclose(st.upp.cpu().numpy(), eupp, atol=1e-14)
     assert np.allclose(st.old1.cpu().numpy(), x) and np.allclose(st.old2.cpu().numpy(), old1)
@@ -96,7 +97,7 @@ def test_cantilever_design_iteration_matches_oracle(torch_gpu):
     prob = problems.cantilever(seed=0)
     shape = prob["shape"]
     sp = SimpProblem(shape, prob["fixed"], prob["forces"], volume_fraction=0.3, settings={"coarsest_dofs": 5000})
-    x = torch_gpu.as_tensor(prob["density"])
+    x = torch_gpu.as_tensor(prob["density"], device="cuda")
     res = sp.evaluate(x)
     # oracle chain
     cone = filters.Cone(np.ones(shape, bool), (1, 1, 1), 1.5)
