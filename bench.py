@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--config", default="frame", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--scheme", default="refine", choices=["refine", "flexible"],
+                    help="mixed solver: FP32 PCG inside FP64 iterative refinement, or FP64 PCG with an FP32 V-cycle")
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--apply-only", action="store_true",
                     help="build the hierarchy once and only time the fine operator (for counter profiles)")
@@ -45,7 +47,7 @@ def cpu_baseline(seconds_budget=30.0):
     import numpy as np
     from deep_frame_amd import element, problems
     from oracle import hex8, system
-    shape = (48, 48, 8)
+    shape = (40, 40, 8)  # about 20 s of single-core SuperLU work; fill-in grows ~N^2 beyond this
     prob = problems.frame(shape, seed=0, rhs=42)
     ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
     mod, _ = problems.moduli(prob["density"])
@@ -82,8 +84,8 @@ def main():
     nel = moduli.numel()
     print(f"# config {args.config} shape {cfg['shape']} ndof {ndof} rhs {rhs.shape[0]}", file=sys.stderr, flush=True)
 
-    def run(precision, steps, warmup, label):
-        mg = GeometricMultigrid(cfg["shape"], (1, 1, 1), ke, {"precision": precision, "levels": cfg["levels"]})
+    def run(settings, steps, warmup, label):
+        mg = GeometricMultigrid(cfg["shape"], (1, 1, 1), ke, {**settings, "levels": cfg["levels"]})
         times, reports = [], None
         for it in range(warmup + steps):
             torch.cuda.synchronize()
@@ -128,7 +130,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration_copy_bytes": src.numel() * 4}))
         return
-    mg, times, reports, x = run("float32", args.steps, args.warmup, "mixed")
+    mixed = {"precision": "float32", "refine": args.scheme == "refine"}
+    mg, times, reports, x = run(mixed, args.steps, args.warmup, f"mixed-{args.scheme}")
     t_mixed = float(np.mean(times))
     # dominant kernel: fine-level FP32 operator on all RHS columns, HIP events on the library stream
     slot = mg.hierarchies[("bench", args.warmup + args.steps - 1)]["slot"]
@@ -141,7 +144,7 @@ def main():
     mg.close()
     t_fp64 = None
     if not args.no_fp64:
-        _, t64, _, _ = run("float64", 1, 0, "fp64")
+        _, t64, _, _ = run({"precision": "float64"}, 1, 0, "fp64")
         t_fp64 = float(np.mean(t64))
     baseline = None
     if not args.no_cpu_baseline:
@@ -156,7 +159,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": t_mixed * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32 V-cycle / fp64 outer", "data": "synthetic",
         "config": {"workload": f"{args.config} {cfg['shape'][0]}x{cfg['shape'][1]}x{cfg['shape'][2]} hex8, "
-                               f"{ndof} DOF, {k} RHS, {cfg['levels']}-level GMG",
+                               f"{ndof} DOF, {k} RHS, {cfg['levels']}-level GMG", "scheme": args.scheme,
                    "mixed_s": t_mixed, "fp64_s": t_fp64, "iterations": max(r[0] for r in reports),
                    "max_rel_res": max(r[1] for r in reports)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -135,13 +135,16 @@ void dinv_scale(const double* y, const double* dinv, const double* inv_norm, dou
                 hipStream_t s);
 void pcg_beta(const double* rzn, const double* rzc, double* rz, double* beta, int k, const int* active,
               hipStream_t s);
-void copy_columns(const double* src, double* dst, long long n, int k, const int* select, hipStream_t s);
+template <typename T>
+void copy_columns(const T* src, T* dst, long long n, int k, const int* select, hipStream_t s);
 template <typename T>
 void fill_zero(T* x, long long n, hipStream_t s);
-void pcg_update(double* x, double* r, const double* p, const double* ap, const double* alpha, long long n, int k,
-                const int* active, double* partial_rr, int nblocks, hipStream_t s);
-void pcg_direction(double* p, const double* z, const double* beta, long long n, int k, const int* active,
-                   hipStream_t s);
+template <typename T>
+void pcg_update(T* x, T* r, const T* p, const T* ap, const double* alpha, long long n, int k, const int* active,
+                double* partial_rr, int nblocks, hipStream_t s);
+template <typename T>
+void pcg_direction(T* p, const T* z, const double* beta, long long n, int k, const int* active, hipStream_t s);
+void add_scaled(double* x, const float* d, const double* scale, long long n, int k, const int* active, hipStream_t s);
 void pcg_scalars_alpha(const double* rz, const double* pap, double* alpha, int k, const int* active, hipStream_t s);
 template <typename T>
 void to_work(const double* r, const double* scale, T* w, long long n, int k, const int* active, hipStream_t s);

@@ -488,7 +488,7 @@ static void precondition(Handle* h, Hierarchy& H, const double* r, double* z, in
   double* scale = h->scal.as<double>() + 8 * (size_t)h->k_cap;
   if (sizeof(T) == sizeof(double)) {
     vcycle<T>(v, 0, reinterpret_cast<const T*>(r));
-    copy_columns(reinterpret_cast<const double*>(F.x.as<T>()), z, n, k, active, s);
+    copy_columns<double>(reinterpret_cast<const double*>(F.x.as<T>()), z, n, k, active, s);
     return;
   }
   const int nb = reduce_blocks(n);
@@ -568,9 +568,9 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
   }
   DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
   fill_zero<double>(x, (long long)k * n, s);
-  copy_columns(b, r, n, k, nullptr, s);
+  copy_columns<double>(b, r, n, k, nullptr, s);
   precond_any(h, H, r, z, k, act);
-  copy_columns(z, p, n, k, nullptr, s);
+  copy_columns<double>(z, p, n, k, nullptr, s);
   const int nbr = reduce_blocks(n);
   column_dot<double>(r, z, n, k, h->partial.as<double>(), nbr, act, s);
   reduce_partials(h->partial.as<double>(), nbr, k, rz, s);
@@ -587,7 +587,7 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
     fine_fp64(h, H, MODE_DOT, p, ap, nullptr, k, act, h->partial.as<double>(), &nbf);
     reduce_partials(h->partial.as<double>(), nbf, k, pap, s);
     pcg_scalars_alpha(rz, pap, alpha, k, act, s);
-    pcg_update(x, r, p, ap, alpha, n, k, act, h->partial.as<double>(), nbr, s);
+    pcg_update<double>(x, r, p, ap, alpha, n, k, act, h->partial.as<double>(), nbr, s);
     reduce_partials(h->partial.as<double>(), nbr, k, rr, s);
     DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     sync(h);
@@ -619,7 +619,7 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
         }
       }
       DFMI_HIP(hipMemcpyAsync(act, replace.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
-      copy_columns(tmp, r, n, k, act, s);
+      copy_columns<double>(tmp, r, n, k, act, s);
       DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
     }
     if (!any()) break;
@@ -629,7 +629,7 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
     column_dot<double>(r, z, n, k, h->partial.as<double>(), nbr, act, s);
     reduce_partials(h->partial.as<double>(), nbr, k, rzc, s);
     pcg_beta(rzn, rzc, rz, beta, k, act, s);
-    pcg_direction(p, zn, beta, n, k, act, s);
+    pcg_direction<double>(p, zn, beta, n, k, act, s);
     std::swap(z, zn);
   }
   if (any()) {
@@ -645,6 +645,122 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
   }
   // keep buffer roles stable for the next call
   if (z != h->z.as<double>()) std::swap(h->z.p, h->zn.p);
+  grid_to_flat(h->pad, h->ux, h->uy, h->uz, x, x_flat, k, s);
+  sync(h);
+}
+
+// Iterative refinement: the FP64 true residual r = b - A x drives outer steps; each outer step
+// solves A d = r / |r| with PCG run entirely in FP32 (vectors, operator, V-cycle) to a relative
+// tolerance, then x += |r| d in FP64.  Convergence is judged on the FP64 residual only, so the
+// result meets the same criterion as the flexible FP64 PCG above.
+static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_flat, int k, dfmi_report* rep) {
+  if (h->st.precision != 1) throw std::invalid_argument("refine mode needs the fp32 V-cycle precision");
+  Hierarchy& H = h->slots[slot];
+  if (!H.built) throw std::runtime_error("hierarchy slot not built");
+  ensure_workspace(h, k);
+  ensure_level_vectors(h, H, k);
+  hipStream_t s = h->stream;
+  const long long n = col(h->pad);
+  const size_t kn = (size_t)k * n;
+  double* b = h->b.as<double>();
+  double* x = h->x.as<double>();
+  double* r = h->r.as<double>();
+  // FP32 inner vectors live in the halves of FP64 buffers that refinement does not use
+  float* r32 = h->z.as<float>();
+  float* d32 = r32 + kn;
+  float* p32 = h->zn.as<float>();
+  float* ap32 = p32 + kn;
+  double* sc = h->scal.as<double>();
+  double* alpha = sc;
+  double* rz = sc + (size_t)h->k_cap;
+  double* pap = sc + 2 * (size_t)h->k_cap;
+  double* rzn = sc + 3 * (size_t)h->k_cap;
+  double* rzc = sc + 4 * (size_t)h->k_cap;
+  double* beta = sc + 5 * (size_t)h->k_cap;
+  double* rr = sc + 6 * (size_t)h->k_cap;
+  double* scale = sc + 8 * (size_t)h->k_cap;
+  int* act = h->active.as<int>();
+  Level& F = *H.levels[0];
+  VArgs v{h, &H, k, act};
+  const int nbr = reduce_blocks(n);
+  double* part = h->partial.as<double>();
+
+  flat_to_grid(h->pad, h->ux, h->uy, h->uz, rhs_flat, b, k, s);
+  mask_vector(F.mask.as<uint8_t>(), b, h->pad.nodes(), k, s);
+  std::vector<double> nb;
+  column_norms(h, b, n, k, nullptr, nb);
+  std::vector<int> active(k), inner(k);
+  for (int c = 0; c < k; ++c) {
+    active[c] = nb[c] > 0 ? 1 : 0;
+    rep[c].iterations = 0;
+    rep[c].relative_residual = 0.0;
+    rep[c].converged = 1;
+  }
+  fill_zero<double>(x, (long long)kn, s);
+  copy_columns<double>(b, r, n, k, nullptr, s);
+  const double tol = h->st.tolerance;
+  int step = 0;
+  std::vector<double> rn, rrh(k), target(k);
+  auto count = [](const std::vector<int>& f) {
+    int m = 0;
+    for (int q : f) m += q;
+    return m;
+  };
+  while (true) {
+    DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+    column_norms(h, r, n, k, act, rn);
+    for (int c = 0; c < k; ++c) {
+      if (!active[c]) continue;
+      rep[c].iterations = step;
+      rep[c].relative_residual = rn[c] / nb[c];
+      rep[c].converged = rep[c].relative_residual < tol;
+      if (rep[c].converged) active[c] = 0;
+      // inner target: the outer tolerance with margin, never tighter than FP32 can track
+      target[c] = std::max(h->st.inner_tolerance, 0.5 * tol * nb[c] / std::max(rn[c], 1e-300));
+    }
+    if (!count(active) || step >= h->st.max_iterations) break;
+    DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+    DFMI_HIP(hipMemcpyAsync(scale, rn.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    to_work<float>(r, scale, r32, n, k, act, s);
+    fill_zero<float>(d32, (long long)kn, s);
+    vcycle<float>(v, 0, r32);
+    copy_columns<float>(F.x.as<float>(), p32, n, k, act, s);
+    column_dot<float>(r32, F.x.as<float>(), n, k, part, nbr, act, s);
+    reduce_partials(part, nbr, k, rz, s);
+    inner = active;
+    while (step < h->st.max_iterations) {
+      int nbf = 0;
+      ApplyArgs a{};
+      a.x = p32;
+      a.y = ap32;
+      a.k = k;
+      a.active = act;
+      a.partial = part;
+      fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_DOT, a, s, &nbf);
+      reduce_partials(part, nbf, k, pap, s);
+      pcg_scalars_alpha(rz, pap, alpha, k, act, s);
+      pcg_update<float>(d32, r32, p32, ap32, alpha, n, k, act, part, nbr, s);
+      reduce_partials(part, nbr, k, rr, s);
+      DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+      sync(h);
+      ++step;
+      for (int c = 0; c < k; ++c)
+        if (inner[c] && std::sqrt(rrh[c]) < target[c]) inner[c] = 0;
+      if (!count(inner)) break;
+      DFMI_HIP(hipMemcpyAsync(act, inner.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+      // flexible (Polak-Ribiere) beta: r_new . z_old is taken before the V-cycle overwrites z
+      column_dot<float>(r32, F.x.as<float>(), n, k, part, nbr, act, s);
+      reduce_partials(part, nbr, k, rzc, s);
+      vcycle<float>(v, 0, r32);
+      column_dot<float>(r32, F.x.as<float>(), n, k, part, nbr, act, s);
+      reduce_partials(part, nbr, k, rzn, s);
+      pcg_beta(rzn, rzc, rz, beta, k, act, s);
+      pcg_direction<float>(p32, F.x.as<float>(), beta, n, k, act, s);
+    }
+    DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+    add_scaled(x, d32, scale, n, k, act, s);
+    fine_fp64(h, H, MODE_RESID, x, r, b, k, act, nullptr, nullptr);
+  }
   grid_to_flat(h->pad, h->ux, h->uy, h->uz, x, x_flat, k, s);
   sync(h);
 }
@@ -779,8 +895,10 @@ int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t 
   DFMI_GUARD({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     if (k < 1 || !rhs || !x || !report) throw std::invalid_argument("solve needs k >= 1 and buffers");
-    if (mg->h.st.refine) throw std::invalid_argument("refine mode is not available in this build");
-    solve(&mg->h, slot, rhs, x, k, report);
+    if (mg->h.st.refine)
+      solve_refine(&mg->h, slot, rhs, x, k, report);
+    else
+      solve(&mg->h, slot, rhs, x, k, report);
   })
 }
 

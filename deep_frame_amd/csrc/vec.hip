@@ -182,18 +182,21 @@ void pcg_beta(const double* rzn, const double* rzc, double* rz, double* beta, in
 }
 
 // dst[c] = src[c] for columns with select[c] != 0
-__global__ void k_copy_columns(const double* __restrict__ src, double* __restrict__ dst, long long n,
-                               const int* select) {
+template <typename T>
+__global__ void k_copy_columns(const T* __restrict__ src, T* __restrict__ dst, long long n, const int* select) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
   if (i >= n || (select && !select[c])) return;
   dst[(long long)c * n + i] = src[(long long)c * n + i];
 }
 
-void copy_columns(const double* src, double* dst, long long n, int k, const int* select, hipStream_t s) {
-  hipLaunchKernelGGL(k_copy_columns, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, src, dst, n, select);
+template <typename T>
+void copy_columns(const T* src, T* dst, long long n, int k, const int* select, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_columns<T>, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, src, dst, n, select);
   DFMI_HIP(hipGetLastError());
 }
+template void copy_columns<float>(const float*, float*, long long, int, const int*, hipStream_t);
+template void copy_columns<double>(const double*, double*, long long, int, const int*, hipStream_t);
 
 template <typename T>
 __global__ void k_fill_zero(T* x, long long n) {
@@ -210,47 +213,70 @@ void fill_zero(T* x, long long n, hipStream_t s) {
 template void fill_zero<float>(float*, long long, hipStream_t);
 template void fill_zero<double>(double*, long long, hipStream_t);
 
+// x += scale[c] * d for active columns (iterative-refinement correction, FP32 -> FP64)
+__global__ void k_add_scaled(double* __restrict__ x, const float* __restrict__ d, const double* scale, long long n,
+                             const int* active) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= n || (active && !active[c])) return;
+  x[(long long)c * n + i] += scale[c] * (double)d[(long long)c * n + i];
+}
+
+void add_scaled(double* x, const float* d, const double* scale, long long n, int k, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, x, d, scale, n, active);
+  DFMI_HIP(hipGetLastError());
+}
+
 // x += alpha p, r -= alpha Ap, partial |r|^2 (columns with alpha = 0 are left unchanged)
-__global__ __launch_bounds__(256) void k_pcg_update(double* __restrict__ x, double* __restrict__ r,
-                                                     const double* __restrict__ p, const double* __restrict__ ap,
-                                                     const double* alpha, long long n, const int* active,
-                                                     double* partial) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                                                     const T* __restrict__ ap, const double* alpha, long long n,
+                                                     const int* active, double* partial) {
   const int c = blockIdx.y;
   double acc = 0.0;
   if (!active || active[c]) {
-    const double a = alpha[c];
+    const T a = (T)alpha[c];
     const long long off = (long long)c * n;
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
       x[off + i] += a * p[off + i];
-      const double rv = r[off + i] - a * ap[off + i];
+      const T rv = r[off + i] - a * ap[off + i];
       r[off + i] = rv;
-      acc += rv * rv;
+      acc += (double)rv * (double)rv;
     }
   }
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[(long long)c * gridDim.x + blockIdx.x] = acc;
 }
 
-void pcg_update(double* x, double* r, const double* p, const double* ap, const double* alpha, long long n, int k,
-                const int* active, double* partial_rr, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_update, dim3(nblocks, k), dim3(256), 0, s, x, r, p, ap, alpha, n, active, partial_rr);
+template <typename T>
+void pcg_update(T* x, T* r, const T* p, const T* ap, const double* alpha, long long n, int k, const int* active,
+                double* partial_rr, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update<T>, dim3(nblocks, k), dim3(256), 0, s, x, r, p, ap, alpha, n, active, partial_rr);
   DFMI_HIP(hipGetLastError());
 }
+template void pcg_update<float>(float*, float*, const float*, const float*, const double*, long long, int, const int*,
+                                double*, int, hipStream_t);
+template void pcg_update<double>(double*, double*, const double*, const double*, const double*, long long, int,
+                                 const int*, double*, int, hipStream_t);
 
-__global__ void k_pcg_direction(double* __restrict__ p, const double* __restrict__ z, const double* beta, long long n,
+template <typename T>
+__global__ void k_pcg_direction(T* __restrict__ p, const T* __restrict__ z, const double* beta, long long n,
                                 const int* active) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
   if (i >= n || (active && !active[c])) return;
   const long long o = (long long)c * n + i;
-  p[o] = z[o] + beta[c] * p[o];
+  p[o] = z[o] + (T)beta[c] * p[o];
 }
 
-void pcg_direction(double* p, const double* z, const double* beta, long long n, int k, const int* active,
-                   hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_direction, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, p, z, beta, n, active);
+template <typename T>
+void pcg_direction(T* p, const T* z, const double* beta, long long n, int k, const int* active, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_direction<T>, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, p, z, beta, n,
+                     active);
   DFMI_HIP(hipGetLastError());
 }
+template void pcg_direction<float>(float*, const float*, const double*, long long, int, const int*, hipStream_t);
+template void pcg_direction<double>(double*, const double*, const double*, long long, int, const int*, hipStream_t);
 
 __global__ void k_alpha(const double* rz, const double* pap, double* alpha, int k, const int* active) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;

@@ -12,6 +12,9 @@ from oracle import hex8, loads, system
 
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
+# solver schemes: FP64 V-cycle, FP32 V-cycle inside FP64 PCG, FP32 PCG inside FP64 iterative refinement
+MODES = {"float64": {"precision": "float64"}, "float32": {"precision": "float32"},
+         "refine": {"precision": "float32", "refine": True}}
 
 
 def make(shape, spacing, ke, moduli, settings=None):
@@ -21,7 +24,7 @@ def make(shape, spacing, ke, moduli, settings=None):
     return mg
 
 
-@pytest.mark.parametrize("precision", ["float64", "float32"])
+@pytest.mark.parametrize("precision", sorted(MODES))
 def test_euler_bernoulli_tip_deflection(precision, torch_gpu):
     """Documented known answer 0.0730389 mm (docs/topology_optimization.md, Numerische Pruefungen)."""
     print(f"eb precision={precision}", flush=True)
@@ -32,7 +35,7 @@ def test_euler_bernoulli_tip_deflection(precision, torch_gpu):
     tip = np.flatnonzero(np.abs(pts[:, 0] - 48.0) < 1e-9)
     f = np.zeros((3 * len(pts), 1))
     f[3 * tip + 2, 0] = -1.0 / len(tip)
-    mg = make(shape, h, ke, np.full(96 * 4, 4430.0), {"precision": precision})
+    mg = make(shape, h, ke, np.full(96 * 4, 4430.0), MODES[precision])
     u, rep = mg.solve(b"eb", fixed, f)
     print(rep["iterations"], rep["relative_residual"], flush=True)
     assert rep["converged"] and rep["relative_residual"][0] < 1e-8
@@ -40,7 +43,7 @@ def test_euler_bernoulli_tip_deflection(precision, torch_gpu):
     mg.close()
 
 
-@pytest.mark.parametrize("precision", ["float64", "float32"])
+@pytest.mark.parametrize("precision", sorted(MODES))
 def test_pcg_matches_direct_solution(precision, torch_gpu):
     print(f"pcg-vs-direct precision={precision}", flush=True)
     shape = (16, 8, 8)
@@ -52,7 +55,7 @@ def test_pcg_matches_direct_solution(precision, torch_gpu):
     f = np.random.default_rng(1).normal(size=(3 * len(pts), 3))
     f[fixed] = 0
     ref, _ = system.solve(dofs, conn, ke, mod, fixed, f, len(pts))
-    mg = make(shape, (1, 1, 1), ke, mod, {"precision": precision})
+    mg = make(shape, (1, 1, 1), ke, mod, MODES[precision])
     u, rep = mg.solve(b"c", fixed, f)
     print(rep["iterations"], rep["relative_residual"], flush=True)
     assert rep["converged"]
@@ -90,7 +93,8 @@ def test_covariance_cantilever_dense_flexibility(torch_gpu):
     mg.close()
 
 
-def test_cantilever_config_compliance_and_sensitivity(torch_gpu):
+@pytest.mark.parametrize("mode", ["float32", "refine"])
+def test_cantilever_config_compliance_and_sensitivity(mode, torch_gpu):
     """48 x 24 x 24 cantilever: compliance 1e-6 and sensitivity 1e-5 against the FP64 oracle."""
     prob = problems.cantilever(seed=0)
     shape = prob["shape"]
@@ -100,7 +104,7 @@ def test_cantilever_config_compliance_and_sensitivity(torch_gpu):
     ref_u, _ = system.solve(dofs, conn, ke, mod, prob["fixed"], prob["forces"], len(pts))
     ref_c, ref_s = system.compliance_sensitivity(dofs, ke, ref_u, prob["forces"], dmod)
     torch = torch_gpu
-    mg = make(shape, (1, 1, 1), ke, mod, {"coarsest_dofs": 5000})
+    mg = make(shape, (1, 1, 1), ke, mod, {"coarsest_dofs": 5000, **MODES[mode]})
     rhs = torch.as_tensor(prob["forces"].T.copy(), device="cuda")
     x, reps = mg.solve_device(b"c1", prob["fixed"], rhs)
     assert all(r[2] and r[1] < 1e-8 for r in reps)
@@ -125,3 +129,23 @@ def test_zero_load_columns_skip_pcg(torch_gpu):
     assert rep["converged"] and rep["zero_rhs_columns"] == 1 and rep["iterations"][1] == 0
     assert not np.any(flat[:, 1]) and np.linalg.norm(flat[:, 0] - single[:, 0]) <= 1e-12 * np.linalg.norm(single[:, 0])
     mg.close()
+
+
+def test_refine_matches_flexible_pcg_on_frame_wrenches(torch_gpu):
+    """42 unit-wrench columns on a 32 x 32 x 8 frame: both schemes meet rel-res 1e-8 and agree."""
+    prob = problems.frame((32, 32, 8), seed=0, rhs=42)
+    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
+    mod, _ = problems.moduli(prob["density"])
+    torch = torch_gpu
+    rhs = torch.as_tensor(np.ascontiguousarray(prob["forces"].T), device="cuda")
+    out = {}
+    for mode in ("float32", "refine"):
+        mg = make((32, 32, 8), (1, 1, 1), ke, mod, {"coarsest_dofs": 3000, **MODES[mode]})
+        x, reps = mg.solve_device(b"f", prob["fixed"], rhs)
+        print(mode, max(r[0] for r in reps), max(r[1] for r in reps), flush=True)
+        assert all(r[2] and r[1] < 1e-8 for r in reps)
+        out[mode] = x.cpu().numpy()
+        mg.close()
+    for c in range(42):
+        a, b = out["float32"][c], out["refine"][c]
+        assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
