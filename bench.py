@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--scheme", default="refine", choices=["refine", "flexible"],
                     help="mixed solver: FP32 PCG inside FP64 iterative refinement, or FP64 PCG with an FP32 V-cycle")
+    ap.add_argument("--inner-tolerance", type=float, default=None, help="refine scheme: inner relative tolerance")
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--apply-only", action="store_true",
                     help="build the hierarchy once and only time the fine operator (for counter profiles)")
@@ -131,6 +132,8 @@ def main():
         print(json.dumps({"calibration_copy_bytes": src.numel() * 4}))
         return
     mixed = {"precision": "float32", "refine": args.scheme == "refine"}
+    if args.inner_tolerance:
+        mixed["inner_tolerance"] = args.inner_tolerance
     mg, times, reports, x = run(mixed, args.steps, args.warmup, f"mixed-{args.scheme}")
     t_mixed = float(np.mean(times))
     # dominant kernel: fine-level FP32 operator on all RHS columns, HIP events on the library stream

@@ -805,7 +805,7 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->dead_ratio = 1e-12;
   o->seed = 0;
   o->refine = 0;
-  o->inner_tolerance = 1e-4;
+  o->inner_tolerance = 1e-5;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
