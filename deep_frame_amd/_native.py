@@ -56,6 +56,19 @@ SIGNATURES = {
     "dfmi_oc_update": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_void_p], c_int32),
     "dfmi_mma_step": ([c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                        c_double, c_void_p, c_double, c_double, c_void_p], c_int32),
+    "dfmi_net_create": ([POINTER(c_void_p)], c_int32),
+    "dfmi_net_destroy": ([c_void_p], c_int32),
+    "dfmi_net_set_stream": ([c_void_p, c_void_p], c_int32),
+    "dfmi_net_param_count": ([c_void_p, POINTER(c_int64)], c_int32),
+    "dfmi_net_set_params": ([c_void_p, c_void_p], c_int32),
+    "dfmi_net_get": ([c_void_p, c_int32, c_void_p], c_int32),
+    "dfmi_net_forward": ([c_void_p, c_void_p, c_int32, POINTER(c_int32), c_void_p], c_int32),
+    "dfmi_net_backward": ([c_void_p, c_void_p], c_int32),
+    "dfmi_net_adam": ([c_void_p, c_double, c_double, c_double, c_double], c_int32),
+    "dfmi_net_train_step": ([c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_double, POINTER(c_double)], c_int32),
+    "dfmi_attach_net": ([c_void_p, c_void_p], c_int32),
+    "dfmi_harvest": ([c_void_p, c_void_p, c_int64], c_int32),
+    "dfmi_harvest_count": ([c_void_p, POINTER(c_int64)], c_int32),
 }
 
 _lib = None

@@ -49,6 +49,25 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nblocks) {
   return x < full ? x * per + i : full * per + (x - full) * (per - 1) + i;
 }
 
+// Block-wide reductions for 256-thread workgroups (four 64-wide wavefronts).
+__device__ inline double block_sum(double v) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__device__ inline double block_max(double v) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
 struct VLay {
   long long ns, rs;
 };

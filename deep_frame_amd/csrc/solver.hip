@@ -13,6 +13,7 @@
 #include <stdexcept>
 
 #include "dfmi_internal.h"
+#include "dfmi_learned.h"
 #include "dfmi.h"
 
 namespace dfmi {
@@ -94,6 +95,10 @@ struct Handle {
   int k_cap = 0;
   DevBuf b, x, r, z, zn, p, ap, tmp, partial, scal, active;
   std::vector<Geom> geoms;
+  // learned coarse correction (not owned) and training-sample harvest (caller's device buffer)
+  NetImpl* net = nullptr;
+  float* harvest = nullptr;
+  long long harvest_cap = 0, harvest_count = 0;
 };
 
 static void hex8_blocks(const double* ke_bits, KeBlocks& kb) {
@@ -264,6 +269,48 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
   if (cur != L.x.as<T>()) std::swap(L.x.p, L.xa.p), std::swap(L.x.bytes, L.xa.bytes);
 }
 
+// Learned coarse correction on the fine level, between the coarse-grid correction and the
+// post-smoother: g = R (b - A x) on level 1, delta = Dinv (g + s N(g / s, f)), scaled per column by the
+// energy-optimal step max(0, delta.g / delta.A_1 delta), then x += P delta.  With the Galerkin level-1
+// operator A_1 = P^T A P this step never increases the A-norm error of x.  The same residual g is
+// what the training harvest records.
+template <typename T>
+static void learned_correction(VArgs& v, const T* b) {
+  if constexpr (sizeof(T) != sizeof(float)) {
+    if (v.h->net) throw std::invalid_argument("the learned correction needs the fp32 V-cycle precision");
+    return;
+  } else {
+    Hierarchy& H = *v.H;
+    if (H.levels.size() < 2) return;
+    Level& L = *H.levels[0];
+    Level& C = *H.levels[1];
+    hipStream_t s = v.h->stream;
+    const VLay cl = interleaved(v.k);
+    ApplyArgs a{};
+    a.x = L.x.as<float>();
+    a.y = L.res.as<float>();
+    a.b = b;
+    level_apply<float>(v, 0, MODE_RESID, a);
+    restrict_full<float>(L.g, L.res.as<float>(), column_major(L.g), C.g, C.mask.as<uint8_t>(), C.rhs.as<float>(), cl,
+                         v.k, v.active, s);
+    const long long nodes = C.g.nodes();
+    if (v.h->harvest && v.h->harvest_count + v.k <= v.h->harvest_cap) {
+      gather_samples(C.rhs.as<float>(), cl, nodes, v.k, v.active, v.h->harvest + v.h->harvest_count * 3 * nodes, s);
+      v.h->harvest_count += v.k;
+    }
+    if (!v.h->net) return;
+    net_set_stream(v.h->net, s);
+    net_correct(v.h->net, C.g, C.rhs.as<float>(), cl, v.k, C.dinv32.as<float>(), v.active, C.x.as<float>());
+    ApplyArgs c{};
+    c.x = C.x.as<float>();
+    c.y = C.xa.as<float>();
+    level_apply<float>(v, 1, MODE_APPLY, c);
+    net_energy_step(v.h->net, C.rhs.as<float>(), C.xa.as<float>(), C.x.as<float>(), cl, nodes, v.k, v.active);
+    prolong_add<float>(C.g, C.x.as<float>(), cl, L.g, L.mask.as<uint8_t>(), L.x.as<float>(), column_major(L.g), v.k,
+                       v.active, s);
+  }
+}
+
 // V-cycle on level idx for right-hand side b (device, T); result in levels[idx].x
 template <typename T>
 static void vcycle(VArgs& v, int idx, const T* b) {
@@ -289,6 +336,7 @@ static void vcycle(VArgs& v, int idx, const T* b) {
   vcycle<T>(v, idx + 1, C.rhs.as<T>());
   prolong_add<T>(C.g, C.x.as<T>(), level_layout(C, idx + 1, v.k), L.g, L.mask.as<uint8_t>(), L.x.as<T>(),
                  level_layout(L, idx, v.k), v.k, v.active, s);
+  if (idx == 0 && (v.h->net || v.h->harvest)) learned_correction<T>(v, b);
   chebyshev<T>(v, idx, b, false, count);
 }
 
@@ -774,6 +822,10 @@ struct dfmi_mg {
   Handle h;
 };
 
+struct dfmi_net {
+  NetImpl* impl = nullptr;
+};
+
 #define DFMI_GUARD(...)                   \
   try {                                   \
     __VA_ARGS__;                          \
@@ -1088,4 +1140,106 @@ int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coa
   })
 }
 
+
+// ---- learned coarse correction ----------------------------------------------------------------
+int dfmi_net_create(dfmi_net** out) {
+  DFMI_GUARD({
+    if (!out) throw std::invalid_argument("null output handle");
+    auto* n = new dfmi_net();
+    n->impl = net_new();
+    *out = n;
+  })
+}
+
+int dfmi_net_destroy(dfmi_net* net) {
+  DFMI_GUARD({
+    if (net) {
+      net_delete(net->impl);
+      delete net;
+    }
+  })
+}
+
+int dfmi_net_set_stream(dfmi_net* net, void* stream) {
+  DFMI_GUARD({ net_set_stream(net->impl, (hipStream_t)stream); })
+}
+
+int dfmi_net_param_count(dfmi_net* net, int64_t* count) { DFMI_GUARD({ *count = net_param_count(net->impl); }) }
+
+int dfmi_net_set_params(dfmi_net* net, const float* params) {
+  DFMI_GUARD({
+    if (!params) throw std::invalid_argument("null parameter buffer");
+    net_set_params(net->impl, params);
+  })
+}
+
+int dfmi_net_get(dfmi_net* net, int32_t which, float* out) {
+  DFMI_GUARD({
+    if (!out || (which != 0 && which != 1)) throw std::invalid_argument("which must be 0 (weights) or 1 (gradients)");
+    net_get(net->impl, which, out);
+  })
+}
+
+int dfmi_net_forward(dfmi_net* net, const float* x, int32_t n, const int32_t dims[3], float* u) {
+  DFMI_GUARD({
+    if (!x || !u || n < 1 || dims[0] < 1 || dims[1] < 1 || dims[2] < 1)
+      throw std::invalid_argument("forward needs n >= 1, positive dims and buffers");
+    const NetDims d{dims[0], dims[1], dims[2]};
+    net_forward(net->impl, n, d, x);
+    net_copy_output(net->impl, u, (long long)n * 3 * d.n());
+  })
+}
+
+int dfmi_net_backward(dfmi_net* net, const float* du) {
+  DFMI_GUARD({
+    if (!du) throw std::invalid_argument("null gradient");
+    net_backward(net->impl, du);
+  })
+}
+
+int dfmi_net_adam(dfmi_net* net, double lr, double beta1, double beta2, double eps) {
+  DFMI_GUARD({ net_adam(net->impl, lr, beta1, beta2, eps); })
+}
+
+int dfmi_net_train_step(dfmi_net* net, dfmi_mg* mg, int32_t slot, const float* g, int32_t n, double lr,
+                        double* loss) {
+  DFMI_GUARD({
+    if (slot < 0 || slot >= 8 || !mg->h.slots[slot].built) throw std::invalid_argument("hierarchy slot not built");
+    Hierarchy& H = mg->h.slots[slot];
+    if (H.levels.size() < 2) throw std::invalid_argument("training needs at least two levels");
+    if (!g || n < 1) throw std::invalid_argument("train step needs n >= 1 samples");
+    Level& C = *H.levels[1];
+    hipStream_t s = mg->h.stream;
+    net_set_stream(net->impl, s);
+    auto apply = [&](const float* x, float* y) {
+      ApplyArgs a{};
+      a.x = x;
+      a.y = y;
+      a.k = n;
+      stencil_apply<float>(C.g, C.st32.as<float>(), C.mask.as<uint8_t>(), MODE_APPLY, a, column_major(C.g), s,
+                           nullptr);
+    };
+    const double l = net_train_step(net->impl, C.g, g, n, C.dinv32.as<float>(), lr, apply);
+    if (loss) *loss = l;
+  })
+}
+
+int dfmi_attach_net(dfmi_mg* mg, dfmi_net* net) {
+  DFMI_GUARD({
+    if (net && mg->h.st.precision != 1) throw std::invalid_argument("the learned correction needs precision fp32");
+    mg->h.net = net ? net->impl : nullptr;
+    if (net) net_invalidate_feature(net->impl);
+  })
+}
+
+int dfmi_harvest(dfmi_mg* mg, float* buffer, int64_t capacity) {
+  DFMI_GUARD({
+    if (capacity < 0 || (capacity > 0 && !buffer)) throw std::invalid_argument("harvest needs a buffer");
+    mg->h.harvest = capacity > 0 ? buffer : nullptr;
+    mg->h.harvest_cap = capacity;
+    mg->h.harvest_count = 0;
+  })
+}
+
+int dfmi_harvest_count(dfmi_mg* mg, int64_t* count) { DFMI_GUARD({ *count = mg->h.harvest_count; }) }
 }  // extern "C"

@@ -12,24 +12,6 @@ int reduce_blocks(long long n) {
   return (int)std::max(1LL, std::min(b, 512LL));
 }
 
-__device__ inline double block_sum(double v) {
-  __shared__ double red[4];
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  return red[0] + red[1] + red[2] + red[3];
-}
-
-__device__ inline double block_max(double v) {
-  __shared__ double red[4];
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_column_dot(const T* __restrict__ a, const T* __restrict__ b, long long n,
                                                      double* partial, const int* active) {

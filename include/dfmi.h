@@ -26,6 +26,7 @@ extern "C" {
 
 typedef struct dfmi_mg dfmi_mg;
 typedef struct dfmi_filter dfmi_filter;
+typedef struct dfmi_net dfmi_net;
 
 /* Settings of deep_frame/topology_multigrid.py:8-10 (MULTIGRID) that the solver uses. */
 typedef struct dfmi_settings {
@@ -144,6 +145,38 @@ int dfmi_oc_update(dfmi_filter* f, const double* x_dev, const double* dc_dev, co
 int dfmi_mma_step(int64_t n, int32_t iteration, double* x_dev, double* xold1_dev, double* xold2_dev,
                   double* low_dev, double* upp_dev, double f0, const double* df0_dev, double g,
                   const double* dg_dev, double move, double scale, void* hip_stream);
+
+/* ---- Learned coarse correction (no counterpart in the reference; the nearest pattern is the
+ * hand-written NumPy MLP with backward and Adam of deep_frame/topology_neural.py:59-113).
+ * A 4-level 3D U-Net (channels 16-32-64-64, BF16 weights, FP32 activations and accumulation) on the
+ * level-1 node grid maps the level-1 residual g = R (b - A x) and a stiffness feature to the correction
+ * delta = Dinv (g + s N(g / s, f)) that the fp32 V-cycle adds, with an energy-optimal step per column,
+ * between the fine coarse-grid correction and the post-smoother.  Parameters: one flat FP32 array
+ * (per layer: weights [c_out][c_in][27], then biases; up layers [c_coarse][c_fine][27]). */
+int dfmi_net_create(dfmi_net** out);
+int dfmi_net_destroy(dfmi_net* net);
+int dfmi_net_set_stream(dfmi_net* net, void* hip_stream);
+int dfmi_net_param_count(dfmi_net* net, int64_t* count);
+/* Load FP32 master weights (device) and reset the Adam state. */
+int dfmi_net_set_params(dfmi_net* net, const float* params_dev);
+/* which = 0: master weights, 1: gradients of the last backward / train step (device out). */
+int dfmi_net_get(dfmi_net* net, int32_t which, float* out_dev);
+/* u_dev (n, 3, X, Y, Z) = N(x_dev (n, 6, X, Y, Z)) on an X x Y x Z node grid. */
+int dfmi_net_forward(dfmi_net* net, const float* x_dev, int32_t n, const int32_t dims[3], float* u_dev);
+/* Gradients of a loss with dL/du = du_dev, for the last forward. */
+int dfmi_net_backward(dfmi_net* net, const float* du_dev);
+int dfmi_net_adam(dfmi_net* net, double lr, double beta1, double beta2, double eps);
+/* One step on the normalised energy loss sum_n [0.5 delta_n.A_1 delta_n - delta_n.g_n] / (g_n.Dinv g_n)
+ * with the level-1 Galerkin operator of a built slot; g_dev: n level-1 residuals [n][nodes][3] (as
+ * harvested).  lr = 0 computes loss and gradients only.  loss may be NULL. */
+int dfmi_net_train_step(dfmi_net* net, dfmi_mg* mg, int32_t slot, const float* g_dev, int32_t n, double lr,
+                        double* loss);
+/* Use the network inside the handle's fp32 V-cycle (NULL detaches). */
+int dfmi_attach_net(dfmi_mg* mg, dfmi_net* net);
+/* Record the level-1 residual g of every fine V-cycle into buffer_dev ([capacity][nodes_1][3], FP32),
+ * zeros for inactive columns, until it is full; capacity 0 stops.  The count restarts at 0. */
+int dfmi_harvest(dfmi_mg* mg, float* buffer_dev, int64_t capacity);
+int dfmi_harvest_count(dfmi_mg* mg, int64_t* count);
 
 #ifdef __cplusplus
 }

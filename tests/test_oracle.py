@@ -216,3 +216,40 @@ def test_mma_subproblem_kkt_and_dual_agree():
     assert np.max(np.abs(grad[inner])) < 1e-5 * np.max(np.abs(p0 / (upp - xn) ** 2))
     # first two iterations place the asymptotes at x -+ 0.5 (asyinit)
     assert np.allclose(low, x - 0.5) and np.allclose(upp, x + 0.5)
+
+
+def test_unet_gradients_match_finite_differences():
+    """Backward pass of the learned correction's CPU restatement against central differences of its
+    energy loss, on the largest-gradient weight of every layer and one bias per layer."""
+    from scipy.sparse import diags, random as sprand
+    from oracle import unet
+    shape = (7, 5, 5)
+    n = 3 * int(np.prod(shape))
+    rng = np.random.default_rng(0)
+    m = sprand(n, n, density=0.03, random_state=1)
+    a = (m @ m.T + diags(rng.uniform(1, 2, n))).tocsr()
+    diag = a.diagonal().reshape(shape + (3,)).transpose(3, 0, 1, 2)
+    dinv = 1 / diag
+
+    def apply_a(v):
+        return (a @ v.transpose(1, 2, 3, 0).ravel()).reshape(shape + (3,)).transpose(3, 0, 1, 2)
+
+    params = unet.init(0)
+    params["out"]["w"] = rng.normal(0, 0.1, params["out"]["w"].shape)
+    params = {k: {"w": np.asarray(v["w"], float), "b": rng.normal(0, 0.05, v["b"].shape)} for k, v in params.items()}
+    g = rng.normal(size=(2, 3) + shape)
+    loss, du, acts = unet.energy_loss(params, g, diag, dinv, apply_a, keep=True, rounded=False)
+    grads = unet.backward(params, acts, du, rounded=False)
+    h = 1e-6
+    for name in params:
+        for which in ("w", "b"):
+            arr = params[name][which]
+            idx = np.unravel_index(np.argmax(np.abs(grads[name][which])), arr.shape)
+            old = arr[idx]
+            arr[idx] = old + h
+            lp = unet.energy_loss(params, g, diag, dinv, apply_a, rounded=False)
+            arr[idx] = old - h
+            lm = unet.energy_loss(params, g, diag, dinv, apply_a, rounded=False)
+            arr[idx] = old
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - grads[name][which][idx]) <= 1e-5 * abs(fd) + 1e-9, (name, which)
