@@ -25,7 +25,7 @@ void net_copy_output(NetImpl* N, float* dst, long long count);  // synchronises 
 void net_backward(NetImpl* N, const float* dout);         // gradients of the last forward
 void net_adam(NetImpl* N, double lr, double b1, double b2, double eps);
 void net_invalidate_feature(NetImpl* N);
-// delta = Dinv (g + s N(g / s, f)) on n level-1 vectors in layout lay
+// delta = Dinv g + Dinv^(1/2) s N(Dinv^(1/2) g / s, f) on n level-1 vectors in layout lay
 void net_correct(NetImpl* N, const Geom& g1, const float* g, VLay lay, int n, const float* dinv, const int* active,
                  float* delta);
 // energy-optimal step per vector: delta *= max(0, delta.g / delta.A delta)

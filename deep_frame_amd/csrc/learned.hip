@@ -253,14 +253,15 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 }
 
 // ---- features, correction and loss on multigrid vectors (layout VLay, nodes of level 1) ----------
+// per-sample max |Dinv^(1/2) g|
 __global__ __launch_bounds__(256) void k_sample_absmax(const float* __restrict__ g, VLay lay, long long nodes,
-                                                       double* partial) {
+                                                       const float* __restrict__ dinv, double* partial) {
   const int n = blockIdx.y;
   double m = 0.0;
   for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < nodes * 3; e += (long long)gridDim.x * 256) {
     const long long node = e / 3;
     const int c = (int)(e % 3);
-    m = fmax(m, (double)fabsf(g[node * lay.ns + n * lay.rs + c]));
+    m = fmax(m, (double)fabsf(g[node * lay.ns + n * lay.rs + c] * sqrtf(dinv[e])));
   }
   m = block_max(m);
   if (threadIdx.x == 0) partial[(long long)n * gridDim.x + blockIdx.x] = m;
@@ -302,8 +303,9 @@ __global__ void k_min_positive_partial(const float* __restrict__ dinv, long long
   if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
+// x[n] = (Dinv^(1/2) g / s, f)
 __global__ void k_features(const float* __restrict__ g, VLay lay, long long nodes, const double* scale,
-                           const float* __restrict__ feat, float* __restrict__ x) {
+                           const float* __restrict__ dinv, const float* __restrict__ feat, float* __restrict__ x) {
   const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int n = blockIdx.y;
   if (node >= nodes) return;
@@ -311,12 +313,12 @@ __global__ void k_features(const float* __restrict__ g, VLay lay, long long node
   const float inv = (float)(1.0 / s);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    x[((long long)n * 6 + c) * nodes + node] = g[node * lay.ns + n * lay.rs + c] * inv;
+    x[((long long)n * 6 + c) * nodes + node] = g[node * lay.ns + n * lay.rs + c] * sqrtf(dinv[node * 3 + c]) * inv;
     x[((long long)n * 6 + 3 + c) * nodes + node] = feat[c * nodes + node];
   }
 }
 
-// delta = Dinv (g + s u)  (= Dinv s (g / s + u))
+// delta = Dinv g + Dinv^(1/2) s u
 __global__ void k_delta(const float* __restrict__ g, VLay lay, long long nodes, const double* scale,
                         const float* __restrict__ u, const float* __restrict__ dinv, float* __restrict__ delta,
                         const int* active) {
@@ -327,11 +329,12 @@ __global__ void k_delta(const float* __restrict__ g, VLay lay, long long nodes, 
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const long long o = node * lay.ns + n * lay.rs + c;
-    delta[o] = dinv[node * 3 + c] * (g[o] + s * u[((long long)n * 3 + c) * nodes + node]);
+    const float di = dinv[node * 3 + c];
+    delta[o] = di * g[o] + sqrtf(di) * s * u[((long long)n * 3 + c) * nodes + node];
   }
 }
 
-// du = (A delta - g) Dinv s / norm; loss partials (0.5 delta.A delta - delta.g) / norm
+// du = (A delta - g) Dinv^(1/2) s / norm; loss partials (0.5 delta.A delta - delta.g) / norm
 __global__ __launch_bounds__(256) void k_energy_grad(const float* __restrict__ g, const float* __restrict__ delta,
                                                      const float* __restrict__ ad, VLay lay, long long nodes,
                                                      const double* scale, const double* norm,
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(256) void k_energy_grad(const float* __restrict__ g
       const long long o = node * lay.ns + n * lay.rs + c;
       const double gv = g[o], dv = delta[o], av = ad[o];
       acc += (0.5 * dv * av - dv * gv) / nrm;
-      du[((long long)n * 3 + c) * nodes + node] = (float)((av - gv) * dinv[node * 3 + c] * s / nrm);
+      du[((long long)n * 3 + c) * nodes + node] = (float)((av - gv) * sqrt((double)dinv[node * 3 + c]) * s / nrm);
     }
   }
   acc = block_sum(acc);
@@ -666,10 +669,11 @@ static void ensure_scal(NetImpl* N, int n) {
   N->part.ensure(sizeof(double) * (size_t)scal_width(n) * 64);
 }
 
-static void sample_scale(NetImpl* N, const float* g, VLay lay, long long nodes, int n) {
+static void sample_scale(NetImpl* N, const float* g, VLay lay, long long nodes, int n, const float* dinv) {
   const int nb = (int)std::min<long long>(64, (nodes * 3 + 255) / 256);
   ensure_scal(N, n);
-  hipLaunchKernelGGL(k_sample_absmax, dim3(nb, n), dim3(256), 0, N->stream, g, lay, nodes, N->part.as<double>());
+  hipLaunchKernelGGL(k_sample_absmax, dim3(nb, n), dim3(256), 0, N->stream, g, lay, nodes, dinv,
+                     N->part.as<double>());
   DFMI_HIP(hipGetLastError());
   reduce_max(N->part.as<double>(), nb, n, N->scal.as<double>(), N->stream);
 }
@@ -703,16 +707,16 @@ static void stiffness_feature(NetImpl* N, const float* dinv, long long nodes) {
 
 void net_invalidate_feature(NetImpl* N) { N->feat_key = nullptr; }
 
-// delta = Dinv (g + s N(g / s, f)) for n samples of level-1 geometry g1 in layout lay
+// delta = Dinv g + Dinv^(1/2) s N(Dinv^(1/2) g / s, f) for n samples of level-1 geometry g1 in layout lay
 void net_correct(NetImpl* N, const Geom& g1, const float* g, VLay lay, int n, const float* dinv, const int* active,
                  float* delta) {
   const long long nodes = g1.nodes();
   stiffness_feature(N, dinv, nodes);
-  sample_scale(N, g, lay, nodes, n);
+  sample_scale(N, g, lay, nodes, n, dinv);
   const NetDims d1{g1.nx, g1.ny, g1.nz};
   N->prepare(n, d1);
   hipLaunchKernelGGL(k_features, dim3((unsigned)((nodes + 255) / 256), n), dim3(256), 0, N->stream, g, lay, nodes,
-                     N->scal.as<double>(), N->feat.as<float>(), N->act[A_X].as<float>());
+                     N->scal.as<double>(), dinv, N->feat.as<float>(), N->act[A_X].as<float>());
   DFMI_HIP(hipGetLastError());
   net_forward(N, n, d1, N->act[A_X].as<float>());
   hipLaunchKernelGGL(k_delta, dim3((unsigned)((nodes + 255) / 256), n), dim3(256), 0, N->stream, g, lay, nodes,

@@ -270,7 +270,7 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
 }
 
 // Learned coarse correction on the fine level, between the coarse-grid correction and the
-// post-smoother: g = R (b - A x) on level 1, delta = Dinv (g + s N(g / s, f)), scaled per column by the
+// post-smoother: g = R (b - A x) on level 1, delta = Dinv g + Dinv^(1/2) s N(Dinv^(1/2) g / s, f), scaled per column by the
 // energy-optimal step max(0, delta.g / delta.A_1 delta), then x += P delta.  With the Galerkin level-1
 // operator A_1 = P^T A P this step never increases the A-norm error of x.  The same residual g is
 // what the training harvest records.

@@ -150,7 +150,7 @@ int dfmi_mma_step(int64_t n, int32_t iteration, double* x_dev, double* xold1_dev
  * hand-written NumPy MLP with backward and Adam of deep_frame/topology_neural.py:59-113).
  * A 4-level 3D U-Net (channels 16-32-64-64, BF16 weights, FP32 activations and accumulation) on the
  * level-1 node grid maps the level-1 residual g = R (b - A x) and a stiffness feature to the correction
- * delta = Dinv (g + s N(g / s, f)) that the fp32 V-cycle adds, with an energy-optimal step per column,
+ * delta = Dinv g + Dinv^(1/2) s N(Dinv^(1/2) g / s, f) that the fp32 V-cycle adds, with an energy-optimal step per column,
  * between the fine coarse-grid correction and the post-smoother.  Parameters: one flat FP32 array
  * (per layer: weights [c_out][c_in][27], then biases; up layers [c_coarse][c_fine][27]). */
 int dfmi_net_create(dfmi_net** out);

@@ -14,12 +14,17 @@ Definitions (shared with the HIP kernels):
     coarse grid back to the fine grid size it came from;
   * weights are rounded to BF16 (round to nearest even) before use; the master
     copy and the Adam state are FP32 on the GPU.
-Layers (name, kind, c_in, c_out): see LAYERS.  Input channels: the restricted
-residual g / s (3) and the stiffness feature f (3), where s is the per-sample max |g|.
-The network output u (3 channels) gives the correction delta = Dinv (s (g / s + u)).
+Layers (name, kind, c_in, c_out): see LAYERS.  The network works in the Jacobi-symmetrised
+space: input channels are h / s (3), with h = Dinv^(1/2) g the scaled level-1 residual and s the
+per-sample max |h|, and the stiffness feature f (3).  The output u (3 channels) gives the
+correction delta = Dinv^(1/2) (h + s u) = Dinv g + Dinv^(1/2) s u, so the untrained network
+(zero output layer) is the Jacobi correction and the energy of the learned part,
+0.5 s^2 u^T (D^-1/2 A_1 D^-1/2) u, has the scale of u whatever the stiffness contrast.
 
 Loss (energy of the prolonged correction, exact for Galerkin coarse operators,
-because P^T A P = A_1):  L = sum_n [0.5 delta_n^T A_1 delta_n - delta_n^T g_n] / (g_n^T Dinv g_n).
+because P^T A P = A_1):  L = sum_n [0.5 delta_n^T A_1 delta_n - delta_n^T g_n] / (g_n^T Dinv g_n);
+the Jacobi correction alone gives L >= -0.5 per sample only when it is a descent step, and the
+minimum over all delta is -0.5 g^T A_1^-1 g / g^T Dinv g.
 Parity of the forward pass and the gradients is pinned by finite differences of this
 module (tests/test_oracle.py); the GPU is checked against this module.
 """
@@ -189,32 +194,33 @@ def backward(params, acts, dout, rounded=True):
 
 def features(g, diag, dinv):
     """Network input (N, 6, ...) and per-sample scales from level-1 residuals g (N, 3, ...) and diag(A_1) (3, ...)."""
-    s = np.max(np.abs(g), axis=(1, 2, 3, 4))
+    h = g * np.sqrt(dinv)[None]
+    s = np.max(np.abs(h), axis=(1, 2, 3, 4))
     s = np.where(s > 0, s, 1.0)
     dmax = diag.max()
     f = np.where(diag > 0, np.log10(np.maximum(diag, 1e-30) / dmax) / 9.0 + 1.0, 0.0)
-    x = np.concatenate([g / s[:, None, None, None, None], np.broadcast_to(f, g.shape)], axis=1)
+    x = np.concatenate([h / s[:, None, None, None, None], np.broadcast_to(f, g.shape)], axis=1)
     return x, s
 
 
 def correction(params, g, diag, dinv):
-    """delta (N, 3, ...) = Dinv (s (g / s + u))."""
+    """delta (N, 3, ...) = Dinv g + Dinv^(1/2) s u."""
     x, s = features(g, diag, dinv)
     u = forward(params, x)
-    return dinv[None] * s[:, None, None, None, None] * (g / s[:, None, None, None, None] + u)
+    return dinv[None] * g + np.sqrt(dinv)[None] * s[:, None, None, None, None] * u
 
 
 def energy_loss(params, g, diag, dinv, apply_a, keep=False, rounded=True):
     """Normalised energy loss and dL/du; apply_a maps (3, X, Y, Z) -> A_1 times it."""
     x, s = features(g, diag, dinv)
     u, acts = forward(params, x, keep=True, rounded=rounded)
-    sc = s[:, None, None, None, None]
-    delta = dinv[None] * sc * (g / sc + u)
+    sq = np.sqrt(dinv)
+    delta = dinv[None] * g + sq[None] * s[:, None, None, None, None] * u
     loss, du = 0.0, np.zeros_like(u)
     for n in range(g.shape[0]):
         ad = apply_a(delta[n])
         norm = float(np.sum(g[n] * dinv * g[n]))
         norm = norm if norm > 0 else 1.0
         loss += (0.5 * np.sum(delta[n] * ad) - np.sum(delta[n] * g[n])) / norm
-        du[n] = (ad - g[n]) * dinv * s[n] / norm
+        du[n] = (ad - g[n]) * sq * s[n] / norm
     return (loss, du, acts) if keep else loss
