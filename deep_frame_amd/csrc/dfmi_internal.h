@@ -134,9 +134,11 @@ void prolong_add(const Geom& coarse, const T* c, VLay clay, const Geom& fine, co
                  VLay flay, int k, const int* active, hipStream_t s);
 void coarsest_assemble(const Geom& g, const double* st, const int* live_index, int n, double* dense, hipStream_t s);
 void dense_sweep_inverse(double* a, int n, hipStream_t s);
+// work_in, work_out: k * n doubles; work_split: coarsest_splits() * k * n doubles
 template <typename T>
 void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out, VLay lay,
-                    double* work_in, double* work_out, int k, const int* active, hipStream_t s);
+                    double* work_in, double* work_out, double* work_split, int k, const int* active, hipStream_t s);
+int coarsest_splits();
 
 // ---- vectors ----------------------------------------------------------------------------
 void reduce_partials(const double* partial, int nblocks, int k, double* out, hipStream_t s);

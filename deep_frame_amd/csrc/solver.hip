@@ -75,7 +75,7 @@ struct Hierarchy {
   bool built = false;
   std::vector<std::unique_ptr<Level>> levels;
   int coarsest_n = 0;
-  DevBuf live_dofs, ainv, work_in, work_out;
+  DevBuf live_dofs, ainv, work_in, work_out, work_split;
   double setup_s = 0.0;
 };
 
@@ -319,7 +319,8 @@ static void vcycle(VArgs& v, int idx, const T* b) {
   hipStream_t s = v.h->stream;
   if (idx == (int)H.levels.size() - 1) {
     coarsest_solve<T>(L.g, H.ainv.as<double>(), H.live_dofs.as<int>(), H.coarsest_n, b, L.x.as<T>(),
-                      level_layout(L, idx, v.k), H.work_in.as<double>(), H.work_out.as<double>(), v.k, v.active, s);
+                      level_layout(L, idx, v.k), H.work_in.as<double>(), H.work_out.as<double>(),
+                      H.work_split.as<double>(), v.k, v.active, s);
     return;
   }
   Level& C = *H.levels[idx + 1];
@@ -356,6 +357,7 @@ static void ensure_level_vectors(Handle* h, Hierarchy& H, int k) {
   if (H.coarsest_n > 0) {
     H.work_in.ensure(sizeof(double) * (size_t)k * H.coarsest_n);
     H.work_out.ensure(sizeof(double) * (size_t)k * H.coarsest_n);
+    H.work_split.ensure(sizeof(double) * (size_t)coarsest_splits() * k * H.coarsest_n);
   }
 }
 

@@ -505,33 +505,52 @@ __global__ void k_gather_live(const T* __restrict__ v, VLay lay, const int* __re
       (active && !active[r]) ? 0.0 : (double)v[(d / 3) * lay.ns + (long long)r * lay.rs + d % 3];
 }
 
-// out[r][j] = sum_i in[r][i] ainv[i][j]; 256 columns per block, 8 right-hand sides per block row
-__global__ __launch_bounds__(256) void k_dense_rows(const double* __restrict__ in, const double* __restrict__ ainv,
-                                                    int n, int k, double* __restrict__ out) {
-  __shared__ double tile[8][64];
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  const int r0 = blockIdx.y * 8;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    for (int v = threadIdx.x; v < 8 * 64; v += 256) {
-      const int rr = v / 64, ii = v % 64;
-      tile[rr][ii] = (r0 + rr < k && i0 + ii < n) ? in[(long long)(r0 + rr) * n + i0 + ii] : 0.0;
+// Coarsest solve out = in A^-1 as a split-K GEMM: block (64 columns j) x (up to 64 right-hand sides)
+// x (one slice of the i range); A^-1 is read once per right-hand-side tile, and the slices are
+// summed in a fixed order afterwards.  Thread (j, q) accumulates right-hand sides q, q + 4, ...
+constexpr int DJ = 64, DI = 32, DR = 64, DSPLIT = 16;
+__global__ __launch_bounds__(256) void k_dense_gemm(const double* __restrict__ in, const double* __restrict__ ainv,
+                                                    int n, int k, int per, double* __restrict__ partial) {
+  __shared__ double tile[DR][DI + 1];
+  const int tj = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int j = blockIdx.x * DJ + tj, r0 = blockIdx.y * DR, split = blockIdx.z;
+  const int ib = split * per, ie = min(n, ib + per);
+  double acc[DR / 4];
+#pragma unroll
+  for (int m = 0; m < DR / 4; ++m) acc[m] = 0.0;
+  for (int i0 = ib; i0 < ie; i0 += DI) {
+    for (int v = threadIdx.x; v < DR * DI; v += 256) {
+      const int rr = v / DI, ii = v % DI;
+      tile[rr][ii] = (r0 + rr < k && i0 + ii < ie) ? in[(long long)(r0 + rr) * n + i0 + ii] : 0.0;
     }
     __syncthreads();
     if (j < n) {
-      const int lim = min(64, n - i0);
+      const int lim = min(DI, ie - i0);
       for (int ii = 0; ii < lim; ++ii) {
         const double a = ainv[(long long)(i0 + ii) * n + j];
 #pragma unroll
-        for (int rr = 0; rr < 8; ++rr) acc[rr] = fma(tile[rr][ii], a, acc[rr]);
+        for (int m = 0; m < DR / 4; ++m) acc[m] = fma(tile[q + 4 * m][ii], a, acc[m]);
       }
     }
     __syncthreads();
   }
-  if (j < n)
-    for (int rr = 0; rr < 8; ++rr)
-      if (r0 + rr < k) out[(long long)(r0 + rr) * n + j] = acc[rr];
+  if (j >= n) return;
+#pragma unroll
+  for (int m = 0; m < DR / 4; ++m) {
+    const int r = r0 + q + 4 * m;
+    if (r < k) partial[((long long)split * k + r) * n + j] = acc[m];
+  }
 }
+
+__global__ void k_sum_splits(const double* __restrict__ partial, int splits, long long kn, double* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kn) return;
+  double s = 0.0;
+  for (int c = 0; c < splits; ++c) s += partial[(long long)c * kn + e];
+  out[e] = s;
+}
+
+int coarsest_splits() { return DSPLIT; }
 
 template <typename T>
 __global__ void k_scatter_live(const double* __restrict__ in, VLay lay, const int* __restrict__ live_dofs, int n,
@@ -545,19 +564,24 @@ __global__ void k_scatter_live(const double* __restrict__ in, VLay lay, const in
 
 template <typename T>
 void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int n, const T* rhs, T* out, VLay lay,
-                    double* work_in, double* work_out, int k, const int* active, hipStream_t s) {
+                    double* work_in, double* work_out, double* work_split, int k, const int* active, hipStream_t s) {
   DFMI_HIP(hipMemsetAsync(out, 0, sizeof(T) * (size_t)k * g.nodes() * 3, s));
   if (n == 0) return;
   const unsigned b = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_gather_live<T>, dim3(b, k), dim3(256), 0, s, rhs, lay, live_dofs, n, work_in, active);
-  hipLaunchKernelGGL(k_dense_rows, dim3(b, (k + 7) / 8), dim3(256), 0, s, work_in, ainv, n, k, work_out);
+  const int per = ((n + DSPLIT - 1) / DSPLIT + DI - 1) / DI * DI;
+  const int splits = (n + per - 1) / per;
+  hipLaunchKernelGGL(k_dense_gemm, dim3((n + DJ - 1) / DJ, (k + DR - 1) / DR, splits), dim3(256), 0, s, work_in,
+                     ainv, n, k, per, work_split);
+  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)(((long long)k * n + 255) / 256)), dim3(256), 0, s, work_split,
+                     splits, (long long)k * n, work_out);
   hipLaunchKernelGGL(k_scatter_live<T>, dim3(b, k), dim3(256), 0, s, work_out, lay, live_dofs, n, out, active);
   DFMI_HIP(hipGetLastError());
 }
 
 template void coarsest_solve<float>(const Geom&, const double*, const int*, int, const float*, float*, VLay, double*,
-                                    double*, int, const int*, hipStream_t);
+                                    double*, double*, int, const int*, hipStream_t);
 template void coarsest_solve<double>(const Geom&, const double*, const int*, int, const double*, double*, VLay,
-                                     double*, double*, int, const int*, hipStream_t);
+                                     double*, double*, double*, int, const int*, hipStream_t);
 
 }  // namespace dfmi
