@@ -132,6 +132,9 @@ void restrict_full(const Geom& fine, const T* f, VLay flay, const Geom& coarse, 
 template <typename T>
 void prolong_add(const Geom& coarse, const T* c, VLay clay, const Geom& fine, const uint8_t* fmask, T* f,
                  VLay flay, int k, const int* active, hipStream_t s);
+// interleaved [node][r][3] <-> column-major [r][node][3] (to_columns: interleaved -> column-major)
+template <typename T>
+void relayout(const T* in, T* out, long long nodes, int k, bool to_columns, const int* active, hipStream_t s);
 void coarsest_assemble(const Geom& g, const double* st, const int* live_index, int n, double* dense, hipStream_t s);
 void dense_sweep_inverse(double* a, int n, hipStream_t s);
 // work_in, work_out: k * n doubles; work_split: coarsest_splits() * k * n doubles

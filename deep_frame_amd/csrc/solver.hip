@@ -332,11 +332,26 @@ static void vcycle(VArgs& v, int idx, const T* b) {
   a.y = L.res.as<T>();
   a.b = b;
   level_apply<T>(v, idx, MODE_RESID, a);
-  restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
-                   level_layout(C, idx + 1, v.k), v.k, v.active, s);
+  // Between the column-major fine level and the interleaved level 1 the transfers run column-major on
+  // both sides (coalesced), with a tiled layout change of the level-1 vector through its free res buffer.
+  const bool change = idx == 0 && v.k > 1;
+  if (change) {
+    restrict_full<T>(L.g, L.res.as<T>(), column_major(L.g), C.g, C.mask.as<uint8_t>(), C.res.as<T>(),
+                     column_major(C.g), v.k, v.active, s);
+    relayout<T>(C.res.as<T>(), C.rhs.as<T>(), C.g.nodes(), v.k, false, v.active, s);
+  } else {
+    restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
+                     level_layout(C, idx + 1, v.k), v.k, v.active, s);
+  }
   vcycle<T>(v, idx + 1, C.rhs.as<T>());
-  prolong_add<T>(C.g, C.x.as<T>(), level_layout(C, idx + 1, v.k), L.g, L.mask.as<uint8_t>(), L.x.as<T>(),
-                 level_layout(L, idx, v.k), v.k, v.active, s);
+  if (change) {
+    relayout<T>(C.x.as<T>(), C.res.as<T>(), C.g.nodes(), v.k, true, v.active, s);
+    prolong_add<T>(C.g, C.res.as<T>(), column_major(C.g), L.g, L.mask.as<uint8_t>(), L.x.as<T>(), column_major(L.g),
+                   v.k, v.active, s);
+  } else {
+    prolong_add<T>(C.g, C.x.as<T>(), level_layout(C, idx + 1, v.k), L.g, L.mask.as<uint8_t>(), L.x.as<T>(),
+                   level_layout(L, idx, v.k), v.k, v.active, s);
+  }
   if (idx == 0 && (v.h->net || v.h->harvest)) learned_correction<T>(v, b);
   chebyshev<T>(v, idx, b, false, count);
 }

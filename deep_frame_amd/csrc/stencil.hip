@@ -321,6 +321,58 @@ void prolong_add(const Geom& coarse, const T* c, VLay clay, const Geom& fine, co
   DFMI_HIP(hipGetLastError());
 }
 
+// Layout change of a level vector between RHS-interleaved [node][r][3] and column-major [r][node][3]
+// through an LDS tile of RN nodes x all columns, so that both the reads and the writes are contiguous
+// runs (the fine level is column-major, the coarse levels interleaved).
+constexpr int RN = 32;
+template <typename T, bool TO_COLUMNS>
+__global__ __launch_bounds__(256) void k_relayout(const T* __restrict__ in, T* __restrict__ out, int N, int k,
+                                                  const int* active) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);  // [RN][3k + 1]
+  const int w = 3 * k, ld = w + 1;
+  const int n0 = (int)xcd_block(blockIdx.x, gridDim.x) * RN;
+  const int cnt = min(RN, N - n0);
+  if (TO_COLUMNS) {
+    for (int v = threadIdx.x; v < cnt * w; v += 256) {
+      const int j = v / w, e = v - j * w;
+      tile[j * ld + e] = in[(long long)n0 * w + v];
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < k * cnt * 3; v += 256) {
+      const int r = v / (cnt * 3), rem = v - r * (cnt * 3), j = rem / 3, c = rem - j * 3;
+      if (active && !active[r]) continue;
+      out[((long long)r * N + n0 + j) * 3 + c] = tile[j * ld + 3 * r + c];
+    }
+  } else {
+    for (int v = threadIdx.x; v < k * cnt * 3; v += 256) {
+      const int r = v / (cnt * 3), rem = v - r * (cnt * 3), j = rem / 3, c = rem - j * 3;
+      tile[j * ld + 3 * r + c] = in[((long long)r * N + n0 + j) * 3 + c];
+    }
+    __syncthreads();
+    for (int v = threadIdx.x; v < cnt * w; v += 256) {
+      const int j = v / w, e = v - j * w;
+      if (active && !active[e / 3]) continue;
+      out[(long long)n0 * w + v] = tile[j * ld + e];
+    }
+  }
+}
+
+template <typename T>
+void relayout(const T* in, T* out, long long nodes, int k, bool to_columns, const int* active, hipStream_t s) {
+  if (nodes * 3 * (long long)k > (long long)INT32_MAX) throw std::runtime_error("relayout: level too large");
+  const unsigned blocks = (unsigned)((nodes + RN - 1) / RN);
+  const size_t lds = sizeof(T) * (size_t)RN * (3 * k + 1);
+  if (lds > 64 * 1024) throw std::runtime_error("relayout: too many right-hand sides for one tile");
+  if (to_columns)
+    hipLaunchKernelGGL((k_relayout<T, true>), dim3(blocks), dim3(256), lds, s, in, out, (int)nodes, k, active);
+  else
+    hipLaunchKernelGGL((k_relayout<T, false>), dim3(blocks), dim3(256), lds, s, in, out, (int)nodes, k, active);
+  DFMI_HIP(hipGetLastError());
+}
+template void relayout<float>(const float*, float*, long long, int, bool, const int*, hipStream_t);
+template void relayout<double>(const double*, double*, long long, int, bool, const int*, hipStream_t);
+
 template void restrict_full<float>(const Geom&, const float*, VLay, const Geom&, const uint8_t*, float*, VLay, int,
                                    const int*, hipStream_t);
 template void restrict_full<double>(const Geom&, const double*, VLay, const Geom&, const uint8_t*, double*, VLay, int,

@@ -82,31 +82,41 @@ void reduce_max(const double* partial, int nblocks, int k, double* out, hipStrea
 }
 
 // first Chebyshev step from a zero iterate: x = c * Dinv * b
+// column-major: grid (blocks over 3 * nodes, columns); interleaved: one thread per stored value
+template <typename T>
+__global__ void k_cheb_start_cols(const T* __restrict__ b, const T* __restrict__ dinv, double c, T* __restrict__ x,
+                                  long long n3, const int* active) {
+  const int r = blockIdx.y;
+  if (active && !active[r]) return;
+  const T cc = (T)c;
+  const long long off = (long long)r * n3;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n3; i += (long long)gridDim.x * 256)
+    x[off + i] = cc * dinv[i] * b[off + i];
+}
+
 template <typename T>
 __global__ void k_cheb_start(const T* __restrict__ b, const T* __restrict__ dinv, double c, T* __restrict__ x,
-                             long long nodes, int k, const int* active, VLay lay) {
-  // one thread per stored value, in memory order; recover (node, column, component) from the layout
-  const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                             int nodes, int k, const int* active) {
+  // interleaved: o = n * 3k + 3r + comp
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= nodes * 3 * k) return;
-  long long n;
-  int r;
-  if (lay.rs == 3) {  // interleaved: o = n * 3k + 3r + c
-    n = o / lay.ns;
-    r = (int)((o - n * lay.ns) / 3);
-  } else {  // column-major: o = r * 3N + 3n + c
-    r = (int)(o / lay.rs);
-    n = (o - (long long)r * lay.rs) / 3;
-  }
+  const int w = 3 * k, n = o / w, rem = o - n * w, r = rem / 3, comp = rem - r * 3;
   if (active && !active[r]) return;
-  const int comp = (int)(o % 3);
   x[o] = (T)c * dinv[n * 3 + comp] * b[o];
 }
 
 template <typename T>
 void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active, VLay lay,
                 hipStream_t s) {
-  hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * 3 * k + 255) / 256)), dim3(256), 0, s, b, dinv, c, x,
-                     nodes, k, active, lay);
+  if (lay.rs != 3 || k == 1) {  // column-major (or a single column)
+    const long long n3 = nodes * 3;
+    const unsigned bx = (unsigned)std::min<long long>((n3 + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_cheb_start_cols<T>, dim3(bx, k), dim3(256), 0, s, b, dinv, c, x, n3, active);
+  } else {
+    if (nodes * 3 * (long long)k > (long long)INT32_MAX) throw std::runtime_error("cheb_start: level too large");
+    hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * 3 * k + 255) / 256)), dim3(256), 0, s, b, dinv, c,
+                       x, (int)nodes, k, active);
+  }
   DFMI_HIP(hipGetLastError());
 }
 template void cheb_start<float>(const float*, const float*, double, float*, long long, int, const int*, VLay,
