@@ -24,6 +24,19 @@ CONFIGS = {
     "cantilever": {"shape": (48, 24, 24), "rhs": 1, "levels": 3},
 }
 HBM_PEAK_GBS = 8000.0
+TRAFFIC = ROOT / "profiles" / "traffic_k_fine_apply_fp32_frame_256x256x48_42rhs.txt"
+
+
+def measured_traffic(config):
+    """HBM bytes per launch of the fine FP32 operator from the committed rocprofv3 --pmc summary (c3 only)."""
+    if config != "frame" or not TRAFFIC.exists():
+        return None
+    vals = {}
+    for line in TRAFFIC.read_text().splitlines():
+        if line and not line.startswith("#"):
+            key, value = line.split()
+            vals[key] = float(value)
+    return vals["fetch_bytes"] + vals["write_bytes"]
 
 
 def parse():
@@ -166,7 +179,8 @@ def main():
                    "mixed_s": t_mixed, "fp64_s": t_fp64, "iterations": max(r[0] for r in reports),
                    "max_rel_res": max(r[1] for r in reports)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_fine_apply<float, MODE_APPLY>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config),
+                     "traffic_source": str(TRAFFIC.relative_to(ROOT)), "kernel": "k_fine_apply<float, MODE_APPLY>",
                      "ms_per_launch": ms.value, "algorithmic_bytes": alg_bytes},
         "cpu_baseline": baseline,
     }
