@@ -87,6 +87,9 @@ enum ApplyMode : int {
   MODE_CHEB = 2,    // y = x + c1 (x - xp) + c2 Dinv (b - A x)   (y may alias xp)
   MODE_DOT = 3,     // y = M A M x and partial sums of x.y per workgroup
   MODE_ENERGY = 4,  // fine only: per-element u_e^T Ke u_e (no modulus)
+  MODE_CHEB0 = 5,   // fine only: first Chebyshev step from zero, the operand x1 = c0 Dinv b is formed
+                    // while staging (a.x unused): y = x1 + c1 x1 + c2 Dinv (b - A x1)
+  MODE_CHEB1 = 6,   // fine only: Chebyshev step whose previous iterate is x1 = c0 Dinv b (a.xp unused)
 };
 
 struct ApplyArgs {
@@ -96,6 +99,7 @@ struct ApplyArgs {
   const void* xp;     // previous iterate for CHEB
   const void* dinv;   // [nodes][3] inverse diagonal (masked), CHEB only
   double c1, c2;      // Chebyshev coefficients, CHEB only
+  double c0;          // first Chebyshev iterate x1 = c0 Dinv b, CHEB0 / CHEB1 only
   double* partial;    // [k][nblocks] partial dot products, DOT only
   const int* active;  // per-column flag (device), nullptr = all active
   int k;
@@ -163,9 +167,10 @@ template <typename T>
 void copy_columns(const T* src, T* dst, long long n, int k, const int* select, hipStream_t s);
 template <typename T>
 void fill_zero(T* x, long long n, hipStream_t s);
+// partial_rr: [k][nblocks] sums of r.r, then (z != nullptr) [k][nblocks] sums of r.z
 template <typename T>
 void pcg_update(T* x, T* r, const T* p, const T* ap, const double* alpha, long long n, int k, const int* active,
-                double* partial_rr, int nblocks, hipStream_t s);
+                double* partial_rr, int nblocks, hipStream_t s, const T* z = nullptr);
 template <typename T>
 void pcg_direction(T* p, const T* z, const double* beta, long long n, int k, const int* active, hipStream_t s);
 void add_scaled(double* x, const float* d, const double* scale, long long n, int k, const int* active, hipStream_t s);

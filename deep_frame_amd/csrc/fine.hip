@@ -186,8 +186,20 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
       if (in && so[k] >= 0) {
         const int n = ip * plane_nodes + so[k];
         const uint8_t m = mask[n];
-        const T* src = X + 3LL * n;
-        const T v0 = src[0], v1 = src[1], v2 = src[2];
+        T v0, v1, v2;
+        if (MODE == MODE_CHEB0) {  // x1 = c0 Dinv b
+          const T* bs = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * n;
+          const T* ds = reinterpret_cast<const T*>(a.dinv) + 3LL * n;
+          const T c0 = (T)a.c0;
+          v0 = c0 * ds[0] * bs[0];
+          v1 = c0 * ds[1] * bs[1];
+          v2 = c0 * ds[2] * bs[2];
+        } else {
+          const T* src = X + 3LL * n;
+          v0 = src[0];
+          v1 = src[1];
+          v2 = src[2];
+        }
         nv[k][0] = (m & 1) ? v0 : T(0);
         nv[k][1] = (m & 2) ? v1 : T(0);
         nv[k][2] = (m & 4) ? v2 : T(0);
@@ -226,14 +238,14 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     // epilogue operands of node (i, ej, el), requested early
     T bv[3] = {T(0), T(0), T(0)}, xpv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)};
     if (out) {
-      if (MODE == MODE_RESID || MODE == MODE_CHEB) {
+      if (MODE == MODE_RESID || MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
         const T* bp = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * nn;
         bv[0] = bp[0];
         bv[1] = bp[1];
         bv[2] = bp[2];
       }
-      if (MODE == MODE_CHEB) {
-        if (a.xp) {
+      if (MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
+        if (MODE == MODE_CHEB && a.xp) {
           const T* pp = reinterpret_cast<const T*>(a.xp) + (long long)r * col + 3LL * nn;
           xpv[0] = pp[0];
           xpv[1] = pp[1];
@@ -309,9 +321,10 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
             dot += (double)xs[c] * (double)yv;
           } else if (MODE == MODE_RESID) {
             Y[3LL * nn + c] = on ? bv[c] - s0[c] : T(0);
-          } else if (MODE == MODE_CHEB) {
+          } else if (MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
             const T c1 = (T)a.c1, c2 = (T)a.c2;
-            Y[3LL * nn + c] = on ? xs[c] + c1 * (xs[c] - xpv[c]) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
+            const T prev = MODE == MODE_CHEB1 ? (T)a.c0 * dv[c] * bv[c] : xpv[c];
+            Y[3LL * nn + c] = on ? xs[c] + c1 * (xs[c] - prev) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
           }
         }
       }
@@ -412,6 +425,8 @@ void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlo
     case MODE_APPLY: DFMI_LAUNCH(MODE_APPLY); break;
     case MODE_RESID: DFMI_LAUNCH(MODE_RESID); break;
     case MODE_CHEB: DFMI_LAUNCH(MODE_CHEB); break;
+    case MODE_CHEB0: DFMI_LAUNCH(MODE_CHEB0); break;
+    case MODE_CHEB1: DFMI_LAUNCH(MODE_CHEB1); break;
     case MODE_DOT: DFMI_LAUNCH(MODE_DOT); break;
     default: throw std::runtime_error("fine_apply: bad mode");
   }

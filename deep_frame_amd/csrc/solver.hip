@@ -235,7 +235,37 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
   T* other = L.xa.as<T>();
   const T* dinv = level_dinv<T>(L);
   bool prev_zero;
-  if (from_zero) {
+  int j0 = 1;
+  if (from_zero && idx == 0 && count >= 2) {
+    // fine level: x1 = Dinv b / theta is formed inside the first step (no separate pass) and re-formed
+    // from b as the previous iterate of the second step
+    const double nr = 1.0 / (2.0 * sigma - rho);
+    ApplyArgs a{};
+    a.b = b;
+    a.dinv = dinv;
+    a.c0 = 1.0 / theta;
+    a.c1 = nr * rho;
+    a.c2 = 2.0 * nr / delta;
+    a.y = cur;
+    level_apply<T>(v, idx, MODE_CHEB0, a);
+    rho = nr;
+    if (count >= 3) {
+      const double nr2 = 1.0 / (2.0 * sigma - rho);
+      ApplyArgs a2{};
+      a2.x = cur;
+      a2.y = other;
+      a2.b = b;
+      a2.dinv = dinv;
+      a2.c0 = 1.0 / theta;
+      a2.c1 = nr2 * rho;
+      a2.c2 = 2.0 * nr2 / delta;
+      level_apply<T>(v, idx, MODE_CHEB1, a2);
+      std::swap(cur, other);
+      rho = nr2;
+    }
+    prev_zero = false;
+    j0 = std::min(count, 3);
+  } else if (from_zero) {
     cheb_start<T>(b, dinv, 1.0 / theta, cur, L.g.nodes(), v.k, v.active, level_layout(L, idx, v.k), v.h->stream);
     prev_zero = true;
   } else {
@@ -251,7 +281,7 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
     std::swap(cur, other);
     prev_zero = false;
   }
-  for (int j = 1; j < count; ++j) {
+  for (int j = j0; j < count; ++j) {
     const double nr = 1.0 / (2.0 * sigma - rho);
     ApplyArgs a{};
     a.x = cur;
@@ -379,7 +409,7 @@ static void ensure_level_vectors(Handle* h, Hierarchy& H, int k) {
 static void ensure_workspace(Handle* h, int k) {
   const size_t n = sizeof(double) * (size_t)k * col(h->pad);
   for (DevBuf* b : {&h->b, &h->x, &h->r, &h->z, &h->zn, &h->p, &h->ap, &h->tmp}) b->ensure(n);
-  const int nb = std::max(fine_blocks(h->pad), reduce_blocks(col(h->pad)));
+  const int nb = std::max(fine_blocks(h->pad), 2 * reduce_blocks(col(h->pad)));
   h->partial.ensure(sizeof(double) * (size_t)k * nb);
   h->scal.ensure(sizeof(double) * (size_t)k * 16);
   h->active.ensure(sizeof(int) * (size_t)k);
@@ -804,8 +834,10 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
       fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_DOT, a, s, &nbf);
       reduce_partials(part, nbf, k, pap, s);
       pcg_scalars_alpha(rz, pap, alpha, k, act, s);
-      pcg_update<float>(d32, r32, p32, ap32, alpha, n, k, act, part, nbr, s);
+      // also r_new . z_old (z_old is the last V-cycle output, still in the fine level's x)
+      pcg_update<float>(d32, r32, p32, ap32, alpha, n, k, act, part, nbr, s, F.x.as<float>());
       reduce_partials(part, nbr, k, rr, s);
+      reduce_partials(part + (size_t)k * nbr, nbr, k, rzc, s);
       DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
       sync(h);
       ++step;
@@ -813,9 +845,7 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
         if (inner[c] && std::sqrt(rrh[c]) < target[c]) inner[c] = 0;
       if (!count(inner)) break;
       DFMI_HIP(hipMemcpyAsync(act, inner.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
-      // flexible (Polak-Ribiere) beta: r_new . z_old is taken before the V-cycle overwrites z
-      column_dot<float>(r32, F.x.as<float>(), n, k, part, nbr, act, s);
-      reduce_partials(part, nbr, k, rzc, s);
+      // flexible (Polak-Ribiere) beta with r_new . z_old from the update above
       vcycle<float>(v, 0, r32);
       column_dot<float>(r32, F.x.as<float>(), n, k, part, nbr, act, s);
       reduce_partials(part, nbr, k, rzn, s);

@@ -219,13 +219,14 @@ void add_scaled(double* x, const float* d, const double* scale, long long n, int
   DFMI_HIP(hipGetLastError());
 }
 
-// x += alpha p, r -= alpha Ap, partial |r|^2 (columns with alpha = 0 are left unchanged)
+// x += alpha p, r -= alpha Ap, partial |r|^2 (columns with alpha = 0 are left unchanged); with z != nullptr
+// also the partial sums of r_new . z (the r . z_old term of the Polak-Ribiere beta) in partial[k * nblocks ...]
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_update(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
                                                      const T* __restrict__ ap, const double* alpha, long long n,
-                                                     const int* active, double* partial) {
+                                                     const int* active, double* partial, const T* __restrict__ z) {
   const int c = blockIdx.y;
-  double acc = 0.0;
+  double acc = 0.0, acz = 0.0;
   if (!active || active[c]) {
     const T a = (T)alpha[c];
     const long long off = (long long)c * n;
@@ -234,22 +235,28 @@ __global__ __launch_bounds__(256) void k_pcg_update(T* __restrict__ x, T* __rest
       const T rv = r[off + i] - a * ap[off + i];
       r[off + i] = rv;
       acc += (double)rv * (double)rv;
+      if (z) acz += (double)rv * (double)z[off + i];
     }
   }
   acc = block_sum(acc);
   if (threadIdx.x == 0) partial[(long long)c * gridDim.x + blockIdx.x] = acc;
+  if (z) {
+    acz = block_sum(acz);
+    if (threadIdx.x == 0) partial[((long long)gridDim.y + c) * gridDim.x + blockIdx.x] = acz;
+  }
 }
 
 template <typename T>
 void pcg_update(T* x, T* r, const T* p, const T* ap, const double* alpha, long long n, int k, const int* active,
-                double* partial_rr, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_update<T>, dim3(nblocks, k), dim3(256), 0, s, x, r, p, ap, alpha, n, active, partial_rr);
+                double* partial_rr, int nblocks, hipStream_t s, const T* z) {
+  hipLaunchKernelGGL(k_pcg_update<T>, dim3(nblocks, k), dim3(256), 0, s, x, r, p, ap, alpha, n, active, partial_rr,
+                     z);
   DFMI_HIP(hipGetLastError());
 }
 template void pcg_update<float>(float*, float*, const float*, const float*, const double*, long long, int, const int*,
-                                double*, int, hipStream_t);
+                                double*, int, hipStream_t, const float*);
 template void pcg_update<double>(double*, double*, const double*, const double*, const double*, long long, int,
-                                 const int*, double*, int, hipStream_t);
+                                 const int*, double*, int, hipStream_t, const double*);
 
 template <typename T>
 __global__ void k_pcg_direction(T* __restrict__ p, const T* __restrict__ z, const double* beta, long long n,
