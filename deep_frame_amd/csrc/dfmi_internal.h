@@ -174,6 +174,10 @@ void pcg_update(T* x, T* r, const T* p, const T* ap, const double* alpha, long l
 template <typename T>
 void pcg_direction(T* p, const T* z, const double* beta, long long n, int k, const int* active, hipStream_t s);
 void add_scaled(double* x, const float* d, const double* scale, long long n, int k, const int* active, hipStream_t s);
+template <typename T>
+void scale_columns(T* v, long long n, int k, const double* f, const int* select, hipStream_t s);
+// dst[c] = src[c] (square_scale false) or dst[c] *= src[c]^2 (true) for selected columns
+void select_scalars(const double* src, double* dst, const int* select, int k, bool square_scale, hipStream_t s);
 void pcg_scalars_alpha(const double* rz, const double* pap, double* alpha, int k, const int* active, hipStream_t s);
 template <typename T>
 void to_work(const double* r, const double* scale, T* w, long long n, int k, const int* active, hipStream_t s);

@@ -744,10 +744,14 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
   sync(h);
 }
 
-// Iterative refinement: the FP64 true residual r = b - A x drives outer steps; each outer step
-// solves A d = r / |r| with PCG run entirely in FP32 (vectors, operator, V-cycle) to a relative
-// tolerance, then x += |r| d in FP64.  Convergence is judged on the FP64 residual only, so the
-// result meets the same criterion as the flexible FP64 PCG above.
+// Mixed-precision PCG with reliable updates.  The Krylov iteration runs entirely in FP32 (vectors,
+// operator, V-cycle; dot products accumulated in FP64) on the scaled system A d = r / s.  A column makes a
+// reliable update when its recursive FP32 residual has dropped by the factor `inner_tolerance` since
+// the last update, or looks converged: the FP32 correction is flushed into the FP64 solution
+// (x += s d, d = 0), the true residual r = b - A x is recomputed in FP64, and the iteration continues on
+// r / |r| with its search direction, previous preconditioned residual and r.z rescaled to the new scale
+// (no restart, so the Krylov information is kept).  Columns converge only on the FP64 true residual, so
+// the result meets the same criterion as the flexible FP64 PCG above.
 static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_flat, int k, dfmi_report* rep) {
   if (h->st.precision != 1) throw std::invalid_argument("refine mode needs the fp32 V-cycle precision");
   Hierarchy& H = h->slots[slot];
@@ -760,22 +764,26 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
   double* b = h->b.as<double>();
   double* x = h->x.as<double>();
   double* r = h->r.as<double>();
-  // FP32 inner vectors live in the halves of FP64 buffers that refinement does not use
+  // FP32 inner vectors live in the halves of FP64 buffers that this scheme does not use
   float* r32 = h->z.as<float>();
   float* d32 = r32 + kn;
   float* p32 = h->zn.as<float>();
   float* ap32 = p32 + kn;
   double* sc = h->scal.as<double>();
+  const size_t kc = (size_t)h->k_cap;
   double* alpha = sc;
-  double* rz = sc + (size_t)h->k_cap;
-  double* pap = sc + 2 * (size_t)h->k_cap;
-  double* rzn = sc + 3 * (size_t)h->k_cap;
-  double* rzc = sc + 4 * (size_t)h->k_cap;
-  double* beta = sc + 5 * (size_t)h->k_cap;
-  double* rr = sc + 6 * (size_t)h->k_cap;
-  double* scale = sc + 8 * (size_t)h->k_cap;
+  double* rz = sc + kc;
+  double* pap = sc + 2 * kc;
+  double* rzn = sc + 3 * kc;
+  double* rzc = sc + 4 * kc;
+  double* beta = sc + 5 * kc;
+  double* rr = sc + 6 * kc;
+  double* scale = sc + 8 * kc;
+  double* factor = sc + 9 * kc;
+  double* rzc_new = sc + 10 * kc;
   int* act = h->active.as<int>();
   Level& F = *H.levels[0];
+  float* z = nullptr;  // current preconditioned residual: the fine level's V-cycle output
   VArgs v{h, &H, k, act};
   const int nbr = reduce_blocks(n);
   double* part = h->partial.as<double>();
@@ -784,77 +792,120 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
   mask_vector(F.mask.as<uint8_t>(), b, h->pad.nodes(), k, s);
   std::vector<double> nb;
   column_norms(h, b, n, k, nullptr, nb);
-  std::vector<int> active(k), inner(k);
+  std::vector<int> active(k), upd(k);
+  std::vector<double> sh(k, 1.0), fh(k, 1.0), rrh(k), tn;
   for (int c = 0; c < k; ++c) {
     active[c] = nb[c] > 0 ? 1 : 0;
+    sh[c] = nb[c] > 0 ? nb[c] : 1.0;
     rep[c].iterations = 0;
     rep[c].relative_residual = 0.0;
     rep[c].converged = 1;
   }
-  fill_zero<double>(x, (long long)kn, s);
-  copy_columns<double>(b, r, n, k, nullptr, s);
   const double tol = h->st.tolerance;
-  int step = 0;
-  std::vector<double> rn, rrh(k), target(k);
+  const double delta = std::min(0.5, std::max(h->st.inner_tolerance, 1e-7));
   auto count = [](const std::vector<int>& f) {
     int m = 0;
     for (int q : f) m += q;
     return m;
   };
-  while (true) {
-    DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
-    column_norms(h, r, n, k, act, rn);
-    for (int c = 0; c < k; ++c) {
-      if (!active[c]) continue;
-      rep[c].iterations = step;
-      rep[c].relative_residual = rn[c] / nb[c];
-      rep[c].converged = rep[c].relative_residual < tol;
-      if (rep[c].converged) active[c] = 0;
-      // inner target: the outer tolerance with margin, never tighter than FP32 can track
-      target[c] = std::max(h->st.inner_tolerance, 0.5 * tol * nb[c] / std::max(rn[c], 1e-300));
-    }
-    if (!count(active) || step >= h->st.max_iterations) break;
-    DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
-    DFMI_HIP(hipMemcpyAsync(scale, rn.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-    to_work<float>(r, scale, r32, n, k, act, s);
-    fill_zero<float>(d32, (long long)kn, s);
+  auto upload = [&](int* dst, const std::vector<int>& src) {
+    DFMI_HIP(hipMemcpyAsync(dst, src.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+  };
+  fill_zero<double>(x, (long long)kn, s);
+  fill_zero<float>(d32, (long long)kn, s);
+  DFMI_HIP(hipMemcpyAsync(scale, sh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+  upload(act, active);
+  to_work<float>(b, scale, r32, n, k, act, s);
+  int step = 0;
+  if (count(active)) {
     vcycle<float>(v, 0, r32);
-    copy_columns<float>(F.x.as<float>(), p32, n, k, act, s);
-    column_dot<float>(r32, F.x.as<float>(), n, k, part, nbr, act, s);
+    z = F.x.as<float>();
+    copy_columns<float>(z, p32, n, k, act, s);
+    column_dot<float>(r32, z, n, k, part, nbr, act, s);
     reduce_partials(part, nbr, k, rz, s);
-    inner = active;
-    while (step < h->st.max_iterations) {
-      int nbf = 0;
-      ApplyArgs a{};
-      a.x = p32;
-      a.y = ap32;
-      a.k = k;
-      a.active = act;
-      a.partial = part;
-      fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_DOT, a, s, &nbf);
-      reduce_partials(part, nbf, k, pap, s);
-      pcg_scalars_alpha(rz, pap, alpha, k, act, s);
-      // also r_new . z_old (z_old is the last V-cycle output, still in the fine level's x)
-      pcg_update<float>(d32, r32, p32, ap32, alpha, n, k, act, part, nbr, s, F.x.as<float>());
-      reduce_partials(part, nbr, k, rr, s);
-      reduce_partials(part + (size_t)k * nbr, nbr, k, rzc, s);
-      DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-      sync(h);
-      ++step;
-      for (int c = 0; c < k; ++c)
-        if (inner[c] && std::sqrt(rrh[c]) < target[c]) inner[c] = 0;
-      if (!count(inner)) break;
-      DFMI_HIP(hipMemcpyAsync(act, inner.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
-      // flexible (Polak-Ribiere) beta with r_new . z_old from the update above
-      vcycle<float>(v, 0, r32);
-      column_dot<float>(r32, F.x.as<float>(), n, k, part, nbr, act, s);
-      reduce_partials(part, nbr, k, rzn, s);
-      pcg_beta(rzn, rzc, rz, beta, k, act, s);
-      pcg_direction<float>(p32, F.x.as<float>(), beta, n, k, act, s);
+  }
+  while (count(active) && step < h->st.max_iterations) {
+    int nbf = 0;
+    ApplyArgs a{};
+    a.x = p32;
+    a.y = ap32;
+    a.k = k;
+    a.active = act;
+    a.partial = part;
+    fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_DOT, a, s, &nbf);
+    reduce_partials(part, nbf, k, pap, s);
+    pcg_scalars_alpha(rz, pap, alpha, k, act, s);
+    // d += alpha p, r -= alpha Ap, |r|^2 and r_new . z_old in one pass
+    pcg_update<float>(d32, r32, p32, ap32, alpha, n, k, act, part, nbr, s, z);
+    reduce_partials(part, nbr, k, rr, s);
+    reduce_partials(part + (size_t)k * nbr, nbr, k, rzc, s);
+    DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    sync(h);
+    ++step;
+    bool any_upd = false;
+    for (int c = 0; c < k; ++c) {
+      upd[c] = 0;
+      if (!active[c]) continue;
+      const double est = std::sqrt(rrh[c]);  // |r| / s since the last update (r / s had norm 1 there)
+      if (est * sh[c] < tol * nb[c] || est < delta) upd[c] = 1, any_upd = true;
     }
-    DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+    if (any_upd) {
+      upload(act, upd);
+      add_scaled(x, d32, scale, n, k, act, s);
+      for (int c = 0; c < k; ++c) fh[c] = 0.0;
+      DFMI_HIP(hipMemcpyAsync(factor, fh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      scale_columns<float>(d32, n, k, factor, act, s);  // d = 0
+      fine_fp64(h, H, MODE_RESID, x, r, b, k, act, nullptr, nullptr);
+      column_norms(h, r, n, k, act, tn);
+      std::vector<int> cont(k, 0);
+      for (int c = 0; c < k; ++c) {
+        fh[c] = 1.0;
+        if (!upd[c]) continue;
+        const double t = tn[c] / nb[c];
+        rep[c].iterations = step;
+        rep[c].relative_residual = t;
+        rep[c].converged = t < tol;
+        if (t < tol || !(tn[c] > 0)) {
+          active[c] = 0;
+        } else {
+          cont[c] = 1;
+          fh[c] = sh[c] / tn[c];
+          sh[c] = tn[c];
+        }
+      }
+      if (count(cont)) {
+        upload(act, cont);
+        DFMI_HIP(hipMemcpyAsync(scale, sh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        DFMI_HIP(hipMemcpyAsync(factor, fh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        to_work<float>(r, scale, r32, n, k, act, s);  // r / |r|
+        scale_columns<float>(p32, n, k, factor, act, s);
+        scale_columns<float>(z, n, k, factor, act, s);
+        select_scalars(factor, rz, act, k, true, s);  // rz *= f^2
+        column_dot<float>(r32, z, n, k, part, nbr, act, s);
+        reduce_partials(part, nbr, k, rzc_new, s);
+        select_scalars(rzc_new, rzc, act, k, false, s);
+      }
+      upload(act, active);
+    }
+    if (!count(active)) break;
+    vcycle<float>(v, 0, r32);
+    z = F.x.as<float>();
+    column_dot<float>(r32, z, n, k, part, nbr, act, s);
+    reduce_partials(part, nbr, k, rzn, s);
+    pcg_beta(rzn, rzc, rz, beta, k, act, s);
+    pcg_direction<float>(p32, z, beta, n, k, act, s);
+  }
+  if (count(active)) {  // iteration limit: flush and report the true residual
+    upload(act, active);
     add_scaled(x, d32, scale, n, k, act, s);
     fine_fp64(h, H, MODE_RESID, x, r, b, k, act, nullptr, nullptr);
+    column_norms(h, r, n, k, act, tn);
+    for (int c = 0; c < k; ++c)
+      if (active[c]) {
+        rep[c].iterations = step;
+        rep[c].relative_residual = tn[c] / nb[c];
+        rep[c].converged = rep[c].relative_residual < tol;
+      }
   }
   grid_to_flat(h->pad, h->ux, h->uy, h->uz, x, x_flat, k, s);
   sync(h);
@@ -904,7 +955,7 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->dead_ratio = 1e-12;
   o->seed = 0;
   o->refine = 0;
-  o->inner_tolerance = 1e-5;
+  o->inner_tolerance = 1e-2;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,

@@ -205,6 +205,38 @@ void fill_zero(T* x, long long n, hipStream_t s) {
 template void fill_zero<float>(float*, long long, hipStream_t);
 template void fill_zero<double>(double*, long long, hipStream_t);
 
+// v[c] *= f[c] for selected columns (stores 0 where f[c] == 0)
+template <typename T>
+__global__ void k_scale_columns(T* __restrict__ v, long long n, const double* f, const int* select) {
+  const int c = blockIdx.y;
+  if (select && !select[c]) return;
+  const T w = (T)f[c];
+  const long long off = (long long)c * n;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    v[off + i] = w == T(0) ? T(0) : v[off + i] * w;
+}
+
+template <typename T>
+void scale_columns(T* v, long long n, int k, const double* f, const int* select, hipStream_t s) {
+  const unsigned bx = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_scale_columns<T>, dim3(bx, k), dim3(256), 0, s, v, n, f, select);
+  DFMI_HIP(hipGetLastError());
+}
+template void scale_columns<float>(float*, long long, int, const double*, const int*, hipStream_t);
+template void scale_columns<double>(double*, long long, int, const double*, const int*, hipStream_t);
+
+// per-column scalars: dst[c] = src[c] (mode 0) or dst[c] *= src[c]^2 (mode 1) for selected columns
+__global__ void k_scalars(const double* src, double* dst, const int* select, int k, int mode) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= k || (select && !select[c])) return;
+  dst[c] = mode == 0 ? src[c] : dst[c] * src[c] * src[c];
+}
+
+void select_scalars(const double* src, double* dst, const int* select, int k, bool square_scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_scalars, dim3((k + 63) / 64), dim3(64), 0, s, src, dst, select, k, square_scale ? 1 : 0);
+  DFMI_HIP(hipGetLastError());
+}
+
 // x += scale[c] * d for active columns (iterative-refinement correction, FP32 -> FP64)
 __global__ void k_add_scaled(double* __restrict__ x, const float* __restrict__ d, const double* scale, long long n,
                              const int* active) {

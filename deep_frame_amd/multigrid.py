@@ -18,7 +18,7 @@ MULTIGRID = {"coarse": "galerkin", "coarsest_dofs": 80000, "max_levels": 8, "smo
              "damping": 1.0, "chebyshev_degree": 3, "chebyshev_ratio": 20.0, "growth": 2.0, "cycle": "V",
              "power_iterations": 20, "precision": "float32", "tolerance": 1e-8, "max_iterations": 2000,
              "batch": 64, "projection": True, "dead_ratio": 1e-12, "zero_rhs": 1e-12, "levels": 0, "seed": 0,
-             "refine": False, "inner_tolerance": 1e-5, "device": "cuda"}
+             "refine": False, "inner_tolerance": 1e-2, "device": "cuda"}
 PRECISIONS = {"float64": 0, "float32": 1}
 SLOTS = 8
 

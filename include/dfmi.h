@@ -43,8 +43,8 @@ typedef struct dfmi_settings {
   double dead_ratio;         /* 1e-12 */
   int32_t seed;              /* power-iteration start vector seed */
   int32_t refine;            /* 0 = flexible PCG with FP64 outer operator (reference scheme);
-                                1 = iterative refinement: FP64 residual, inner PCG in the V-cycle precision */
-  double inner_tolerance;    /* refine = 1: inner relative tolerance per outer step */
+                                1 = FP32 PCG with reliable FP64 updates of solution and residual */
+  double inner_tolerance;    /* refine = 1: reliable-update factor (FP32 residual drop between FP64 updates) */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */
