@@ -234,90 +234,131 @@ void stencil_mask(const Geom& g, double* st, const uint8_t* mask, hipStream_t s)
 
 // ---- transfer operators ------------------------------------------------------------------
 // thread id -> (node, column): column fastest for interleaved outputs, node fastest otherwise
-__device__ inline bool node_col(long long tid, long long nodes, int k, bool col_fast, long long& n, int& r) {
-  if (tid >= nodes * k) return false;
-  if (col_fast) {
-    n = tid / k;
-    r = (int)(tid - n * k);
-  } else {
-    r = (int)(tid / nodes);
-    n = tid - (long long)r * nodes;
+// Transfers.  Column-major vectors: grid (node blocks, columns), so no division recovers the column;
+// interleaved vectors: one thread per (node, column), column fastest.  Indices are 32-bit (levels are
+// checked to hold fewer than 2^31 values) and sums are in the working precision (the weights are powers
+// of two, exact in both precisions).
+__device__ inline void node_xyz(int n, const Geom& g, int& i, int& j, int& l) {
+  const int plane = g.ny * g.nz;
+  i = n / plane;
+  const int rem = n - i * plane;
+  j = rem / g.nz;
+  l = rem - j * g.nz;
+}
+
+template <typename T>
+__device__ inline void restrict_node(const Geom& gf, const T* __restrict__ F, int fns, const Geom& gc, int n,
+                                     T (&acc)[3]) {
+  int I, J, K;
+  node_xyz(n, gc, I, J, K);
+  acc[0] = acc[1] = acc[2] = T(0);
+#pragma unroll
+  for (int e = 0; e < 27; ++e) {
+    const int ex = e / 9 - 1, ey = (e / 3) % 3 - 1, ez = e % 3 - 1;
+    const int fi = 2 * I + ex, fj = 2 * J + ey, fk = 2 * K + ez;
+    if (fi < 0 || fj < 0 || fk < 0 || fi >= gf.nx || fj >= gf.ny || fk >= gf.nz) continue;
+    const T w = T((ex == 0 ? 1.0 : 0.5) * (ey == 0 ? 1.0 : 0.5) * (ez == 0 ? 1.0 : 0.5));
+    const T* v = F + ((fi * gf.ny + fj) * gf.nz + fk) * fns;
+    acc[0] += w * v[0];
+    acc[1] += w * v[1];
+    acc[2] += w * v[2];
   }
-  return true;
+}
+
+template <typename T>
+__device__ inline void prolong_node(const Geom& gc, const T* __restrict__ C, int cns, const Geom& gf, int n,
+                                    T (&acc)[3]) {
+  int i, j, kz;
+  node_xyz(n, gf, i, j, kz);
+  acc[0] = acc[1] = acc[2] = T(0);
+  const int i0 = i >> 1, j0 = j >> 1, k0 = kz >> 1;
+  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (kz & 1) ? 2 : 1;
+  const T w = T(((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((kz & 1) ? 0.5 : 1.0));
+  for (int a = 0; a < ni; ++a)
+    for (int b = 0; b < nj; ++b)
+      for (int d = 0; d < nk; ++d) {
+        const T* v = C + (((i0 + a) * gc.ny + j0 + b) * gc.nz + k0 + d) * cns;
+        acc[0] += v[0];
+        acc[1] += v[1];
+        acc[2] += v[2];
+      }
+  acc[0] *= w;
+  acc[1] *= w;
+  acc[2] *= w;
 }
 
 template <typename T>
 __global__ void k_restrict(Geom gf, const T* __restrict__ f, VLay fl, Geom gc, const uint8_t* __restrict__ cmask,
                            T* __restrict__ c, VLay cl, int k, const int* active) {
-  const long long Nc = gc.nodes();
-  long long n;
-  int r;
-  if (!node_col((long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, Nc, k, cl.rs == 3 && k > 1,
-                n, r))
-    return;
-  if (active && !active[r]) return;
-  const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
-  const T* F = f + (long long)r * fl.rs;
-  double acc[3] = {0, 0, 0};
-  for (int e = 0; e < 27; ++e) {
-    const int ex = e / 9 - 1, ey = (e / 3) % 3 - 1, ez = e % 3 - 1;
-    const int fi = 2 * I + ex, fj = 2 * J + ey, fk = 2 * K + ez;
-    if (fi < 0 || fj < 0 || fk < 0 || fi >= gf.nx || fj >= gf.ny || fk >= gf.nz) continue;
-    const double w = weight1(ex) * weight1(ey) * weight1(ez);
-    const T* v = F + (((long long)fi * gf.ny + fj) * gf.nz + fk) * fl.ns;
-    acc[0] += w * (double)v[0];
-    acc[1] += w * (double)v[1];
-    acc[2] += w * (double)v[2];
+  const int Nc = (int)gc.nodes();
+  int n, r;
+  if (cl.rs == 3 && k > 1) {  // interleaved coarse: column fastest
+    const int tid = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (tid >= Nc * k) return;
+    n = tid / k;
+    r = tid - n * k;
+  } else {
+    r = blockIdx.y;
+    n = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (n >= Nc) return;
   }
+  if (active && !active[r]) return;
+  T acc[3];
+  restrict_node<T>(gf, f + (long long)r * fl.rs, (int)fl.ns, gc, n, acc);
   const uint8_t m = cmask[n];
-  T* C = c + n * cl.ns + (long long)r * cl.rs;
-  for (int q = 0; q < 3; ++q) C[q] = live_bit(m, q) ? (T)acc[q] : T(0);
+  T* C = c + (long long)n * cl.ns + (long long)r * cl.rs;
+  for (int q = 0; q < 3; ++q) C[q] = live_bit(m, q) ? acc[q] : T(0);
 }
 
 template <typename T>
 void restrict_full(const Geom& fine, const T* f, VLay flay, const Geom& coarse, const uint8_t* cmask, T* c,
                    VLay clay, int k, const int* active, hipStream_t s) {
-  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((coarse.nodes() * k + 255) / 256)), dim3(256), 0, s, fine, f,
-                     flay, coarse, cmask, c, clay, k, active);
+  if (fine.nodes() * 3 * (long long)k > (long long)INT32_MAX) throw std::runtime_error("restrict: level too large");
+  const long long nc = coarse.nodes();
+  if (clay.rs == 3 && k > 1)
+    hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((nc * k + 255) / 256)), dim3(256), 0, s, fine, f, flay, coarse,
+                       cmask, c, clay, k, active);
+  else
+    hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((nc + 255) / 256), k), dim3(256), 0, s, fine, f, flay, coarse,
+                       cmask, c, clay, k, active);
   DFMI_HIP(hipGetLastError());
 }
 
 template <typename T>
 __global__ void k_prolong_add(Geom gc, const T* __restrict__ c, VLay cl, Geom gf, const uint8_t* __restrict__ fmask,
                               T* __restrict__ f, VLay fl, int k, const int* active) {
-  const long long Nf = gf.nodes();
-  long long n;
-  int r;
-  if (!node_col((long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, Nf, k, fl.rs == 3 && k > 1,
-                n, r))
-    return;
+  const int Nf = (int)gf.nodes();
+  int n, r;
+  if (fl.rs == 3 && k > 1) {  // interleaved fine: column fastest
+    const int tid = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (tid >= Nf * k) return;
+    n = tid / k;
+    r = tid - n * k;
+  } else {
+    r = blockIdx.y;
+    n = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (n >= Nf) return;
+  }
   if (active && !active[r]) return;
-  const int kz = (int)(n % gf.nz), j = (int)((n / gf.nz) % gf.ny), i = (int)(n / ((long long)gf.nz * gf.ny));
-  const T* C = c + (long long)r * cl.rs;
-  double acc[3] = {0, 0, 0};
-  const int i0 = i >> 1, j0 = j >> 1, k0 = kz >> 1;
-  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (kz & 1) ? 2 : 1;
-  const double wi = (i & 1) ? 0.5 : 1.0, wj = (j & 1) ? 0.5 : 1.0, wk = (kz & 1) ? 0.5 : 1.0;
-  for (int a = 0; a < ni; ++a)
-    for (int b = 0; b < nj; ++b)
-      for (int d = 0; d < nk; ++d) {
-        const T* v = C + (((long long)(i0 + a) * gc.ny + j0 + b) * gc.nz + k0 + d) * cl.ns;
-        const double w = wi * wj * wk;
-        acc[0] += w * (double)v[0];
-        acc[1] += w * (double)v[1];
-        acc[2] += w * (double)v[2];
-      }
+  T acc[3];
+  prolong_node<T>(gc, c + (long long)r * cl.rs, (int)cl.ns, gf, n, acc);
   const uint8_t m = fmask[n];
-  T* F = f + n * fl.ns + (long long)r * fl.rs;
+  T* F = f + (long long)n * fl.ns + (long long)r * fl.rs;
   for (int q = 0; q < 3; ++q)
-    if (live_bit(m, q)) F[q] = F[q] + (T)acc[q];
+    if (live_bit(m, q)) F[q] = F[q] + acc[q];
 }
 
 template <typename T>
 void prolong_add(const Geom& coarse, const T* c, VLay clay, const Geom& fine, const uint8_t* fmask, T* f,
                  VLay flay, int k, const int* active, hipStream_t s) {
-  hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((fine.nodes() * k + 255) / 256)), dim3(256), 0, s, coarse, c,
-                     clay, fine, fmask, f, flay, k, active);
+  if (fine.nodes() * 3 * (long long)k > (long long)INT32_MAX) throw std::runtime_error("prolong: level too large");
+  const long long nf = fine.nodes();
+  if (flay.rs == 3 && k > 1)
+    hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((nf * k + 255) / 256)), dim3(256), 0, s, coarse, c, clay,
+                       fine, fmask, f, flay, k, active);
+  else
+    hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((nf + 255) / 256), k), dim3(256), 0, s, coarse, c, clay,
+                       fine, fmask, f, flay, k, active);
   DFMI_HIP(hipGetLastError());
 }
 
