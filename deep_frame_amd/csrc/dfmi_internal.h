@@ -159,6 +159,7 @@ template <typename T>
 void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int k, const int* active, VLay lay,
                 hipStream_t s);
 void power_init(const uint8_t* mask, double* v, long long nodes, unsigned seed, hipStream_t s);
+void inv_sqrt(const double* in, double* out, hipStream_t s);  // out[0] = 1 / sqrt(in[0])
 void dinv_scale(const double* y, const double* dinv, const double* inv_norm, double* v, long long nodes,
                 hipStream_t s);
 void pcg_beta(const double* rzn, const double* rzc, double* rz, double* beta, int k, const int* active,

@@ -456,16 +456,23 @@ static double device_max(Handle* h, const double* v, long long n) {
   return out;
 }
 
-// lambda_max(Dinv A) by power iteration (topology_multigrid.py:214-223), in FP64
+// lambda_max(Dinv A) by power iteration (topology_multigrid.py:214-223), in FP64.  The iterate is
+// normalised on the device (w = Dinv A v, v = w / |w|), so the host waits once per level.
 static double spectrum(Handle* h, Hierarchy& H, int idx) {
   Level& L = *H.levels[idx];
   const long long n = col(L.g);
-  DevBuf v, y, inv;
+  DevBuf v, y, nrm;
   v.ensure(sizeof(double) * n);
   y.ensure(sizeof(double) * n);
-  inv.ensure(sizeof(double));
+  nrm.ensure(sizeof(double) * 4);
+  double* sq = nrm.as<double>();       // |w|^2
+  double* inv = nrm.as<double>() + 1;  // 1 / |w|
+  double* one = nrm.as<double>() + 2;
+  const double host_one = 1.0;
+  DFMI_HIP(hipMemcpyAsync(one, &host_one, sizeof(double), hipMemcpyHostToDevice, h->stream));
   power_init(L.mask.as<uint8_t>(), v.as<double>(), L.g.nodes(), (unsigned)h->st.seed + 7919u * idx, h->stream);
-  double est = 1.0;
+  const int nb = reduce_blocks(n);
+  require_workspace(h);
   for (int it = 0; it < h->st.power_iterations; ++it) {
     ApplyArgs a{};
     a.x = v.as<double>();
@@ -476,15 +483,16 @@ static double spectrum(Handle* h, Hierarchy& H, int idx) {
     else
       stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, column_major(L.g),
                             h->stream, nullptr);
-    const double one = 1.0;
-    DFMI_HIP(hipMemcpyAsync(inv.as<double>(), &one, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    dinv_scale(y.as<double>(), L.dinv64.as<double>(), inv.as<double>(), v.as<double>(), L.g.nodes(), h->stream);
-    est = column_norm(h, v.as<double>(), n);
-    if (!(est > 0)) break;
-    scale_vector(v.as<double>(), 1.0 / est, n, h->stream);
+    dinv_scale(y.as<double>(), L.dinv64.as<double>(), one, y.as<double>(), L.g.nodes(), h->stream);
+    column_dot<double>(y.as<double>(), y.as<double>(), n, 1, h->partial.as<double>(), nb, nullptr, h->stream);
+    reduce_partials(h->partial.as<double>(), nb, 1, sq, h->stream);
+    inv_sqrt(sq, inv, h->stream);
+    dinv_scale(y.as<double>(), nullptr, inv, v.as<double>(), L.g.nodes(), h->stream);
   }
+  double out = 0.0;
+  DFMI_HIP(hipMemcpyAsync(&out, sq, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   sync(h);
-  return est;
+  return std::sqrt(out);
 }
 
 static void build(Handle* h, int slot, const uint8_t* fixed_dev) {

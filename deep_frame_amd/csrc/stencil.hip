@@ -148,10 +148,13 @@ template void stencil_apply<double>(const Geom&, const double*, const uint8_t*, 
 // ---- Galerkin product P^T A P in stencil form -------------------------------------------
 // Coarse node N, offset D: sum over fine f = 2N + e (weight w(e)) and fine neighbours
 // g = f + df inside the support of coarse node N + D (weight w(g - 2(N + D))).
+// One thread per (coarse node, coarse offset D).  The 27 offsets of one range of coarse nodes are
+// consecutive logical blocks on one XCD, so they share the fine-stencil reads in that XCD's L2.
 __global__ void k_rap(Geom gf, const double* __restrict__ fst, Geom gc, double* __restrict__ cst) {
   const long long Nc = gc.nodes(), Nf = gf.nodes();
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int D = blockIdx.y;
+  const unsigned logical = xcd_block(blockIdx.x, gridDim.x);
+  const int D = (int)(logical % 27);
+  const long long n = (long long)(logical / 27) * blockDim.x + threadIdx.x;
   if (n >= Nc) return;
   const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
   const int MI = I + D / 9 - 1, MJ = J + (D / 3) % 3 - 1, MK = K + D % 3 - 1;
@@ -179,8 +182,8 @@ __global__ void k_rap(Geom gf, const double* __restrict__ fst, Geom gc, double* 
 }
 
 void stencil_rap(const Geom& fine, const double* fst, const Geom& coarse, double* cst, hipStream_t s) {
-  hipLaunchKernelGGL(k_rap, dim3((unsigned)((coarse.nodes() + 255) / 256), 27), dim3(256), 0, s, fine, fst, coarse,
-                     cst);
+  hipLaunchKernelGGL(k_rap, dim3((unsigned)((coarse.nodes() + 255) / 256) * 27), dim3(256), 0, s, fine, fst,
+                     coarse, cst);
   DFMI_HIP(hipGetLastError());
 }
 

@@ -141,17 +141,27 @@ void power_init(const uint8_t* mask, double* v, long long nodes, unsigned seed, 
   DFMI_HIP(hipGetLastError());
 }
 
-// v = Dinv y * inv_norm[0]
-__global__ void k_dinv_scale(const double* __restrict__ y, const double* __restrict__ dinv, const double* inv_norm,
-                             double* __restrict__ v, long long n) {
+// v = Dinv y * inv_norm[0] (dinv == nullptr: v = y * inv_norm[0]); v may alias y
+__global__ void k_dinv_scale(const double* y, const double* __restrict__ dinv, const double* inv_norm, double* v,
+                             long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = dinv[i] * y[i] * inv_norm[0];
+  if (i < n) v[i] = (dinv ? dinv[i] : 1.0) * y[i] * inv_norm[0];
 }
 
 void dinv_scale(const double* y, const double* dinv, const double* inv_norm, double* v, long long nodes,
                 hipStream_t s) {
   hipLaunchKernelGGL(k_dinv_scale, dim3((unsigned)((nodes * 3 + 255) / 256)), dim3(256), 0, s, y, dinv, inv_norm, v,
                      nodes * 3);
+  DFMI_HIP(hipGetLastError());
+}
+
+// out[0] = 1 / sqrt(in[0]) (0 when in[0] is not positive)
+__global__ void k_inv_sqrt(const double* in, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = in[0] > 0 ? 1.0 / sqrt(in[0]) : 0.0;
+}
+
+void inv_sqrt(const double* in, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_inv_sqrt, dim3(1), dim3(64), 0, s, in, out);
   DFMI_HIP(hipGetLastError());
 }
 
