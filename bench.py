@@ -26,15 +26,19 @@ CONFIGS = {
     "variants": {"count": 64},
 }
 HBM_PEAK_GBS = 8000.0
-TRAFFIC = ROOT / "profiles" / "traffic_k_fine_apply_fp32_frame_256x256x48_42rhs.txt"
+TRAFFIC = {1: ROOT / "profiles" / "traffic_k_fine_apply_fp32_frame_256x256x48_42rhs.txt",
+           2: ROOT / "profiles" / "traffic_k_fine_mfma_frame_256x256x48_42rhs.txt"}
+KERNELS = {1: "k_fine_apply<float, MODE_APPLY> (FP32 vector units)",
+           2: "k_fine_mfma<MODE_APPLY> (split-bf16 matrix cores, FP32 vectors)"}
 
 
-def measured_traffic(config):
-    """HBM bytes per launch of the fine FP32 operator from the committed rocprofv3 --pmc summary (c3 only)."""
-    if config != "frame" or not TRAFFIC.exists():
+def measured_traffic(config, precision):
+    """HBM bytes per launch of a fine operator kernel from its committed rocprofv3 --pmc summary (frame only)."""
+    path = TRAFFIC[precision]
+    if config != "frame" or not path.exists():
         return None
     vals = {}
-    for line in TRAFFIC.read_text().splitlines():
+    for line in path.read_text().splitlines():
         if line and not line.startswith("#"):
             key, value = line.split()
             vals[key] = float(value)
@@ -54,6 +58,8 @@ def parse():
                     help="mixed solver: FP32 PCG inside FP64 iterative refinement, or FP64 PCG with an FP32 V-cycle")
     ap.add_argument("--inner-tolerance", type=float, default=None, help="refine scheme: inner relative tolerance")
     ap.add_argument("--apply-reps", type=int, default=20)
+    ap.add_argument("--fine-smoother", default="vector", choices=["matrix", "vector"],
+                    help="fine level of the fp32 V-cycle on the matrix cores (split bf16) or the FP32 vector units")
     ap.add_argument("--apply-only", action="store_true",
                     help="build the hierarchy once and only time the fine operator (for counter profiles)")
     return ap.parse_args()
@@ -126,20 +132,23 @@ def main():
         mg = GeometricMultigrid(cfg["shape"], (1, 1, 1), ke, {"precision": "float32", "levels": cfg["levels"]})
         mg.update(moduli)
         slot = mg.hierarchy("apply", fixed)["slot"]
+        mg.tune_fine("apply", rhs.shape[0])
         for level in (0, 1):
-            for precision in (1, 0):
+            for precision in ((2, 1, 0) if level == 0 else (1, 0)):
                 ms = c_double()
                 mg._stream()
                 _native.call("dfmi_time_apply", mg.handle, slot, level, precision, rhs.shape[0], args.apply_reps,
                              byref(ms))
-                size = 4 if precision else 8
+                size = 8 if precision == 0 else 4
                 nodes = int(np.prod([n // 2 ** level + 1 for n in mg.padded]))
                 if level == 0:
                     alg = 2.0 * ndof * rhs.shape[0] * size + nel * size
                 else:  # vectors in and out plus the 243-entry stencil per node
                     alg = 2.0 * 3 * nodes * rhs.shape[0] * size + 243.0 * nodes * size
-                print(json.dumps({"kernel": "k_fine_apply" if level == 0 else "k_stencil_apply", "level": level,
-                                  "precision": "fp32" if precision else "fp64", "rhs": rhs.shape[0],
+                name = {2: "k_fine_mfma", 1: "k_fine_apply", 0: "k_fine_apply"}[precision] if level == 0 \
+                    else "k_stencil_apply"
+                print(json.dumps({"kernel": name, "level": level,
+                                  "precision": {0: "fp64", 1: "fp32", 2: "bf16x3"}[precision], "rhs": rhs.shape[0],
                                   "ms_per_launch": ms.value, "algorithmic_GBs": alg / (ms.value * 1e-3) / 1e9}))
         mg.close()
         # plain copy of a known size: calibrates the FETCH_SIZE / WRITE_SIZE counters
@@ -149,19 +158,27 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration_copy_bytes": src.numel() * 4}))
         return
-    mixed = {"precision": "float32", "refine": args.scheme == "refine"}
+    mixed = {"precision": "float32", "refine": args.scheme == "refine", "fine_smoother": args.fine_smoother}
     if args.inner_tolerance:
         mixed["inner_tolerance"] = args.inner_tolerance
     mg, times, reports, x = run(mixed, args.steps, args.warmup, f"mixed-{args.scheme}")
     t_mixed = float(np.mean(times))
-    # dominant kernel: fine-level FP32 operator on all RHS columns, HIP events on the library stream
-    slot = mg.hierarchies[("bench", args.warmup + args.steps - 1)]["slot"]
-    ms = c_double()
-    mg._stream()
-    _native.call("dfmi_time_apply", mg.handle, slot, 0, 1, rhs.shape[0], args.apply_reps, byref(ms))
+    # dominant K.u kernel of the mixed solve: the fine-level smoother / residual operator of the V-cycle
+    # (matrix cores unless --fine-smoother vector), on all RHS columns, HIP events on the library stream;
+    # the FP32 vector kernel (the Krylov operator) is timed beside it
+    key = ("bench", args.warmup + args.steps - 1)
+    slot = mg.hierarchies[key]["slot"]
     k = rhs.shape[0]
+    tile = mg.tune_fine(key, k) if args.fine_smoother == "matrix" and cfg["shape"][2] <= 64 else None
     alg_bytes = 2.0 * ndof * k * 4 + nel * 4
-    achieved = alg_bytes / (ms.value * 1e-3) / 1e9
+    timings = {}
+    for precision in ((2, 1) if tile else (1,)):
+        ms = c_double()
+        mg._stream()
+        _native.call("dfmi_time_apply", mg.handle, slot, 0, precision, k, args.apply_reps, byref(ms))
+        timings[precision] = ms.value
+    main_precision = 2 if tile else 1
+    achieved = alg_bytes / (timings[main_precision] * 1e-3) / 1e9
     mg.close()
     t_fp64 = None
     if not args.no_fp64:
@@ -182,11 +199,15 @@ def main():
         "config": {"workload": f"{args.config} {cfg['shape'][0]}x{cfg['shape'][1]}x{cfg['shape'][2]} hex8, "
                                f"{ndof} DOF, {k} RHS, {cfg['levels']}-level GMG", "scheme": args.scheme,
                    "mixed_s": t_mixed, "fp64_s": t_fp64, "iterations": max(r[0] for r in reports),
-                   "max_rel_res": max(r[1] for r in reports)},
+                   "max_rel_res": max(r[1] for r in reports), "fine_smoother": args.fine_smoother,
+                   "fine_tile": list(tile[:2]) if tile else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config),
-                     "traffic_source": str(TRAFFIC.relative_to(ROOT)), "kernel": "k_fine_apply<float, MODE_APPLY>",
-                     "ms_per_launch": ms.value, "algorithmic_bytes": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, main_precision),
+                     "traffic_source": str(TRAFFIC[main_precision].relative_to(ROOT)),
+                     "kernel": KERNELS[main_precision], "ms_per_launch": timings[main_precision],
+                     "algorithmic_bytes": alg_bytes,
+                     "fp32_vector_kernel": {"ms_per_launch": timings[1],
+                                            "achieved_GBs": alg_bytes / (timings[1] * 1e-3) / 1e9}},
         "cpu_baseline": baseline,
     }
     print(json.dumps(out))

@@ -18,11 +18,12 @@ class Settings(Structure):
                 ("chebyshev_degree", c_int32), ("chebyshev_ratio", c_double), ("growth", c_double),
                 ("power_iterations", c_int32), ("precision", c_int32), ("tolerance", c_double),
                 ("max_iterations", c_int32), ("dead_ratio", c_double), ("seed", c_int32), ("refine", c_int32),
-                ("inner_tolerance", c_double)]
+                ("inner_tolerance", c_double), ("fine_smoother", c_int32), ("retries", c_int32)]
 
 
 class Report(Structure):
-    _fields_ = [("iterations", c_int32), ("relative_residual", c_double), ("converged", c_int32)]
+    _fields_ = [("iterations", c_int32), ("relative_residual", c_double), ("converged", c_int32),
+                ("retried", c_int32)]
 
 
 SIGNATURES = {
@@ -44,6 +45,7 @@ SIGNATURES = {
     "dfmi_precondition": ([c_void_p, c_int32, c_void_p, c_void_p, c_int32], c_int32),
     "dfmi_transfer": ([c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int32], c_int32),
     "dfmi_time_apply": ([c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(c_double)], c_int32),
+    "dfmi_tune_fine": ([c_void_p, c_int32, c_int32, POINTER(c_int32), POINTER(c_double)], c_int32),
     "dfmi_diagnostics": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
     "dfmi_simp": ([c_void_p, c_void_p, c_int64, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p], c_int32),
     "dfmi_filter_create": ([POINTER(c_int32), POINTER(c_double), c_double, POINTER(c_uint8), POINTER(c_uint8),

@@ -110,6 +110,19 @@ template <typename T>
 void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlocks& kb, int mode,
                 const ApplyArgs& a, hipStream_t s, int* nblocks_out);
 int fine_blocks(const Geom& g);
+// Fine level on the matrix cores (FP32 vectors, split-bf16 MFMA products): modes APPLY, RESID, CHEB,
+// CHEB0, CHEB1 on grids with at most 16 * FINE_MFMA_ZMAX elements along z.  atab: per-lane A operands
+// from fine_mfma_table (64 * 12 words, device).
+constexpr int FINE_MFMA_ZMAX = 4;
+struct FineMfmaTile {
+  int tye;     // element rows per workgroup tile (8 or 16); TYE - 1 node rows are owned
+  int xchunk;  // node planes per workgroup along x
+};
+void fine_mfma_table(const double* ke_bits, std::vector<unsigned>& out);
+bool fine_mfma_supported(const Geom& g, int mode);
+void fine_apply_mfma(const Geom& g, const float* moduli, const uint8_t* mask, const unsigned* atab, int mode,
+                     const ApplyArgs& a, const FineMfmaTile& tile, hipStream_t s);
+std::vector<FineMfmaTile> fine_mfma_candidates(const Geom& g, int k);
 void fine_element_energy(const Geom& pad, int ux, int uy, int uz, const double* u, const KeBlocks& kb,
                          double* energy, int k, hipStream_t s);
 void fine_diagonal(const Geom& g, const double* moduli, const uint8_t* mask, const double* ke_diag, double* diag,

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 
@@ -89,6 +90,9 @@ struct Handle {
   dfmi_settings st{};
   hipStream_t stream = nullptr;
   DevBuf m64, m32, ke_dev, kediag_dev;
+  // fine level on the matrix cores: per-lane split-bf16 Ke operands and the tile chosen per column count
+  DevBuf atab;
+  std::map<int, FineMfmaTile> mfma_tiles;
   bool have_moduli = false;
   Hierarchy slots[8];
   // PCG workspace (padded grid, FP64)
@@ -207,12 +211,33 @@ const double* level_stencil<double>(Level& L) {
   return L.st64.as<double>();
 }
 
+// Workgroup tile of the matrix-core fine kernel for k columns: the autotuned one when tune_fine has run
+// for k, else a default with about two thousand workgroups.
+static FineMfmaTile fine_tile(Handle* h, int k) {
+  auto it = h->mfma_tiles.find(k);
+  if (it != h->mfma_tiles.end()) return it->second;
+  const auto cands = fine_mfma_candidates(h->pad, k);
+  for (const auto& c : cands)
+    if (c.tye == 16) {
+      const long long wg = (long long)((h->pad.ny + 14) / 15) * k * ((h->pad.nx + c.xchunk - 1) / c.xchunk);
+      if (wg >= 2048) return c;
+    }
+  return cands.front();
+}
+
+static bool use_mfma(Handle* h, const Geom& g, int mode) {
+  return h->st.precision == 1 && h->st.fine_smoother == 1 && fine_mfma_supported(g, mode);
+}
+
 template <typename T>
 static void level_apply(VArgs& v, int idx, int mode, ApplyArgs& a) {
   Level& L = *v.H->levels[idx];
   a.k = v.k;
   a.active = v.active;
-  if (idx == 0)
+  if (idx == 0 && sizeof(T) == sizeof(float) && use_mfma(v.h, L.g, mode))
+    fine_apply_mfma(L.g, v.h->m32.as<float>(), L.mask.as<uint8_t>(), v.h->atab.as<unsigned>(), mode, a,
+                    fine_tile(v.h, v.k), v.h->stream);
+  else if (idx == 0)
     fine_apply<T>(L.g, level_moduli<T>(v.h), L.mask.as<uint8_t>(), v.h->kb, mode, a, v.h->stream, nullptr);
   else
     stencil_apply<T>(L.g, level_stencil<T>(L), L.mask.as<uint8_t>(), mode, a, interleaved(v.k), v.h->stream, nullptr);
@@ -406,6 +431,54 @@ static void ensure_level_vectors(Handle* h, Hierarchy& H, int k) {
   }
 }
 
+// Autotuner of the matrix-core fine kernel: times every candidate tile (MODE_APPLY on the fine level's
+// x and res vectors, which hold nothing between V-cycles) with HIP events on the handle's stream and
+// keeps the fastest for this column count.  Runs once per k.
+static FineMfmaTile tune_fine(Handle* h, Hierarchy& H, int k, double* best_ms = nullptr) {
+  Level& F = *H.levels[0];
+  auto found = h->mfma_tiles.find(k);
+  if (found != h->mfma_tiles.end() && !best_ms) return found->second;
+  const long long n = (long long)k * col(F.g);
+  fill_zero<float>(F.x.as<float>(), n, h->stream);
+  ApplyArgs a{};
+  a.x = F.x.as<float>();
+  a.y = F.res.as<float>();
+  a.k = k;
+  hipEvent_t e0, e1;
+  DFMI_HIP(hipEventCreate(&e0));
+  DFMI_HIP(hipEventCreate(&e1));
+  FineMfmaTile best{};
+  float best_t = -1.f;
+  try {
+    for (const auto& c : fine_mfma_candidates(F.g, k)) {
+      fine_apply_mfma(F.g, h->m32.as<float>(), F.mask.as<uint8_t>(), h->atab.as<unsigned>(), MODE_APPLY, a, c,
+                      h->stream);
+      DFMI_HIP(hipEventRecord(e0, h->stream));
+      for (int rep = 0; rep < 3; ++rep)
+        fine_apply_mfma(F.g, h->m32.as<float>(), F.mask.as<uint8_t>(), h->atab.as<unsigned>(), MODE_APPLY, a, c,
+                        h->stream);
+      DFMI_HIP(hipEventRecord(e1, h->stream));
+      DFMI_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      DFMI_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (best_t < 0 || ms < best_t) best_t = ms, best = c;
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->mfma_tiles[k] = best;
+  if (best_ms) *best_ms = best_t / 3.0;
+  return best;
+}
+
+static void maybe_tune_fine(Handle* h, Hierarchy& H, int k) {
+  if (use_mfma(h, H.levels[0]->g, MODE_CHEB)) tune_fine(h, H, k);
+}
+
 static void ensure_workspace(Handle* h, int k) {
   const size_t n = sizeof(double) * (size_t)k * col(h->pad);
   for (DevBuf* b : {&h->b, &h->x, &h->r, &h->z, &h->zn, &h->p, &h->ap, &h->tmp}) b->ensure(n);
@@ -587,6 +660,7 @@ static void precondition(Handle* h, Hierarchy& H, const double* r, double* z, in
   Level& F = *H.levels[0];
   const long long n = col(F.g);
   ensure_level_vectors(h, H, k);
+  maybe_tune_fine(h, H, k);
   VArgs v{h, &H, k, active};
   double* scale = h->scal.as<double>() + 8 * (size_t)h->k_cap;
   if (sizeof(T) == sizeof(double)) {
@@ -632,11 +706,14 @@ static void column_norms(Handle* h, const double* v, long long n, int k, const i
   for (auto& x : out) x = std::sqrt(x);
 }
 
-static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, int k, dfmi_report* rep) {
+// warm: start from the flat iterate in x_flat (masked) instead of zero
+static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, int k, dfmi_report* rep,
+                  bool warm = false) {
   Hierarchy& H = h->slots[slot];
   if (!H.built) throw std::runtime_error("hierarchy slot not built");
   ensure_workspace(h, k);
   ensure_level_vectors(h, H, k);
+  maybe_tune_fine(h, H, k);
   hipStream_t s = h->stream;
   const long long n = col(h->pad);
   double* b = h->b.as<double>();
@@ -670,8 +747,14 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
     rep[c].converged = 1;
   }
   DFMI_HIP(hipMemcpyAsync(act, active.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
-  fill_zero<double>(x, (long long)k * n, s);
-  copy_columns<double>(b, r, n, k, nullptr, s);
+  if (warm) {
+    flat_to_grid(h->pad, h->ux, h->uy, h->uz, x_flat, x, k, s);
+    mask_vector(F.mask.as<uint8_t>(), x, h->pad.nodes(), k, s);
+    fine_fp64(h, H, MODE_RESID, x, r, b, k, nullptr, nullptr, nullptr);
+  } else {
+    fill_zero<double>(x, (long long)k * n, s);
+    copy_columns<double>(b, r, n, k, nullptr, s);
+  }
   precond_any(h, H, r, z, k, act);
   copy_columns<double>(z, p, n, k, nullptr, s);
   const int nbr = reduce_blocks(n);
@@ -766,6 +849,7 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
   if (!H.built) throw std::runtime_error("hierarchy slot not built");
   ensure_workspace(h, k);
   ensure_level_vectors(h, H, k);
+  maybe_tune_fine(h, H, k);
   hipStream_t s = h->stream;
   const long long n = col(h->pad);
   const size_t kn = (size_t)k * n;
@@ -964,6 +1048,8 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->seed = 0;
   o->refine = 0;
   o->inner_tolerance = 1e-2;
+  o->fine_smoother = 0;
+  o->retries = 2;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
@@ -980,6 +1066,8 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
     dfmi_default_settings(&st);
     if (settings) st = *settings;
     if (st.precision != 0 && st.precision != 1) throw std::invalid_argument("precision must be 0 or 1");
+    if (st.fine_smoother != 0 && st.fine_smoother != 1) throw std::invalid_argument("fine_smoother must be 0 or 1");
+    if (st.retries < 0) throw std::invalid_argument("retries must be >= 0");
     if (st.chebyshev_degree < 1 || st.max_iterations < 1 || !(st.tolerance > 0))
       throw std::invalid_argument("invalid multigrid settings");
     h.st = st;
@@ -1005,6 +1093,10 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
     h.kediag_dev.ensure(sizeof(double) * 24);
     DFMI_HIP(hipMemcpy(h.ke_dev.p, h.ke_bits, sizeof(double) * 576, hipMemcpyHostToDevice));
     DFMI_HIP(hipMemcpy(h.kediag_dev.p, kd, sizeof(double) * 24, hipMemcpyHostToDevice));
+    std::vector<unsigned> tab;
+    fine_mfma_table(h.ke_bits, tab);
+    h.atab.ensure(sizeof(unsigned) * tab.size());
+    DFMI_HIP(hipMemcpy(h.atab.p, tab.data(), sizeof(unsigned) * tab.size(), hipMemcpyHostToDevice));
     h.m64.ensure(sizeof(double) * h.pad.elems());
     h.m32.ensure(sizeof(float) * h.pad.elems());
     *out = mg.release();
@@ -1049,6 +1141,57 @@ int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed, double
   })
 }
 
+// Retry of failed columns in FP64 (topology_multigrid.py:376-406, _retried / _precise): up to `retries`
+// times, the columns whose relative residual is not below the tolerance are solved again by flexible
+// PCG with the FP64 V-cycle, from their previous iterate when its relative residual is below 1 and from
+// zero otherwise; iterations add up and the relative residual is replaced.
+static void retry_failed(Handle* h, int slot, const double* rhs, double* x, int k, dfmi_report* rep) {
+  const long long nflat = 3LL * (h->ux + 1) * (h->uy + 1) * (h->uz + 1);
+  for (int c = 0; c < k; ++c) rep[c].retried = 0;
+  for (int round = 0; round < h->st.retries; ++round) {
+    std::vector<int> failed;
+    for (int c = 0; c < k; ++c)
+      if (!(rep[c].relative_residual < h->st.tolerance)) failed.push_back(c);
+    if (failed.empty()) return;
+    const int nf = (int)failed.size();
+    DevBuf rb, xb;
+    rb.ensure(sizeof(double) * nf * nflat);
+    xb.ensure(sizeof(double) * nf * nflat);
+    const size_t colb = sizeof(double) * (size_t)nflat;
+    for (int j = 0; j < nf; ++j) {
+      const int c = failed[j];
+      DFMI_HIP(hipMemcpyAsync(rb.as<double>() + (size_t)j * nflat, rhs + (size_t)c * nflat, colb,
+                              hipMemcpyDeviceToDevice, h->stream));
+      if (rep[c].relative_residual < 1.0)
+        DFMI_HIP(hipMemcpyAsync(xb.as<double>() + (size_t)j * nflat, x + (size_t)c * nflat, colb,
+                                hipMemcpyDeviceToDevice, h->stream));
+      else
+        fill_zero<double>(xb.as<double>() + (size_t)j * nflat, nflat, h->stream);
+    }
+    std::vector<dfmi_report> again(nf);
+    const dfmi_settings saved = h->st;
+    h->st.precision = 0;
+    h->st.refine = 0;
+    try {
+      solve(h, slot, rb.as<double>(), xb.as<double>(), nf, again.data(), true);
+    } catch (...) {
+      h->st = saved;
+      throw;
+    }
+    h->st = saved;
+    for (int j = 0; j < nf; ++j) {
+      const int c = failed[j];
+      DFMI_HIP(hipMemcpyAsync(x + (size_t)c * nflat, xb.as<double>() + (size_t)j * nflat, colb,
+                              hipMemcpyDeviceToDevice, h->stream));
+      rep[c].iterations += again[j].iterations;
+      rep[c].relative_residual = again[j].relative_residual;
+      rep[c].converged = again[j].relative_residual < h->st.tolerance;
+      rep[c].retried += 1;
+    }
+    sync(h);
+  }
+}
+
 int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t k, dfmi_report* report) {
   DFMI_GUARD({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
@@ -1057,6 +1200,7 @@ int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t 
       solve_refine(&mg->h, slot, rhs, x, k, report);
     else
       solve(&mg->h, slot, rhs, x, k, report);
+    retry_failed(&mg->h, slot, rhs, x, k, report);
   })
 }
 
@@ -1093,7 +1237,12 @@ int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x, do
                          xf.as<float>(), k * n);
       a.x = xf.as<float>();
       a.y = yf.as<float>();
-      fine_apply<float>(h.pad, h.m32.as<float>(), mask, h.kb, MODE_APPLY, a, s, nullptr);
+      if (precision == 2) {
+        if (!fine_mfma_supported(h.pad, MODE_APPLY)) throw std::invalid_argument("grid not supported on the matrix cores");
+        fine_apply_mfma(h.pad, h.m32.as<float>(), mask, h.atab.as<unsigned>(), MODE_APPLY, a, fine_tile(&h, k), s);
+      } else {
+        fine_apply<float>(h.pad, h.m32.as<float>(), mask, h.kb, MODE_APPLY, a, s, nullptr);
+      }
       std::vector<double> ones(k, 1.0);
       DFMI_HIP(hipMemcpyAsync(h.scal.as<double>(), ones.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
       from_work<float>(yf.as<float>(), h.scal.as<double>(), h.tmp.as<double>(), n, k, nullptr, s);
@@ -1217,6 +1366,9 @@ int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision,
       else if (precision == 0)
         fine_apply<double>(h.pad, h.m64.as<double>(), H.levels[0]->mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream,
                            nullptr);
+      else if (precision == 2)
+        fine_apply_mfma(h.pad, h.m32.as<float>(), H.levels[0]->mask.as<uint8_t>(), h.atab.as<unsigned>(), MODE_APPLY, a,
+                        fine_tile(&h, k), h.stream);
       else
         fine_apply<float>(h.pad, h.m32.as<float>(), H.levels[0]->mask.as<uint8_t>(), h.kb, MODE_APPLY, a, h.stream,
                           nullptr);
@@ -1234,6 +1386,27 @@ int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms_per_launch = ms / reps;
+  })
+}
+
+int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double* ms_per_launch) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    if (k < 1) throw std::invalid_argument("k must be positive");
+    if (!fine_mfma_supported(h.pad, MODE_CHEB)) throw std::invalid_argument("grid not supported on the matrix cores");
+    Hierarchy& H = h.slots[slot];
+    if (tile && tile[0] > 0) {  // set this tile for k without timing
+      if ((tile[0] != 8 && tile[0] != 16) || tile[1] < 1) throw std::invalid_argument("tile must be {8 or 16, >= 1}");
+      h.mfma_tiles[k] = FineMfmaTile{tile[0], tile[1]};
+      if (ms_per_launch) *ms_per_launch = 0.0;
+      return 0;
+    }
+    ensure_level_vectors(&h, H, k);
+    double ms = 0.0;
+    const FineMfmaTile best = tune_fine(&h, H, k, &ms);
+    if (tile) tile[0] = best.tye, tile[1] = best.xchunk;
+    if (ms_per_launch) *ms_per_launch = ms;
   })
 }
 

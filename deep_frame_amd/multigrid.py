@@ -18,8 +18,13 @@ MULTIGRID = {"coarse": "galerkin", "coarsest_dofs": 80000, "max_levels": 8, "smo
              "damping": 1.0, "chebyshev_degree": 3, "chebyshev_ratio": 20.0, "growth": 2.0, "cycle": "V",
              "power_iterations": 20, "precision": "float32", "tolerance": 1e-8, "max_iterations": 2000,
              "batch": 64, "projection": True, "dead_ratio": 1e-12, "zero_rhs": 1e-12, "levels": 0, "seed": 0,
-             "refine": False, "inner_tolerance": 1e-2, "device": "cuda"}
+             "refine": False, "inner_tolerance": 1e-2, "device": "cuda", "fine_smoother": "vector", "retries": 2}
 PRECISIONS = {"float64": 0, "float32": 1}
+# operator precisions of dfmi_apply: "bf16x3" is the V-cycle's fine-level matrix-core kernel (FP32 vectors,
+# three split-bf16 products per element block)
+APPLY_PRECISIONS = {**PRECISIONS, "bf16x3": 2}
+# fine-level smoother and residual of the fp32 V-cycle: on the matrix cores or on the vector units
+FINE_SMOOTHERS = {"vector": 0, "matrix": 1}
 SLOTS = 8
 
 
@@ -34,7 +39,7 @@ class GeometricMultigrid:
         self.settings = {**MULTIGRID, **(settings or {})}
         s = self.settings
         if (s["precision"] not in PRECISIONS or s["coarse"] != "galerkin" or s["smoother"] != "chebyshev"
-                or s["cycle"] != "V"):
+                or s["cycle"] != "V" or s["fine_smoother"] not in FINE_SMOOTHERS):
             raise ValueError("Invalid multigrid settings")
         if not torch.cuda.is_available():
             raise RuntimeError("GeometricMultigrid needs a ROCm GPU (libdfmi has no CPU path)")
@@ -65,6 +70,8 @@ class GeometricMultigrid:
         n.power_iterations, n.precision = int(s["power_iterations"]), PRECISIONS[s["precision"]]
         n.tolerance, n.max_iterations, n.dead_ratio = float(s["tolerance"]), int(s["max_iterations"]), float(s["dead_ratio"])
         n.seed, n.refine, n.inner_tolerance = int(s["seed"]), int(bool(s["refine"])), float(s["inner_tolerance"])
+        n.fine_smoother = FINE_SMOOTHERS[s["fine_smoother"]]
+        n.retries = int(s["retries"])
         return n
 
     def _stream(self):
@@ -114,7 +121,8 @@ class GeometricMultigrid:
         return entry
 
     def solve_device(self, key, fixed, rhs):
-        """rhs: (k, ndof) FP64 device tensor -> (x (k, ndof) device tensor, list of per-column reports)."""
+        """rhs: (k, ndof) FP64 device tensor -> (x (k, ndof) device tensor, per-column reports
+        (iterations, relative residual, converged, FP64 retry rounds))."""
         torch = self.torch
         entry = self.hierarchy(key, fixed)
         rhs = rhs.to(device=self.device, dtype=torch.float64).contiguous()
@@ -123,7 +131,7 @@ class GeometricMultigrid:
         reports = (_native.Report * k)()
         self._stream()
         _native.call("dfmi_solve", self.handle, entry["slot"], _native.ptr(rhs), _native.ptr(x), k, reports)
-        return x, [(r.iterations, r.relative_residual, bool(r.converged)) for r in reports]
+        return x, [(r.iterations, r.relative_residual, bool(r.converged), r.retried) for r in reports]
 
     def solve(self, key, fixed, forces, support=None):
         """Reference signature (topology_multigrid.py:336-375); forces (ndof, k) host FP64."""
@@ -136,17 +144,18 @@ class GeometricMultigrid:
         live = np.flatnonzero(norms > self.settings["zero_rhs"] * max(norms.max(initial=0.0), 1e-300))
         flat = np.zeros_like(forces)
         iterations, relative = np.zeros(forces.shape[1], dtype=int), np.zeros(forces.shape[1])
-        batches = 0
+        batches = retried = 0
         for start in range(0, len(live), self.settings["batch"]):
             cols = live[start:start + self.settings["batch"]]
             x, reps = self.solve_device(key, fixed, self._tensor(forces[:, cols].T))
             flat[:, cols] = x.cpu().numpy().T
             iterations[cols] = [r[0] for r in reps]
             relative[cols] = [r[1] for r in reps]
+            retried += sum(r[3] for r in reps)
             batches += 1
         converged = bool(np.all(relative[live] < self.settings["tolerance"])) if len(live) else True
         report = {"iterations": iterations.tolist(), "relative_residual": relative.tolist(), "converged": converged,
-                  "batches": batches, "retried_columns": 0, "zero_rhs_columns": int(forces.shape[1] - len(live)),
+                  "batches": batches, "retried_columns": int(retried), "zero_rhs_columns": int(forces.shape[1] - len(live)),
                   "seconds": perf_counter() - started, "setup_s": entry.pop("setup_s", 0.0), "floating": False,
                   "levels": self.level_count, "coarsest_dofs": entry["coarsest_dofs"], "kernel": 0,
                   "lambda_max": entry["lambda_max"]}
@@ -159,8 +168,17 @@ class GeometricMultigrid:
         y = self.torch.empty_like(x)
         slot = -1 if key is None else self.hierarchies[key]["slot"]
         self._stream()
-        _native.call("dfmi_apply", self.handle, slot, PRECISIONS[precision], _native.ptr(x), _native.ptr(y), x.shape[0])
+        _native.call("dfmi_apply", self.handle, slot, APPLY_PRECISIONS[precision], _native.ptr(x), _native.ptr(y),
+                     x.shape[0])
         return y
+
+    def tune_fine(self, key, k, tile=None):
+        """Run the matrix-core fine-kernel autotuner for k columns, or set ``tile`` = (rows, x chunk) for k;
+        returns (tile rows, x chunk, ms per launch)."""
+        tile, ms = (c_int32 * 2)(*(tile or (0, 0))), c_double()
+        self._stream()
+        _native.call("dfmi_tune_fine", self.handle, self.hierarchies[key]["slot"], int(k), tile, byref(ms))
+        return int(tile[0]), int(tile[1]), ms.value
 
     def level_operator(self, key, level, x):
         """Level operator on the padded node grid of ``level``; x (k, 3 * nodes_l) FP64 device tensor."""

@@ -45,6 +45,11 @@ typedef struct dfmi_settings {
   int32_t refine;            /* 0 = flexible PCG with FP64 outer operator (reference scheme);
                                 1 = FP32 PCG with reliable FP64 updates of solution and residual */
   double inner_tolerance;    /* refine = 1: reliable-update factor (FP32 residual drop between FP64 updates) */
+  int32_t fine_smoother;     /* precision = 1: fine-level smoother and residual of the V-cycle on the FP32 vector
+                                units (0, default) or on the matrix cores (1, split-bf16 MFMA); the Krylov
+                                operator is always the FP32 / FP64 vector kernel */
+  int32_t retries;           /* 2: rounds of FP64 re-solves of columns that did not converge (warm start when
+                                their relative residual is below 1), topology_multigrid.py:376-406 */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */
@@ -52,6 +57,7 @@ typedef struct dfmi_report {
   int32_t iterations;        /* PCG iterations (all outer steps for refine = 1) */
   double relative_residual;  /* true FP64 |b - K x| / |b| at exit */
   int32_t converged;         /* relative_residual < tolerance */
+  int32_t retried;           /* FP64 retry rounds this column took (0 if the first solve converged) */
 } dfmi_report;
 
 const char* dfmi_last_error(void);
@@ -77,12 +83,14 @@ int dfmi_update(dfmi_mg* mg, const double* moduli_dev);
 int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed_dev, double* setup_seconds);
 
 /* GeometricMultigrid.solve(key, fixed, forces) for a built slot — topology_multigrid.py:336-375
- * (batched flexible PCG :407-454).  rhs_dev, x_dev: k flat FP64 columns.  report: host array of k. */
+ * (batched flexible PCG :407-454, FP64 retries of failed columns :376-406).  rhs_dev, x_dev: k flat FP64
+ * columns.  report: host array of k. */
 int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs_dev, double* x_dev, int32_t k, dfmi_report* report);
 
 /* GeometricMultigrid.operator / product — topology_multigrid.py:105-109, :455-457.
  * slot < 0: unmasked K(E) x; slot >= 0: M K M x with that hierarchy's fine mask.
- * precision: 0 = FP64, 1 = FP32 (inputs and outputs stay FP64). */
+ * precision: 0 = FP64, 1 = FP32, 2 = FP32 vectors with split-bf16 matrix-core products (the V-cycle's
+ * fine-level kernel); inputs and outputs stay FP64. */
 int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x_dev, double* y_dev, int32_t k);
 
 /* GeometricMultigrid.element_energy — topology_multigrid.py:458-459: energy_dev[r*Ne + e] = u_e^T Ke u_e. */
@@ -112,9 +120,16 @@ int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, c
 
 /* Average duration (ms, HIP events on the handle's stream) of `reps` launches of the operator kernel of
  * `level` (0 = matrix-free fine level, >= 1 = Galerkin stencil level, not the coarsest) on k columns of a
- * built slot, precision 0 = FP64, 1 = FP32.  For benchmarks. */
+ * built slot, precision 0 = FP64, 1 = FP32, 2 = FP32 on the matrix cores (fine level).  For benchmarks. */
 int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision, int32_t k, int32_t reps,
                     double* ms_per_launch);
+
+/* Autotuner of the matrix-core fine-level kernel (no counterpart in the reference, whose fine operator is
+ * a cuDNN convolution, deep_frame/topology_multigrid.py:101-109): times each candidate workgroup tile on k
+ * columns of a built slot with HIP events and keeps the fastest for k.  tile[0] = element rows per tile,
+ * tile[1] = node planes per workgroup along x; ms_per_launch = the winner's time.  Either may be NULL.
+ * With tile[0] > 0 on entry the given tile is set for k without timing (tile[0] in {8, 16}). */
+int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double* ms_per_launch);
 
 /* Diagnostics of a built slot: lambda_max per smoothed level and the coarsest live DOF count. */
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs);

@@ -27,6 +27,8 @@ def test_default_settings_mirror_reference_multigrid_defaults():
     s = _native.default_settings()
     assert (s.coarsest_dofs, s.max_levels, s.chebyshev_degree, s.power_iterations, s.max_iterations) == (80000, 8, 3, 20, 2000)
     assert (s.chebyshev_ratio, s.growth, s.tolerance, s.dead_ratio) == (20.0, 2.0, 1e-8, 1e-12)
+    assert s.retries == 2  # MULTIGRID["retries"], topology_multigrid.py:9
+    assert s.fine_smoother == 0 and s.precision == 1
 
 
 def test_invalid_arguments_raise_without_touching_a_gpu():

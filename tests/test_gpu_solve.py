@@ -12,9 +12,12 @@ from oracle import hex8, loads, system
 
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
-# solver schemes: FP64 V-cycle, FP32 V-cycle inside FP64 PCG, FP32 PCG inside FP64 iterative refinement
+# solver schemes: FP64 V-cycle, FP32 V-cycle inside FP64 PCG, FP32 PCG with reliable FP64 updates; the FP32
+# V-cycle's fine level runs on the FP32 vector units, or on the matrix cores (split bf16) for "-matrix"
 MODES = {"float64": {"precision": "float64"}, "float32": {"precision": "float32"},
-         "refine": {"precision": "float32", "refine": True}}
+         "refine": {"precision": "float32", "refine": True},
+         "float32-matrix": {"precision": "float32", "fine_smoother": "matrix"},
+         "refine-matrix": {"precision": "float32", "refine": True, "fine_smoother": "matrix"}}
 
 
 def make(shape, spacing, ke, moduli, settings=None):
@@ -93,7 +96,7 @@ def test_covariance_cantilever_dense_flexibility(torch_gpu):
     mg.close()
 
 
-@pytest.mark.parametrize("mode", ["float32", "refine"])
+@pytest.mark.parametrize("mode", ["float32", "refine", "refine-matrix"])
 def test_cantilever_config_compliance_and_sensitivity(mode, torch_gpu):
     """48 x 24 x 24 cantilever: compliance 1e-6 and sensitivity 1e-5 against the FP64 oracle."""
     prob = problems.cantilever(seed=0)
@@ -149,3 +152,34 @@ def test_refine_matches_flexible_pcg_on_frame_wrenches(torch_gpu):
     for c in range(42):
         a, b = out["float32"][c], out["refine"][c]
         assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
+
+
+@pytest.mark.parametrize("mode", ["float32", "refine"])
+def test_failed_columns_are_retried_in_fp64(mode, torch_gpu):
+    """Columns that miss the tolerance within max_iterations are solved again with the FP64 V-cycle from
+    their iterate (topology_multigrid.py:376-406): with retries they converge to the direct solution, the
+    iterations add up and the report counts the retry rounds; without retries they stay unconverged."""
+    print(f"retry mode={mode}", flush=True)
+    shape = (16, 8, 8)
+    ke, _, _ = hex8.hex8((1, 1, 1), hex8.isotropic_d(0.3))
+    rho = np.clip(np.random.default_rng(0).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
+    mod = 1e-6 + (1 - 1e-6) * rho ** 3
+    pts, conn, dofs = hex8.grid(shape)
+    fixed = problems.clamp_dofs(np.flatnonzero(pts[:, 0] == 0))
+    f = np.random.default_rng(1).normal(size=(3 * len(pts), 3))
+    f[fixed] = 0
+    ref, _ = system.solve(dofs, conn, ke, mod, fixed, f, len(pts))
+    limit = 6
+    mg = make(shape, (1, 1, 1), ke, mod, {**MODES[mode], "max_iterations": limit, "retries": 0})
+    _, rep0 = mg.solve(b"r", fixed, f)
+    mg.close()
+    assert not rep0["converged"] and rep0["retried_columns"] == 0
+    assert all(it <= limit for it in rep0["iterations"])
+    mg = make(shape, (1, 1, 1), ke, mod, {**MODES[mode], "max_iterations": limit, "retries": 3})
+    u, rep = mg.solve(b"r", fixed, f)
+    print(rep["iterations"], rep["relative_residual"], rep["retried_columns"], flush=True)
+    assert rep["converged"] and rep["retried_columns"] >= 3
+    assert all(it > limit for it in rep["iterations"])
+    for c in range(3):
+        assert np.linalg.norm(u[:, c] - ref[:, c]) < 1e-6 * np.linalg.norm(ref[:, c])
+    mg.close()

@@ -9,14 +9,15 @@ from oracle import gmg
 pytestmark = pytest.mark.gpu
 
 
-def case(shape, spacing, seed, precision, torch):
+def case(shape, spacing, seed, precision, torch, fine_smoother="vector"):
     from deep_frame_amd.multigrid import GeometricMultigrid
     ke = element.stiffness(spacing, element.isotropic(0.3))
     rho = np.clip(np.random.default_rng(seed).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
     mod = 1e-6 + (1 - 1e-6) * rho ** 3
     pts = element.node_points(shape, spacing)
     fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
-    mg = GeometricMultigrid(shape, spacing, ke, {"coarsest_dofs": 200, "precision": precision})
+    mg = GeometricMultigrid(shape, spacing, ke, {"coarsest_dofs": 200, "precision": precision,
+                                                 "fine_smoother": fine_smoother})
     mg.update(mod)
     entry = mg.hierarchy("v", fixed)
     levels = gmg.hierarchy(shape, ke, mod, fixed, gmg.level_count(shape, 200))
@@ -39,11 +40,15 @@ def test_transfer_operators_match_prolongation(shape, torch_gpu):
     mg.close()
 
 
-@pytest.mark.parametrize("shape,spacing", [((24, 4, 4), (2.0, 1.5, 1.5)), ((16, 8, 8), (1.0, 1.0, 1.0))])
-@pytest.mark.parametrize("precision", ["float64", "float32"])
+@pytest.mark.parametrize("shape,spacing", [((24, 4, 4), (2.0, 1.5, 1.5)), ((16, 8, 8), (1.0, 1.0, 1.0)),
+                                           ((16, 8, 40), (1.0, 1.0, 0.5))])
+@pytest.mark.parametrize("precision", ["float64", "float32", "float32-matrix"])
 def test_vcycle_matches_oracle(shape, spacing, precision, torch_gpu):
+    """FP64 V-cycle to 1e-9; FP32 V-cycle to 1e-4, with the fine level on the FP32 vector units (the
+    default) or on the matrix cores (split bf16)."""
     print(f"vcycle shape={shape} precision={precision}", flush=True)
-    mg, entry, levels, fixed = case(shape, spacing, 1, precision, torch_gpu)
+    smoother = "matrix" if precision.endswith("matrix") else "vector"
+    mg, entry, levels, fixed = case(shape, spacing, 1, precision.split("-")[0], torch_gpu, smoother)
     r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], 2)) * levels[0]["mask"][:, None]
     ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c]) for c in range(2)])
     z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
