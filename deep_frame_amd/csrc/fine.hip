@@ -178,14 +178,16 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     }
   }
   T nv[2][3];
+  // branch-free staging: slots outside the grid read node 0 and are zeroed by their mask
   auto fetch = [&](int ip) {
     const bool in = ip >= 0 && ip < g.nx;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      nv[k][0] = nv[k][1] = nv[k][2] = T(0);
-      if (in && so[k] >= 0) {
-        const int n = ip * plane_nodes + so[k];
-        const uint8_t m = mask[n];
+      {
+        const bool ok = in && so[k] >= 0;
+        const int n = ok ? ip * plane_nodes + so[k] : 0;
+        const uint8_t mr = mask[n];
+        const uint8_t m = ok ? mr : 0;
         T v0, v1, v2;
         if (MODE == MODE_CHEB0) {  // x1 = c0 Dinv b
           const T* bs = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * n;
@@ -218,7 +220,9 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     }
   };
   auto modulus = [&](int i) -> T {
-    return (elem_in && i >= 0 && i < g.ex) ? moduli[((long long)i * g.ey + ej) * g.ez + el] : T(0);
+    const bool ok = elem_in && i >= 0 && i < g.ex;
+    const T e = moduli[ok ? (i * g.ey + ej) * g.ez + el : 0];
+    return ok ? e : T(0);
   };
 
   T carry[3] = {T(0), T(0), T(0)};
@@ -234,24 +238,26 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     const T e_next = modulus(i + 1);
     const int nn = i * plane_nodes + ej * g.nz + el;
     const bool out = own && i >= i0;
-    const uint8_t mk = out ? mask[nn] : 0;
+    const int nnc = out ? nn : 0;  // loads below are unconditional (node 0 when this thread has no output)
+    const uint8_t mkr = mask[nnc];
+    const uint8_t mk = out ? mkr : 0;
     // epilogue operands of node (i, ej, el), requested early
     T bv[3] = {T(0), T(0), T(0)}, xpv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)};
-    if (out) {
+    {
       if (MODE == MODE_RESID || MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
-        const T* bp = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * nn;
+        const T* bp = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * nnc;
         bv[0] = bp[0];
         bv[1] = bp[1];
         bv[2] = bp[2];
       }
       if (MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
         if (MODE == MODE_CHEB && a.xp) {
-          const T* pp = reinterpret_cast<const T*>(a.xp) + (long long)r * col + 3LL * nn;
+          const T* pp = reinterpret_cast<const T*>(a.xp) + (long long)r * col + 3LL * nnc;
           xpv[0] = pp[0];
           xpv[1] = pp[1];
           xpv[2] = pp[2];
         }
-        const T* dp = reinterpret_cast<const T*>(a.dinv) + 3LL * nn;
+        const T* dp = reinterpret_cast<const T*>(a.dinv) + 3LL * nnc;
         dv[0] = dp[0];
         dv[1] = dp[1];
         dv[2] = dp[2];
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     xs[1] = U[b0][1][self];
     xs[2] = U[b0][2][self];
     T y[8][3];
-    if (e_cur != T(0)) {
+    {  // every element is evaluated (E = 0 gives zero contributions)
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int buf = (q & 1) ? b1 : b0;
@@ -289,9 +295,6 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
         y[q][1] = w[q][1];
         y[q][2] = w[q][2];
       }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) y[q][0] = y[q][1] = y[q][2] = T(0);
     }
 #pragma unroll
     for (int q = 2; q < 8; ++q) {

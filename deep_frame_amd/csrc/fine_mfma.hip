@@ -62,8 +62,12 @@ __global__ __launch_bounds__(TYE * 32) __attribute__((amdgpu_waves_per_eu(ZB <= 
   constexpr int NOWN = (TYE - 1) * SZ;  // owned node slots of a plane
   constexpr int KEP = (NOWN + NT - 1) / NT;
   constexpr bool NEEDX = MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1;
-  __shared__ unsigned Pk[2][3][NS];    // node planes i, i+1 of the operand, split into bf16 hi / lo
-  __shared__ float ACC[2][2][3][NS];   // [plane i / i+1][y corner][component][node slot], z already summed
+  // LDS strides padded so that the four 16-lane groups of a wave (two element rows, two node planes) fall
+  // on different banks: plane buffers 32 words apart modulo 64, accumulator groups likewise
+  constexpr int NSP = NS + ((32 - NS % 64) + 64) % 64;  // Pk stride per component, = 32 (mod 64)
+  constexpr int NSA = NS + ((16 - NS % 32) + 32) % 32;  // ACC stride per (y corner, component), = 16 (mod 32)
+  __shared__ unsigned Pk[2][3][NSP];   // node planes i, i+1 of the operand, split into bf16 hi / lo
+  __shared__ float ACC[2][2][3][NSA];  // [plane i / i+1][y corner][component][node slot], z already summed
 
   const int r = blockIdx.y;
   if (a.active && !a.active[r]) return;
@@ -100,18 +104,24 @@ __global__ __launch_bounds__(TYE * 32) __attribute__((amdgpu_waves_per_eu(ZB <= 
   const mf_bf16x8 Ahb = __builtin_bit_cast(mf_bf16x8, ahb), Alb = __builtin_bit_cast(mf_bf16x8, alb);
   const mf_s16x4 Ahs = __builtin_bit_cast(mf_s16x4, ahs), Als = __builtin_bit_cast(mf_s16x4, als);
 
-  // staging of node plane ip (slots t + k NT)
+  // staging of node plane ip (slots t + k NT); so[k]: node offset within a plane, or -1 outside the grid
+  int so[KST];
+#pragma unroll
+  for (int k = 0; k < KST; ++k) {
+    const int v = t + k * NT;
+    const int yy = v / SZ, zz = v - (v / SZ) * SZ;
+    const int gy = y0 - 1 + yy;
+    so[k] = (v < NS && gy >= 0 && gy < g.ny && zz < g.nz) ? gy * g.nz + zz : -1;
+  }
   unsigned sv[KST][3];
   auto fetch = [&](int ip) {
+    const bool plane_in = ip >= 0 && ip < g.nx;
 #pragma unroll
     for (int k = 0; k < KST; ++k) {
       float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-      const int v = t + k * NT;
-      if (v < NS && ip >= 0 && ip < g.nx) {
-        const int yy = v / SZ, zz = v - (v / SZ) * SZ;
-        const int gy = y0 - 1 + yy;
-        if (gy >= 0 && gy < g.ny && zz < g.nz) {
-          const int nn = ip * plane_nodes + gy * g.nz + zz;
+      {
+        if (plane_in && so[k] >= 0) {
+          const int nn = ip * plane_nodes + so[k];
           const uint8_t m = mask[nn];
           if (MODE == MODE_CHEB0) {
             const float c0 = (float)a.c0;
