@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="frame", choices=sorted(CONFIGS))
     ap.add_argument("--variants", type=int, default=None, help="variants config: number of variants (default 64)")
+    ap.add_argument("--streams", type=int, default=4, help="variants config: concurrent HIP streams")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--scheme", default="refine", choices=["refine", "flexible"],
@@ -60,6 +61,7 @@ def parse():
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--fine-smoother", default="vector", choices=["matrix", "vector"],
                     help="fine level of the fp32 V-cycle on the matrix cores (split bf16) or the FP32 vector units")
+    ap.add_argument("--fp64-only", action="store_true", help="only time the plain FP64 solve (for kernel traces)")
     ap.add_argument("--apply-only", action="store_true",
                     help="build the hierarchy once and only time the fine operator (for counter profiles)")
     return ap.parse_args()
@@ -158,6 +160,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration_copy_bytes": src.numel() * 4}))
         return
+    if args.fp64_only:
+        _, t64, rep64, _ = run({"precision": "float64"}, args.steps, args.warmup, "fp64")
+        print(json.dumps({"fp64_s": float(np.mean(t64)), "iterations": max(r[0] for r in rep64)}))
+        return
     mixed = {"precision": "float32", "refine": args.scheme == "refine", "fine_smoother": args.fine_smoother}
     if args.inner_tolerance:
         mixed["inner_tolerance"] = args.inner_tolerance
@@ -220,7 +226,7 @@ def run_variants(args, cfg):
     vs = variants.generate(count=args.variants or cfg["count"], seed=0)
     print(f"# variants {len(vs)}, total RHS {sum(v['rhs'] for v in vs)}", file=sys.stderr, flush=True)
     variants.run(vs[:1])  # warm-up
-    out = variants.run(vs, check=not args.no_fp64)
+    out = variants.run(vs, check=not args.no_fp64, streams=args.streams)
     recs = out["variants"]
     summary = {"max_iterations": max(r["iterations"] for r in recs), "max_rel_res": max(r["max_rel_res"] for r in recs),
                "all_converged": all(r["converged"] for r in recs), "seconds": out["seconds"]}
@@ -233,6 +239,7 @@ def run_variants(args, cfg):
         "ms_per_step": out["seconds"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 inner PCG / fp64 refinement", "data": "synthetic",
         "config": {"workload": f"{len(vs)} frame variants, nx, ny in [48, 160], nz in [12, 32], 6-42 RHS",
+                   "streams": args.streams,
                    **summary, "per_variant": recs}}))
 
 

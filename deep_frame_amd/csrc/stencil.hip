@@ -17,8 +17,27 @@ __device__ inline double weight1(int t) { return t == 0 ? 1.0 : ((t == 1 || t ==
 // gather: stencil entries towards missing or dead neighbours are zero (Galerkin product and
 // stencil_mask) and level vectors are zero on dead DOFs, so a neighbour index that runs off a grid
 // line only multiplies a zero; it is clamped into the array.
-constexpr int SNT = 256;
+constexpr int SNT = 512;
 constexpr int SP = 244;  // padded LDS row of 243 stencil entries
+
+// Three consecutive entries of a level vector through a raw buffer load: byte offsets outside the
+// buffer (a neighbour beyond either end of the array; its stencil entries are zero) read zeros, so the
+// gather needs no clamp, and the address is one 32-bit offset from a scalar base.
+__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, unsigned byte, float& a, float& b, float& c) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, byte, 0, 0);
+  a = __uint_as_float(v[0]);
+  b = __uint_as_float(v[1]);
+  c = __uint_as_float(v[2]);
+}
+__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, unsigned byte, double& a, double& b, double& c) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte, 0, 0);
+  const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, byte + 16, 0, 0);
+  a = __hiloint2double((int)v[1], (int)v[0]);
+  b = __hiloint2double((int)v[3], (int)v[2]);
+  c = __hiloint2double((int)w[1], (int)w[0]);
+}
+// buffer descriptor word 3 for raw 32-bit loads on gfx9 (data format 32)
+constexpr int kBufferWord3 = 0x00020000;
 
 template <typename T, int RP, int MODE>
 __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restrict__ st,
@@ -29,10 +48,14 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
   const int N = (int)g.nodes();
   const int n0 = (int)xcd_block(blockIdx.x, gridDim.x) * nodes_per_block;
   const int t = threadIdx.x;
-  for (int v = t; v < 243 * nodes_per_block; v += SNT) {
-    const int e = v / nodes_per_block, j = v - e * nodes_per_block;
+  {  // stage the block's stencils: thread t handles node j = t % npb and entries e = t / npb, + SNT / npb, ...
+    const int npb = nodes_per_block, step = SNT / npb;
+    const int j = t % npb, e0 = t / npb;
     const int n = n0 + j;
-    sst[j * SP + e] = n < N ? st[(long long)e * N + n] : T(0);
+    if (e0 < step) {
+      const T* src = st + n;
+      for (int e = e0; e < 243; e += step) sst[j * SP + e] = n < N ? src[(long long)e * N] : T(0);
+    }
   }
   __syncthreads();
   const int j = t / groups, grp = t - (t / groups) * groups;
@@ -48,22 +71,28 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
   if (!any) return;
   const int ns = (int)lay.ns, rs = (int)lay.rs;
   const T* X = reinterpret_cast<const T*>(a.x);
+  // the buffer spans the whole multi-vector (interleaved or column-major)
+  const long long extent = ((long long)(N - 1) * ns + (long long)(a.k - 1) * rs + 3) * (long long)sizeof(T);
+  const __amdgpu_buffer_rsrc_t xr =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(X), (short)0, (int)extent, kBufferWord3);
   const T* s = sst + j * SP;
   T acc[RP][3];
 #pragma unroll
   for (int q = 0; q < RP; ++q) acc[q][0] = acc[q][1] = acc[q][2] = T(0);
-  const int pz = 1, py = g.nz, px = g.ny * g.nz;
+  const int py = g.nz, px = g.ny * g.nz;
+  const int nsb = ns * (int)sizeof(T);
+  unsigned cb[RP];  // byte offset of each of this thread's columns within node n
+#pragma unroll
+  for (int q = 0; q < RP; ++q) cb[q] = (unsigned)(n * nsb + min(r0 + q, a.k - 1) * rs * (int)sizeof(T));
 #pragma unroll 3
   for (int d = 0; d < 27; ++d) {
-    int m = n + (d / 9 - 1) * px + ((d / 3) % 3 - 1) * py + (d % 3 - 1) * pz;
-    m = min(max(m, 0), N - 1);
+    const int od = ((d / 9 - 1) * px + ((d / 3) % 3 - 1) * py + (d % 3 - 1)) * nsb;  // uniform
     const T* sd = s + d * 9;
     const T s0 = sd[0], s1 = sd[1], s2 = sd[2], s3 = sd[3], s4 = sd[4], s5 = sd[5], s6 = sd[6], s7 = sd[7], s8 = sd[8];
-    const T* xm = X + m * ns;
 #pragma unroll
     for (int q = 0; q < RP; ++q) {
-      const int col = min(r0 + q, a.k - 1) * rs;  // a missing odd column re-reads the last one
-      const T x0 = xm[col], x1 = xm[col + 1], x2 = xm[col + 2];
+      T x0, x1, x2;  // a missing odd column re-reads the last one
+      load3(xr, cb[q] + (unsigned)od, x0, x1, x2);
       acc[q][0] = fma(s0, x0, fma(s1, x1, fma(s2, x2, acc[q][0])));
       acc[q][1] = fma(s3, x0, fma(s4, x1, fma(s5, x2, acc[q][1])));
       acc[q][2] = fma(s6, x0, fma(s7, x1, fma(s8, x2, acc[q][2])));
@@ -115,8 +144,8 @@ void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, co
                    hipStream_t s, int* nblocks_out) {
   const int rp = stencil_rp(a.k), groups = (a.k + rp - 1) / rp;
   if (groups > SNT) throw std::runtime_error("stencil_apply: too many right-hand sides per call");
-  if (g.nodes() * 3 * (long long)a.k > (long long)INT32_MAX)
-    throw std::runtime_error("stencil_apply: level too large for 32-bit offsets");
+  if (g.nodes() * 3 * (long long)a.k * (long long)sizeof(T) > (long long)INT32_MAX)
+    throw std::runtime_error("stencil_apply: level vectors too large for 32-bit byte offsets");
   const int npb = stencil_npb<T>(a.k);
   const unsigned bx = (unsigned)((g.nodes() + npb - 1) / npb);
   const size_t lds = sizeof(T) * SP * (size_t)npb;

@@ -9,11 +9,12 @@ from oracle import hex8, system
 pytestmark = pytest.mark.gpu
 
 
-def test_variant_batch_matches_fp64(torch_gpu):
+@pytest.mark.parametrize("streams", [1, 3])
+def test_variant_batch_matches_fp64(streams, torch_gpu):
     from deep_frame_amd import variants
     vs = variants.generate(count=3, seed=5, nxy=(16, 28), nz=(6, 10), rhs=(6, 12))
-    print("variants", vs, flush=True)
-    out = variants.run(vs, check=True)
+    print("variants", vs, "streams", streams, flush=True)
+    out = variants.run(vs, check=True, streams=streams)
     for rec in out["variants"]:
         assert rec["converged"] and rec["max_rel_res"] < 1e-8
         assert rec["compliance_rel_err"] < 1e-6
