@@ -47,6 +47,8 @@ SIGNATURES = {
     "dfmi_time_apply": ([c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(c_double)], c_int32),
     "dfmi_tune_fine": ([c_void_p, c_int32, c_int32, POINTER(c_int32), POINTER(c_double)], c_int32),
     "dfmi_diagnostics": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
+    "dfmi_gram": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p], c_int32),
+    "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_void_p], c_int32),
     "dfmi_simp": ([c_void_p, c_void_p, c_int64, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p], c_int32),
     "dfmi_filter_create": ([POINTER(c_int32), POINTER(c_double), c_double, POINTER(c_uint8), POINTER(c_uint8),
                             POINTER(c_void_p)], c_int32),

@@ -1,0 +1,153 @@
+// Interface load-covariance measures (deep_frame/topology_problem.py:573-684) on device-resident
+// unit-wrench solutions: the unit flexibility F_w = W^T U (all pairs of load and solution columns)
+// and the combined displacement fields U M (load cases, eigen-directions) whose element energies
+// give the gradients.  FP64 throughout; HBM-bound streams over the k columns.
+#include <stdexcept>
+
+#include "dfmi_internal.h"
+#include "dfmi.h"
+
+namespace dfmi {
+
+constexpr int GRAM_NT = 256;
+constexpr int GRAM_CH = 64;     // DOFs staged per pass
+constexpr int GRAM_MAXK = 64;   // columns of each operand
+constexpr int GRAM_T = 4;       // each thread owns a 4 x 4 tile of output pairs
+
+// Per-block partial sums of C[i][j] = sum_n A[i][n] B[j][n] over the block's DOF chunks.
+__global__ __launch_bounds__(GRAM_NT) void k_gram(const double* __restrict__ a, const double* __restrict__ b,
+                                                   long long n, int ka, int kb, double* __restrict__ partial) {
+  __shared__ double As[GRAM_MAXK][GRAM_CH + 1];
+  __shared__ double Bs[GRAM_MAXK][GRAM_CH + 1];
+  const int t = threadIdx.x;
+  const int tiles_j = (kb + GRAM_T - 1) / GRAM_T;
+  const int ti = t / tiles_j, tj = t - (t / tiles_j) * tiles_j;
+  const bool owner = ti * GRAM_T < ka;
+  double acc[GRAM_T][GRAM_T];
+#pragma unroll
+  for (int x = 0; x < GRAM_T; ++x)
+#pragma unroll
+    for (int y = 0; y < GRAM_T; ++y) acc[x][y] = 0.0;
+  for (long long c0 = (long long)blockIdx.x * GRAM_CH; c0 < n; c0 += (long long)gridDim.x * GRAM_CH) {
+    for (int v = t; v < GRAM_MAXK * GRAM_CH; v += GRAM_NT) {
+      const int r = v / GRAM_CH, c = v - (v / GRAM_CH) * GRAM_CH;
+      const long long idx = c0 + c;
+      As[r][c] = (r < ka && idx < n) ? a[(long long)r * n + idx] : 0.0;
+      Bs[r][c] = (r < kb && idx < n) ? b[(long long)r * n + idx] : 0.0;
+    }
+    __syncthreads();
+    if (owner) {
+      for (int c = 0; c < GRAM_CH; ++c) {
+        double av[GRAM_T], bv[GRAM_T];
+#pragma unroll
+        for (int x = 0; x < GRAM_T; ++x) {
+          av[x] = As[min(ti * GRAM_T + x, GRAM_MAXK - 1)][c];
+          bv[x] = Bs[min(tj * GRAM_T + x, GRAM_MAXK - 1)][c];
+        }
+#pragma unroll
+        for (int x = 0; x < GRAM_T; ++x)
+#pragma unroll
+          for (int y = 0; y < GRAM_T; ++y) acc[x][y] = fma(av[x], bv[y], acc[x][y]);
+      }
+    }
+    __syncthreads();
+  }
+  if (!owner) return;
+  double* out = partial + (long long)blockIdx.x * ka * kb;
+#pragma unroll
+  for (int x = 0; x < GRAM_T; ++x)
+#pragma unroll
+    for (int y = 0; y < GRAM_T; ++y) {
+      const int i = ti * GRAM_T + x, j = tj * GRAM_T + y;
+      if (i < ka && j < kb) out[i * kb + j] = acc[x][y];
+    }
+}
+
+// out[e] = sum over blocks of partial[block][e], in block order
+__global__ void k_sum_blocks(const double* __restrict__ partial, int nblocks, int count, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  double s = 0.0;
+  for (int blk = 0; blk < nblocks; ++blk) s += partial[(long long)blk * count + e];
+  out[e] = s;
+}
+
+constexpr int COMB_M = 16;  // output columns per pass
+
+// Y[m][n] = sum_k U[k][n] M[k][m] for output columns m0 .. m0 + COMB_M
+__global__ __launch_bounds__(256) void k_combine(const double* __restrict__ u, long long n, int k,
+                                                  const double* __restrict__ mat, int mcols, int m0,
+                                                  double* __restrict__ y) {
+  __shared__ double Ms[GRAM_MAXK][COMB_M];
+  for (int v = threadIdx.x; v < GRAM_MAXK * COMB_M; v += blockDim.x) {
+    const int r = v / COMB_M, c = v - (v / COMB_M) * COMB_M;
+    Ms[r][c] = (r < k && m0 + c < mcols) ? mat[(long long)r * mcols + m0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int mc = min(COMB_M, mcols - m0);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double acc[COMB_M];
+#pragma unroll
+    for (int c = 0; c < COMB_M; ++c) acc[c] = 0.0;
+    for (int r = 0; r < k; ++r) {
+      const double v = u[(long long)r * n + i];
+#pragma unroll
+      for (int c = 0; c < COMB_M; ++c) acc[c] = fma(v, Ms[r][c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < COMB_M; ++c)
+      if (c < mc) y[(long long)(m0 + c) * n + i] = acc[c];
+  }
+}
+
+}  // namespace dfmi
+
+using namespace dfmi;
+
+#define DFMI_GUARD3(...)                 \
+  try {                                  \
+    __VA_ARGS__;                         \
+    return 0;                            \
+  } catch (const std::exception& e) {    \
+    dfmi::set_error(e.what());           \
+    return 1;                            \
+  } catch (...) {                        \
+    dfmi::set_error("unknown failure");  \
+    return 1;                            \
+  }
+
+extern "C" {
+
+int dfmi_gram(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, int32_t kb, double* out_dev,
+              double* work_dev, int32_t work_blocks, void* hip_stream) {
+  DFMI_GUARD3({
+    if (n < 1 || ka < 1 || kb < 1 || ka > GRAM_MAXK || kb > GRAM_MAXK)
+      throw std::invalid_argument("gram: need n >= 1 and 1 <= ka, kb <= 64");
+    if (((ka + GRAM_T - 1) / GRAM_T) * ((kb + GRAM_T - 1) / GRAM_T) > GRAM_NT)
+      throw std::invalid_argument("gram: too many output tiles");
+    if (work_blocks < 1) throw std::invalid_argument("gram: work_blocks must be >= 1");
+    hipStream_t s = (hipStream_t)hip_stream;
+    const long long chunks = (n + GRAM_CH - 1) / GRAM_CH;
+    const int blocks = (int)std::min<long long>(chunks, work_blocks);
+    hipLaunchKernelGGL(k_gram, dim3(blocks), dim3(GRAM_NT), 0, s, a_dev, b_dev, (long long)n, ka, kb, work_dev);
+    DFMI_HIP(hipGetLastError());
+    const int count = ka * kb;
+    hipLaunchKernelGGL(k_sum_blocks, dim3((count + 255) / 256), dim3(256), 0, s, work_dev, blocks, count, out_dev);
+    DFMI_HIP(hipGetLastError());
+  })
+}
+
+int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev, int32_t mcols, double* y_dev,
+                 void* hip_stream) {
+  DFMI_GUARD3({
+    if (n < 1 || k < 1 || k > GRAM_MAXK || mcols < 1) throw std::invalid_argument("combine: need n >= 1, 1 <= k <= 64");
+    hipStream_t s = (hipStream_t)hip_stream;
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+    for (int m0 = 0; m0 < mcols; m0 += COMB_M) {
+      hipLaunchKernelGGL(k_combine, dim3(blocks), dim3(256), 0, s, u_dev, (long long)n, k, m_dev, mcols, m0, y_dev);
+      DFMI_HIP(hipGetLastError());
+    }
+  })
+}
+
+}  // extern "C"

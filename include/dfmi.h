@@ -134,6 +134,17 @@ int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double
 /* Diagnostics of a built slot: lambda_max per smoothed level and the coarsest live DOF count. */
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs);
 
+/* InterfaceCovariance.matrix (deep_frame/topology_problem.py:639-646): out_dev[i * kb + j] =
+ * sum_n a_dev[i * n + n'] b_dev[j * n + n'] for ka load columns and kb solution columns (1..64 each), FP64,
+ * deterministic.  work_dev holds work_blocks * ka * kb doubles (per-block partial sums). */
+int dfmi_gram(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, int32_t kb, double* out_dev,
+              double* work_dev, int32_t work_blocks, void* hip_stream);
+
+/* Combined displacement fields of InterfaceCovariance.measure / energy (topology_problem.py:647-652, 675-684):
+ * y_dev[m * n + i] = sum_r u_dev[r * n + i] m_dev[r * mcols + m] for k (1..64) columns, FP64. */
+int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev, int32_t mcols, double* y_dev,
+                 void* hip_stream);
+
 /* SIMP interpolation E = E0 (eps + (1 - eps) rho^p), dE — topology_optimization.py:657-661;
  * inactive cells (active_dev[e] == 0) get 0. */
 int dfmi_simp(const double* rho_dev, const uint8_t* active_dev, int64_t n, double young, double penal, double eps,

@@ -3,8 +3,8 @@
 This package is the checker, never the product: only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 import it.  Every function restates, in plain FP64 NumPy/SciPy, the algorithm
-of the reference file:line it cites, and is pinned by the checks listed in
-``oracle/PINNING.md`` (golden vectors, known answers, closed forms).
+of the reference file:line it cites, and is pinned by the checks of
+``tests/test_oracle.py`` (golden vectors, known answers, closed forms, finite differences).
 
 Parity status per module:
   hex8      pinned (rigid modes, patch energy, Euler-Bernoulli KAT, dense
@@ -17,4 +17,6 @@ Parity status per module:
             KKT conditions of Svanberg's subproblem and a second, independent
             solver of the same subproblem
   loads     pinned (sum f = F, sum r x f = M exactly; min-norm property)
+  covariance pinned (golden load_mean / load_worst of the covariance cantilever;
+            central differences of the mean and KS-worst gradients)
 """

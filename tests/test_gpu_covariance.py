@@ -1,0 +1,103 @@
+"""Interface load-covariance measures on the GPU (deep_frame/topology_problem.py:573-684) against the
+oracle (oracle/covariance.py, pinned to the reference's golden load_mean / load_worst and to central
+differences) and the golden covariance cantilever."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from deep_frame_amd import element, problems
+from oracle import covariance as ocov
+from oracle import hex8, loads, system
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def test_gram_and_combine_match_numpy(torch_gpu):
+    from deep_frame_amd.covariance import InterfaceCovariance
+    torch = torch_gpu
+    rng = np.random.default_rng(0)
+    k, n = 42, 100003
+    a, b = rng.normal(size=(k, n)), rng.normal(size=(k, n))
+    cov = InterfaceCovariance(np.eye(k))
+    ta, tb = torch.as_tensor(a, device="cuda"), torch.as_tensor(b, device="cuda")
+    gram = cov.unit_flexibility(tb, ta)
+    assert np.max(np.abs(gram - a @ b.T)) < 1e-10 * np.max(np.abs(a @ b.T))
+    m = rng.normal(size=(k, 19))
+    y = cov.fields(tb, m).cpu().numpy()
+    assert np.max(np.abs(y - m.T @ b)) < 1e-12 * np.max(np.abs(m.T @ b))
+
+
+def test_golden_cantilever_mean_and_worst(torch_gpu):
+    """Golden docs/validation/formulation_covariance_cantilever.json: load_mean and load_worst of the
+    dense full model from GPU solves of its three unit loads."""
+    from deep_frame_amd.covariance import InterfaceCovariance
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    torch = torch_gpu
+    gold = json.loads((GOLDEN / "covariance_cantilever.json").read_text())
+    d = hex8.orthotropic_d(gold["orthotropic"]) / gold["orthotropic"]["e_xy_mpa"]
+    ke, _, _ = hex8.hex8((1, 1, 1), d)
+    shape = tuple(gold["shape_full"])
+    pts, conn, dofs = hex8.grid(shape, (1, 1, 1), gold["origin_mm"])
+    tol = 1e-6
+    fixed = problems.clamp_dofs(hex8.select_nodes(pts, [-tol, -6 - tol, -tol], [tol, 6 + tol, 12 + tol]))
+    cols = {}
+    for name, spec in gold["interfaces"].items():
+        lo, hi = np.asarray(spec["box"][0]) - tol, np.asarray(spec["box"][1]) + tol
+        cols[name] = loads.wrench_columns(pts, hex8.select_nodes(pts, lo, hi), spec["reference_mm"], 3 * len(pts))
+    index = {"Fx": 0, "Fy": 1, "Fz": 2}
+    w = np.column_stack([cols[name][:, index[dof]] for name, dof in gold["loads"]])
+    dens = np.random.default_rng(gold["density_seed"]).uniform(0.3, 1.0, (32, 6, 12))
+    rho = np.concatenate([dens[:, ::-1, :], dens], axis=1).ravel()
+    mod = gold["orthotropic"]["e_xy_mpa"] * (gold["simp"]["eps"] + (1 - gold["simp"]["eps"]) * rho ** 3)
+    mg = GeometricMultigrid(shape, (1, 1, 1), ke, {"precision": "float32", "refine": True})
+    mg.update(mod)
+    f = torch.as_tensor(np.ascontiguousarray(w.T), device="cuda")
+    u, reps = mg.solve_device(b"cov", fixed, f)
+    assert all(r[2] for r in reps)
+    cov = InterfaceCovariance(gold["sigma"])
+    rows = cov.measure(mg, u, f, None)
+    assert rows[0]["value"] == pytest.approx(gold["dense_full_model"]["load_mean"], rel=1e-7)
+    assert rows[1]["value"] == pytest.approx(gold["dense_full_model"]["load_worst"], rel=1e-7)
+    mg.close()
+
+
+@pytest.mark.parametrize("precision", ["float64", "refine"])
+def test_frame_measure_and_gradients_match_oracle(precision, torch_gpu):
+    """42 unit wrenches on a 24 x 24 x 6 frame with a seeded 42 x 42 covariance: values, KS weights and both
+    gradients against the oracle's SuperLU solution."""
+    from deep_frame_amd.covariance import InterfaceCovariance
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    torch = torch_gpu
+    shape = (24, 24, 6)
+    prob = problems.frame(shape, seed=3, rhs=42)
+    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
+    mod, dmod = problems.moduli(prob["density"])
+    a = np.random.default_rng(7).normal(size=(42, 42))
+    sigma = a @ a.T / 42 + 0.05 * np.eye(42)
+    pts, conn, dofs = hex8.grid(shape)
+    ref_u, _ = system.solve(dofs, conn, ke, mod, prob["fixed"], prob["forces"], len(pts))
+    dirs, _ = ocov.directions(sigma)
+    flex_ref = ocov.matrix(prob["forces"].T @ ref_u, dirs)
+    mean_ref, worst_ref, _, _, _ = ocov.aggregate(flex_ref, ks=50.0)
+    limits = {"mean_n_mm": 2.0 * mean_ref, "worst_n_mm": 0.9 * worst_ref}
+    _, _, _, weights, vectors = ocov.aggregate(flex_ref, ks=50.0, worst_limit=limits["worst_n_mm"])
+    g_mean, g_worst = ocov.gradients(dofs, ke, ref_u, dirs, vectors, weights, dmod)
+    settings = {"precision": "float64"} if precision == "float64" else {"precision": "float32", "refine": True}
+    mg = GeometricMultigrid(shape, (1, 1, 1), ke, {"coarsest_dofs": 3000, **settings})
+    mg.update(mod)
+    f = torch.as_tensor(np.ascontiguousarray(prob["forces"].T), device="cuda")
+    u, reps = mg.solve_device(b"frame", prob["fixed"], f)
+    assert all(r[2] for r in reps)
+    cov = InterfaceCovariance(sigma)
+    rows = cov.measure(mg, u, f, torch.as_tensor(dmod, device="cuda"), limits)
+    print("mean", rows[0]["value"], mean_ref, "worst", rows[1]["value"], worst_ref, flush=True)
+    assert rows[0]["value"] == pytest.approx(mean_ref, rel=1e-7)
+    assert rows[1]["value"] == pytest.approx(worst_ref, rel=1e-7)
+    gm = rows[0]["gradient"].cpu().numpy() * limits["mean_n_mm"]
+    gw = rows[1]["gradient"].cpu().numpy() * limits["worst_n_mm"]
+    assert np.linalg.norm(gm - g_mean) <= 1e-6 * np.linalg.norm(g_mean)
+    assert np.linalg.norm(gw - g_worst) <= 1e-6 * np.linalg.norm(g_worst)
+    mg.close()

@@ -9,7 +9,7 @@ import pytest
 from scipy.sparse.linalg import spsolve
 
 from deep_frame_amd import element, problems
-from oracle import filters, gmg, hex8, loads, mma, oc, system
+from oracle import covariance, filters, gmg, hex8, loads, mma, oc, system
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -77,6 +77,57 @@ def test_covariance_cantilever_golden_flexibility():
     assert np.max(np.abs(flex - expected)) < 1e-11 * np.max(np.abs(expected))
     sigma = np.asarray(gold["sigma"])
     assert np.trace(flex @ sigma) == pytest.approx(gold["dense_full_model"]["load_mean"], rel=1e-11)
+
+
+def test_covariance_measure_matches_golden_mean_and_worst():
+    """Golden docs/validation/formulation_covariance_cantilever.json: the dense full model's load_mean and
+    load_worst follow from its unit flexibility and Sigma (topology_problem.py:583-591, 639-670)."""
+    gold = json.loads((GOLDEN / "covariance_cantilever.json").read_text())
+    d, rank = covariance.directions(gold["sigma"])
+    assert rank == 3 and np.allclose(d @ d.T, gold["sigma"], atol=1e-14)
+    flex = covariance.matrix(gold["dense_flexibility"], d, factor=1.0)
+    mean, worst, ks, weights, vectors = covariance.aggregate(flex, ks=50.0)
+    assert mean == pytest.approx(gold["dense_full_model"]["load_mean"], rel=1e-12)
+    assert worst == pytest.approx(gold["dense_full_model"]["load_worst"], rel=1e-12)
+    assert worst <= ks <= worst * (1 + np.log(3) / 50) and weights.sum() == pytest.approx(1.0)
+
+
+def test_covariance_gradients_match_central_difference():
+    """d mean / d rho and d KS-worst / d rho of the covariance measure against central differences of
+    the measure recomputed from SuperLU solves."""
+    shape = (4, 3, 3)
+    ke, _, _ = hex8.hex8((1, 1, 1), hex8.isotropic_d(0.3))
+    pts, conn, dofs = hex8.grid(shape)
+    fixed = problems.clamp_dofs(np.flatnonzero(pts[:, 0] == 0))
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=(3 * len(pts), 3))
+    w[fixed] = 0
+    a = rng.normal(size=(3, 3))
+    sigma = a @ a.T + 0.1 * np.eye(3)
+    d, _ = covariance.directions(sigma)
+    rho = rng.uniform(0.3, 0.9, 36)
+
+    limit = None
+
+    def measure(r, grad=False):
+        # the worst-case gradient is used with a limit, which then also fixes the KS scale
+        mod, dmod = hex8.simp(r, 1.0, 3.0, 1e-9)
+        u, _ = system.solve(dofs, conn, ke, mod, fixed, w, len(pts))
+        flex = covariance.matrix(w.T @ u, d)
+        mean, worst, ks, weights, vectors = covariance.aggregate(flex, ks=5.0, worst_limit=limit)
+        if not grad:
+            return mean, ks
+        return covariance.gradients(dofs, ke, u, d, vectors, weights, dmod)
+
+    mod0, _ = hex8.simp(rho, 1.0, 3.0, 1e-9)
+    u0, _ = system.solve(dofs, conn, ke, mod0, fixed, w, len(pts))
+    limit = 0.8 * covariance.aggregate(covariance.matrix(w.T @ u0, d))[1]
+    g_mean, g_ks = measure(rho, grad=True)
+    direction = rng.normal(size=36)
+    h = 1e-6
+    plus, minus = measure(rho + h * direction), measure(rho - h * direction)
+    assert g_mean @ direction == pytest.approx((plus[0] - minus[0]) / (2 * h), rel=1e-6)
+    assert g_ks @ direction == pytest.approx((plus[1] - minus[1]) / (2 * h), rel=1e-5)
 
 
 def test_unit_wrenches_reproduce_the_wrench_with_minimum_norm():
