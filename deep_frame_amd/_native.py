@@ -38,6 +38,11 @@ SIGNATURES = {
     "dfmi_build_hierarchy": ([c_void_p, c_int32, c_void_p, POINTER(c_double)], c_int32),
     "dfmi_solve": ([c_void_p, c_int32, c_void_p, c_void_p, c_int32, POINTER(Report)], c_int32),
     "dfmi_apply": ([c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32], c_int32),
+    "dfmi_set_mass": ([c_void_p, POINTER(c_double)], c_int32),
+    "dfmi_mass_apply": ([c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32], c_int32),
+    "dfmi_mask": ([c_void_p, c_int32, c_void_p, c_int32], c_int32),
+    "dfmi_modal_sensitivity": ([c_void_p, c_void_p, c_int32, POINTER(c_double), POINTER(c_double), c_void_p, c_void_p,
+                                c_double, c_void_p], c_int32),
     "dfmi_element_energy": ([c_void_p, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_compliance_sensitivity": ([c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_double, c_void_p, c_void_p],
                                     c_int32),
@@ -48,7 +53,7 @@ SIGNATURES = {
     "dfmi_tune_fine": ([c_void_p, c_int32, c_int32, POINTER(c_int32), POINTER(c_double)], c_int32),
     "dfmi_diagnostics": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
     "dfmi_gram": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p], c_int32),
-    "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_void_p], c_int32),
+    "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_simp": ([c_void_p, c_void_p, c_int64, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p], c_int32),
     "dfmi_filter_create": ([POINTER(c_int32), POINTER(c_double), c_double, POINTER(c_uint8), POINTER(c_uint8),
                             POINTER(c_void_p)], c_int32),

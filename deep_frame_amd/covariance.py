@@ -71,7 +71,7 @@ class InterfaceCovariance:
         m_dev = torch.as_tensor(mat, device=u.device)
         y = torch.empty((mat.shape[1], u.shape[1]), dtype=torch.float64, device=u.device)
         _native.call("dfmi_combine", _native.ptr(u.contiguous()), u.shape[1], u.shape[0], _native.ptr(m_dev),
-                     mat.shape[1], _native.ptr(y), _native.stream_handle(torch, u.device))
+                     mat.shape[1], _native.ptr(y), 0, _native.stream_handle(torch, u.device))
         return y
 
     def energy(self, mg, u, mat, dmoduli):

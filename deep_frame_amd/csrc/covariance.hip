@@ -74,10 +74,10 @@ __global__ void k_sum_blocks(const double* __restrict__ partial, int nblocks, in
 
 constexpr int COMB_M = 16;  // output columns per pass
 
-// Y[m][n] = sum_k U[k][n] M[k][m] for output columns m0 .. m0 + COMB_M
+// Y[m][n] (+)= sum_k U[k][n] M[k][m] for output columns m0 .. m0 + COMB_M
 __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ u, long long n, int k,
                                                   const double* __restrict__ mat, int mcols, int m0,
-                                                  double* __restrict__ y) {
+                                                  double* __restrict__ y, int accumulate) {
   __shared__ double Ms[GRAM_MAXK][COMB_M];
   for (int v = threadIdx.x; v < GRAM_MAXK * COMB_M; v += blockDim.x) {
     const int r = v / COMB_M, c = v - (v / COMB_M) * COMB_M;
@@ -96,7 +96,10 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ u, l
     }
 #pragma unroll
     for (int c = 0; c < COMB_M; ++c)
-      if (c < mc) y[(long long)(m0 + c) * n + i] = acc[c];
+      if (c < mc) {
+        double* out = y + (long long)(m0 + c) * n + i;
+        *out = accumulate ? *out + acc[c] : acc[c];
+      }
   }
 }
 
@@ -138,13 +141,14 @@ int dfmi_gram(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, i
 }
 
 int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev, int32_t mcols, double* y_dev,
-                 void* hip_stream) {
+                 int32_t accumulate, void* hip_stream) {
   DFMI_GUARD3({
     if (n < 1 || k < 1 || k > GRAM_MAXK || mcols < 1) throw std::invalid_argument("combine: need n >= 1, 1 <= k <= 64");
     hipStream_t s = (hipStream_t)hip_stream;
     const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
     for (int m0 = 0; m0 < mcols; m0 += COMB_M) {
-      hipLaunchKernelGGL(k_combine, dim3(blocks), dim3(256), 0, s, u_dev, (long long)n, k, m_dev, mcols, m0, y_dev);
+      hipLaunchKernelGGL(k_combine, dim3(blocks), dim3(256), 0, s, u_dev, (long long)n, k, m_dev, mcols, m0, y_dev,
+                         accumulate);
       DFMI_HIP(hipGetLastError());
     }
   })

@@ -90,6 +90,8 @@ enum ApplyMode : int {
   MODE_CHEB0 = 5,   // fine only: first Chebyshev step from zero, the operand x1 = c0 Dinv b is formed
                     // while staging (a.x unused): y = x1 + c1 x1 + c2 Dinv (b - A x1)
   MODE_CHEB1 = 6,   // fine only: Chebyshev step whose previous iterate is x1 = c0 Dinv b (a.xp unused)
+  MODE_MASS = 7,    // fine only: y = M (sum_e m_e Me + diag(lumped)) M x with the element field m_e in place of
+                    // the moduli, Me from a.mdiag (Hadamard-diagonal form) and lumped in a.dinv (may be null)
 };
 
 struct ApplyArgs {
@@ -103,6 +105,7 @@ struct ApplyArgs {
   double* partial;    // [k][nblocks] partial dot products, DOT only
   const int* active;  // per-column flag (device), nullptr = all active
   int k;
+  double mdiag[8];    // MASS only: unit-density element mass in the corner-Hadamard basis, divided by 64
 };
 
 // ---- fine level (matrix-free hex8) -------------------------------------------------
@@ -125,6 +128,12 @@ void fine_apply_mfma(const Geom& g, const float* moduli, const uint8_t* mask, co
 std::vector<FineMfmaTile> fine_mfma_candidates(const Geom& g, int k);
 void fine_element_energy(const Geom& pad, int ux, int uy, int uz, const double* u, const KeBlocks& kb,
                          double* energy, int k, hipStream_t s);
+constexpr int MODAL_MAX_MODES = 16;
+// per element sum_r weight_r (stiffness_slope_e u_re^T Ke u_re - value_r mass_slope_e u_re^T Me u_re) / target
+void fine_modal_sensitivity(const Geom& pad, int ux, int uy, int uz, const double* u, int k, const KeBlocks& kb,
+                            const double* mdiag, const double* weight, const double* value,
+                            const double* stiffness_slope, const double* mass_slope, double target, double* out,
+                            hipStream_t s);
 void fine_diagonal(const Geom& g, const double* moduli, const uint8_t* mask, const double* ke_diag, double* diag,
                    hipStream_t s);
 void fine_stencil(const Geom& g, const double* moduli, const uint8_t* mask, const double* ke, double* stencil,

@@ -16,6 +16,7 @@
 // i and carries the four of layer i+1 in registers.  Sums are in a fixed order,
 // so results are deterministic.
 #include <cmath>
+#include <stdexcept>
 
 #include "dfmi_internal.h"
 
@@ -280,14 +281,24 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
         y[q][2] = U[buf][2][v];
       }
       butterfly(y);
-#pragma unroll
-      for (int q = 1; q < 8; ++q) {
-        y[q][0] *= e_cur;
-        y[q][1] *= e_cur;
-        y[q][2] *= e_cur;
-      }
       T w[8][3];
-      sym_mul(ks, y, w);
+      if constexpr (MODE == MODE_MASS) {  // the mass is diagonal in the Hadamard basis, every mode included
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const T m = e_cur * (T)a.mdiag[q];
+          w[q][0] = m * y[q][0];
+          w[q][1] = m * y[q][1];
+          w[q][2] = m * y[q][2];
+        }
+      } else {
+#pragma unroll
+        for (int q = 1; q < 8; ++q) {
+          y[q][0] *= e_cur;
+          y[q][1] *= e_cur;
+          y[q][2] *= e_cur;
+        }
+        sym_mul(ks, y, w);
+      }
       butterfly(w);
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -328,6 +339,9 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
             const T c1 = (T)a.c1, c2 = (T)a.c2;
             const T prev = MODE == MODE_CHEB1 ? (T)a.c0 * dv[c] * bv[c] : xpv[c];
             Y[3LL * nn + c] = on ? xs[c] + c1 * (xs[c] - prev) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
+          } else if (MODE == MODE_MASS) {
+            const T lump = a.dinv ? reinterpret_cast<const T*>(a.dinv)[3LL * nn + c] : T(0);
+            Y[3LL * nn + c] = on ? s0[c] + lump * xs[c] : T(0);
           }
         }
       }
@@ -431,6 +445,7 @@ void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlo
     case MODE_CHEB0: DFMI_LAUNCH(MODE_CHEB0); break;
     case MODE_CHEB1: DFMI_LAUNCH(MODE_CHEB1); break;
     case MODE_DOT: DFMI_LAUNCH(MODE_DOT); break;
+    case MODE_MASS: DFMI_LAUNCH(MODE_MASS); break;
     default: throw std::runtime_error("fine_apply: bad mode");
   }
 #undef DFMI_LAUNCH
@@ -480,6 +495,75 @@ void fine_element_energy(const Geom& pad, int ux, int uy, int uz, const double* 
   const long long ne = (long long)ux * uy * uz;
   dim3 grid((unsigned)((ne + 255) / 256), k);
   hipLaunchKernelGGL(k_element_energy, grid, dim3(256), 0, s, pad, ux, uy, uz, u, kb, energy);
+  DFMI_HIP(hipGetLastError());
+}
+
+// Sensitivity of the modal KS aggregate (topology_optimization.py:956-964): per element
+// sum_r w_r (s_e u_re^T Ke u_re - lambda_r m_e u_re^T Me u_re) / target over k mode shapes, with
+// Me = M8 (x) I3 diagonal in the corner-Hadamard basis (mdiag carries the 1/64 of H^T H = 8 I per axis).
+struct ModalTerms {
+  double weight[MODAL_MAX_MODES];
+  double value[MODAL_MAX_MODES];
+  double mdiag[8];
+};
+
+__global__ void k_modal_sensitivity(Geom pad, int ux, int uy, int uz, const double* __restrict__ u, int k,
+                                    Blocks<double> kb, ModalTerms mt, const double* __restrict__ stiffness_slope,
+                                    const double* __restrict__ mass_slope, double inv_target,
+                                    double* __restrict__ out) {
+  const long long ne = (long long)ux * uy * uz;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int el = (int)(e % uz);
+  const int ej = (int)((e / uz) % uy);
+  const int ei = (int)(e / ((long long)uz * uy));
+  const double ss = stiffness_slope[e], ms = mass_slope[e];
+  double acc = 0.0;
+  for (int r = 0; r < k; ++r) {
+    const double* X = u + (long long)r * pad.nodes() * 3;
+    double v[8][3];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const long long n = ((long long)(ei + (q & 1)) * pad.ny + ej + ((q >> 1) & 1)) * pad.nz + el + ((q >> 2) & 1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[q][c] = X[n * 3 + c];
+    }
+    butterfly(v);
+    double w[8][3];
+    blocks_mul(kb, v, w, 1.0);
+    double strain = 0.0, kinetic = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double sq = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        strain = fma(v[q][c], w[q][c], strain);
+        sq = fma(v[q][c], v[q][c], sq);
+      }
+      kinetic = fma(mt.mdiag[q], sq, kinetic);
+    }
+    acc += mt.weight[r] * (ss * strain - mt.value[r] * ms * kinetic);
+  }
+  out[e] = acc * inv_target;
+}
+
+void fine_modal_sensitivity(const Geom& pad, int ux, int uy, int uz, const double* u, int k, const KeBlocks& kbd,
+                            const double* mdiag, const double* weight, const double* value,
+                            const double* stiffness_slope, const double* mass_slope, double target, double* out,
+                            hipStream_t s) {
+  if (k < 1 || k > MODAL_MAX_MODES) throw std::invalid_argument("modal sensitivity takes 1 to 16 mode shapes");
+  Blocks<double> kb;
+  for (int p = 0; p < 8; ++p)
+    for (int e = 0; e < 9; ++e) kb.b[p][e] = kbd.b[p][e / 3][e % 3];
+  ModalTerms mt{};
+  for (int r = 0; r < k; ++r) {
+    mt.weight[r] = weight[r];
+    mt.value[r] = value[r];
+  }
+  for (int q = 0; q < 8; ++q) mt.mdiag[q] = mdiag[q];
+  const long long ne = (long long)ux * uy * uz;
+  hipLaunchKernelGGL(k_modal_sensitivity, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, pad, ux, uy, uz, u, k,
+                     kb, mt, stiffness_slope, mass_slope, 1.0 / target, out);
   DFMI_HIP(hipGetLastError());
 }
 

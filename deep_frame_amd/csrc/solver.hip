@@ -93,6 +93,10 @@ struct Handle {
   // fine level on the matrix cores: per-lane split-bf16 Ke operands and the tile chosen per column count
   DevBuf atab;
   std::map<int, FineMfmaTile> mfma_tiles;
+  // unit-density element mass in the corner-Hadamard basis (dfmi_set_mass) and a padded element field
+  double mdiag[8]{};
+  bool have_mass = false;
+  DevBuf mass_pad;
   bool have_moduli = false;
   Hierarchy slots[8];
   // PCG workspace (padded grid, FP64)
@@ -1253,6 +1257,92 @@ int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x, do
   })
 }
 
+int dfmi_set_mass(dfmi_mg* mg, const double* me) {
+  DFMI_GUARD({
+    if (!me) throw std::invalid_argument("null mass matrix");
+    Handle& h = mg->h;
+    double mb[24][24];
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        const int qi = kRefToBits[i / 3] * 3 + i % 3, qj = kRefToBits[j / 3] * 3 + j % 3;
+        mb[qi][qj] = me[i * 24 + j];
+      }
+    // Me = M8 (x) I3 with M8 diagonal in the corner-Hadamard basis: D = H M8 H^T, M8 = H^T D H / 64
+    double big = 0.0;
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) big = std::max(big, std::fabs(mb[i][j]));
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        const double expect = (i % 3 == j % 3) ? mb[3 * (i / 3)][3 * (j / 3)] : 0.0;
+        if (std::fabs(mb[i][j] - expect) > 1e-12 * big)
+          throw std::invalid_argument("element mass is not the same scalar matrix for each component");
+      }
+    double d[8][8];
+    for (int m = 0; m < 8; ++m)
+      for (int n = 0; n < 8; ++n) {
+        double acc = 0.0;
+        for (int q = 0; q < 8; ++q)
+          for (int r = 0; r < 8; ++r) {
+            const double hm = (__builtin_popcount(m & q) & 1) ? -1.0 : 1.0, hn = (__builtin_popcount(n & r) & 1) ? -1.0 : 1.0;
+            acc += hm * mb[3 * q][3 * r] * hn;
+          }
+        d[m][n] = acc;
+      }
+    for (int m = 0; m < 8; ++m)
+      for (int n = 0; n < 8; ++n)
+        if (m != n && std::fabs(d[m][n]) > 1e-10 * 64 * big)
+          throw std::invalid_argument("element mass lacks the mirror symmetry of an axis-aligned brick");
+    for (int m = 0; m < 8; ++m) h.mdiag[m] = d[m][m] / 64.0;
+    h.have_mass = true;
+  })
+}
+
+int dfmi_mass_apply(dfmi_mg* mg, int32_t slot, const double* mass, const double* lumped, const double* x, double* y,
+                    int32_t k) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (!h.have_mass) throw std::runtime_error("dfmi_set_mass must precede mass_apply");
+    if (k < 1 || !mass || !x || !y) throw std::invalid_argument("mass_apply needs k >= 1 and buffers");
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    ensure_workspace(&h, k);
+    hipStream_t s = h.stream;
+    h.mass_pad.ensure(sizeof(double) * h.pad.elems());
+    DevBuf f32;
+    f32.ensure(sizeof(float) * h.pad.elems());
+    pad_moduli(h.pad, h.ux, h.uy, h.uz, mass, h.mass_pad.as<double>(), f32.as<float>(), s);
+    const double* lump = nullptr;
+    if (lumped) {
+      flat_to_grid(h.pad, h.ux, h.uy, h.uz, lumped, h.tmp.as<double>(), 1, s);
+      lump = h.tmp.as<double>();
+    }
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, x, h.b.as<double>(), k, s);
+    ApplyArgs a{};
+    a.k = k;
+    a.x = h.b.as<double>();
+    a.y = h.r.as<double>();
+    a.dinv = lump;
+    for (int m = 0; m < 8; ++m) a.mdiag[m] = h.mdiag[m];
+    fine_apply<double>(h.pad, h.mass_pad.as<double>(), h.slots[slot].levels[0]->mask.as<uint8_t>(), h.kb, MODE_MASS, a,
+                       s, nullptr);
+    grid_to_flat(h.pad, h.ux, h.uy, h.uz, h.r.as<double>(), y, k, s);
+    sync(&h);
+  })
+}
+
+int dfmi_mask(dfmi_mg* mg, int32_t slot, double* x, int32_t k) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (k < 1 || !x) throw std::invalid_argument("mask needs k >= 1 and a buffer");
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    ensure_workspace(&h, k);
+    hipStream_t s = h.stream;
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, x, h.b.as<double>(), k, s);
+    mask_vector(h.slots[slot].levels[0]->mask.as<uint8_t>(), h.b.as<double>(), h.pad.nodes(), k, s);
+    grid_to_flat(h.pad, h.ux, h.uy, h.uz, h.b.as<double>(), x, k, s);
+    sync(&h);
+  })
+}
+
 int dfmi_element_energy(dfmi_mg* mg, const double* u, int32_t k, double* energy) {
   DFMI_GUARD({
     Handle& h = mg->h;
@@ -1276,6 +1366,21 @@ int dfmi_compliance_sensitivity(dfmi_mg* mg, const double* u, const double* f, i
     fine_element_energy(h.pad, h.ux, h.uy, h.uz, h.b.as<double>(), h.kb, energy.as<double>(), k, h.stream);
     compliance_sensitivity(u, f, ndof, k, energy.as<double>(), ne, dmoduli, factor, compliance, sensitivity,
                            h.partial.as<double>(), h.stream);
+    sync(&h);
+  })
+}
+
+int dfmi_modal_sensitivity(dfmi_mg* mg, const double* u, int32_t k, const double* weight, const double* value,
+                           const double* stiffness_slope, const double* mass_slope, double target, double* out) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (!h.have_mass) throw std::runtime_error("dfmi_set_mass must precede modal_sensitivity");
+    if (!u || !weight || !value || !stiffness_slope || !mass_slope || !out) throw std::invalid_argument("null buffer");
+    if (!(target > 0)) throw std::invalid_argument("target must be positive");
+    ensure_workspace(&h, k);
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, u, h.b.as<double>(), k, h.stream);
+    fine_modal_sensitivity(h.pad, h.ux, h.uy, h.uz, h.b.as<double>(), k, h.kb, h.mdiag, weight, value,
+                           stiffness_slope, mass_slope, target, out, h.stream);
     sync(&h);
   })
 }

@@ -2,7 +2,8 @@
 
 Mirrors ``deep_frame/topology_multigrid.py`` ``GeometricMultigrid``: constructor
 (:47-79), ``update`` (:97-100), ``hierarchy`` (:145-208), ``solve`` (:336-375),
-``product`` (:455-457), ``element_energy`` (:458-459), ``diagnostics`` (:541-542),
+``product`` (:455-457), ``element_energy`` (:458-459), ``eigenpairs`` (:494-540, in ``modal``),
+``diagnostics`` (:541-542),
 ``close`` (:543-548) and the ``statistics`` list.  PyTorch only carries device
 memory and the current stream; all arithmetic runs in libdfmi.
 """
@@ -18,7 +19,9 @@ MULTIGRID = {"coarse": "galerkin", "coarsest_dofs": 80000, "max_levels": 8, "smo
              "damping": 1.0, "chebyshev_degree": 3, "chebyshev_ratio": 20.0, "growth": 2.0, "cycle": "V",
              "power_iterations": 20, "precision": "float32", "tolerance": 1e-8, "max_iterations": 2000,
              "batch": 64, "projection": True, "dead_ratio": 1e-12, "zero_rhs": 1e-12, "levels": 0, "seed": 0,
-             "refine": False, "inner_tolerance": 1e-2, "device": "cuda", "fine_smoother": "vector", "retries": 2}
+             "refine": False, "inner_tolerance": 1e-2, "device": "cuda", "fine_smoother": "vector", "retries": 2,
+             "eigen_tolerance": 1e-8, "eigen_iterations": 300, "eigen_guard": 2, "eigen_refresh": 10,
+             "eigen_drop": 1e-12, "eigen_seed": 0}
 PRECISIONS = {"float64": 0, "float32": 1}
 # operator precisions of dfmi_apply: "bf16x3" is the V-cycle's fine-level matrix-core kernel (FP32 vectors,
 # three split-bf16 products per element block)
@@ -61,6 +64,10 @@ class GeometricMultigrid:
         self.moduli = None
         self.hierarchies, self.statistics = {}, []
         self._fixed_cache = {}
+        self.me = None
+        if me is not None:
+            self.me = np.ascontiguousarray(np.asarray(me, dtype=float).reshape(24, 24))
+            _native.call("dfmi_set_mass", self.handle, self.me.ctypes.data_as(ctypes.POINTER(c_double)))
 
     def _settings(self):
         s = self.settings
@@ -171,6 +178,27 @@ class GeometricMultigrid:
         _native.call("dfmi_apply", self.handle, slot, APPLY_PRECISIONS[precision], _native.ptr(x), _native.ptr(y),
                      x.shape[0])
         return y
+
+    def mass_device(self, key, mass, lumped, x):
+        """M x with M = sum_e m_e Me + diag(lumped), masked by a hierarchy's fine mask (the mass operator of
+        topology_multigrid.py:491); mass (Ne,), lumped (ndof,) or None, x (k, ndof) FP64 device tensors."""
+        torch = self.torch
+        if self.me is None:
+            raise RuntimeError("GeometricMultigrid was built without the element mass matrix (me)")
+        x = x.to(device=self.device, dtype=torch.float64).contiguous()
+        mass = mass.to(device=self.device, dtype=torch.float64).contiguous()
+        lump = None if lumped is None else lumped.to(device=self.device, dtype=torch.float64).contiguous()
+        y = torch.empty_like(x)
+        self._stream()
+        _native.call("dfmi_mass_apply", self.handle, self.hierarchies[key]["slot"], _native.ptr(mass),
+                     None if lump is None else _native.ptr(lump), _native.ptr(x), _native.ptr(y), x.shape[0])
+        return y
+
+    def eigenpairs(self, key, fixed, mass, lumped, count, initial=None):
+        """Lowest eigenpairs of (K, M) by V-cycle preconditioned LOBPCG (topology_multigrid.py:494-540):
+        (eigenvalues (count,) host, vectors (count + eigen_guard, ndof) device tensor, report)."""
+        from deep_frame_amd import modal
+        return modal.eigenpairs(self, key, fixed, mass, lumped, count, initial)
 
     def tune_fine(self, key, k, tile=None):
         """Run the matrix-core fine-kernel autotuner for k columns, or set ``tile`` = (rows, x chunk) for k;

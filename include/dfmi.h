@@ -93,6 +93,29 @@ int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs_dev, double* x_dev, 
  * fine-level kernel); inputs and outputs stay FP64. */
 int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x_dev, double* y_dev, int32_t k);
 
+/* Unit-density element mass me (24x24 row-major FP64, reference DOF order), the `me` argument of
+ * GeometricMultigrid(shape, spacing, ke, settings, me) (topology_multigrid.py:47-79; hexahedron_matrices,
+ * topology_optimization.py:269-299).  Must be the same scalar matrix for each component. */
+int dfmi_set_mass(dfmi_mg* mg, const double* me);
+
+/* Mass operator of GeometricMultigrid.eigenpairs (topology_multigrid.py:491): y = M (sum_e m_e Me +
+ * diag(lumped)) M x with a built slot's fine mask M; mass_dev: element field (Ne), lumped_dev: flat DOF
+ * vector or NULL; x_dev, y_dev: k flat FP64 columns. */
+int dfmi_mass_apply(dfmi_mg* mg, int32_t slot, const double* mass_dev, const double* lumped_dev, const double* x_dev,
+                    double* y_dev, int32_t k);
+
+/* x *= M in place for k flat FP64 columns, M the built slot's fine mask (free DOFs of active nodes): the
+ * `grid * mask` of GeometricMultigrid.eigenpairs (topology_multigrid.py:501-503, :518). */
+int dfmi_mask(dfmi_mg* mg, int32_t slot, double* x_dev, int32_t k);
+
+/* Sensitivity of ModalConstraint.aggregate (deep_frame/topology_optimization.py:956-964) for k (1..16)
+ * flat FP64 mode shapes u_dev (M-normalised): out_dev[e] = sum_r weight[r] (stiffness_slope_dev[e] u_re^T Ke
+ * u_re - value[r] mass_slope_dev[e] u_re^T Me u_re) / target; weight and value are host arrays of k, the
+ * slopes element fields (zero on inactive elements).  Needs dfmi_set_mass. */
+int dfmi_modal_sensitivity(dfmi_mg* mg, const double* u_dev, int32_t k, const double* weight, const double* value,
+                           const double* stiffness_slope_dev, const double* mass_slope_dev, double target,
+                           double* out_dev);
+
 /* GeometricMultigrid.element_energy — topology_multigrid.py:458-459: energy_dev[r*Ne + e] = u_e^T Ke u_e. */
 int dfmi_element_energy(dfmi_mg* mg, const double* u_dev, int32_t k, double* energy_dev);
 
@@ -134,16 +157,19 @@ int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double
 /* Diagnostics of a built slot: lambda_max per smoothed level and the coarsest live DOF count. */
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs);
 
-/* InterfaceCovariance.matrix (deep_frame/topology_problem.py:639-646): out_dev[i * kb + j] =
+/* InterfaceCovariance.matrix (deep_frame/topology_problem.py:639-646) and the Gram matrices of
+ * GeometricMultigrid.eigenpairs (topology_multigrid.py:462-500): out_dev[i * kb + j] =
  * sum_n a_dev[i * n + n'] b_dev[j * n + n'] for ka load columns and kb solution columns (1..64 each), FP64,
  * deterministic.  work_dev holds work_blocks * ka * kb doubles (per-block partial sums). */
 int dfmi_gram(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, int32_t kb, double* out_dev,
               double* work_dev, int32_t work_blocks, void* hip_stream);
 
-/* Combined displacement fields of InterfaceCovariance.measure / energy (topology_problem.py:647-652, 675-684):
- * y_dev[m * n + i] = sum_r u_dev[r * n + i] m_dev[r * mcols + m] for k (1..64) columns, FP64. */
+/* Combined displacement fields of InterfaceCovariance.measure / energy (topology_problem.py:647-652, 675-684)
+ * and the block combinations of GeometricMultigrid.eigenpairs (topology_multigrid.py:476-480):
+ * y_dev[m * n + i] (+ y_dev[m * n + i] if accumulate) = sum_r u_dev[r * n + i] m_dev[r * mcols + m] for k
+ * (1..64) columns, FP64; y must not overlap u. */
 int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev, int32_t mcols, double* y_dev,
-                 void* hip_stream);
+                 int32_t accumulate, void* hip_stream);
 
 /* SIMP interpolation E = E0 (eps + (1 - eps) rho^p), dE — topology_optimization.py:657-661;
  * inactive cells (active_dev[e] == 0) get 0. */

@@ -9,7 +9,7 @@ import pytest
 from scipy.sparse.linalg import spsolve
 
 from deep_frame_amd import element, problems
-from oracle import covariance, filters, gmg, hex8, loads, mma, oc, system
+from oracle import covariance, filters, gmg, hex8, loads, mma, modal, oc, system
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -53,6 +53,55 @@ def test_euler_bernoulli_known_answer():
     u, res = system.solve(dofs, conn, ke, mod, problems.clamp_dofs(fixn), f, len(pts))
     assert res[0] < 1e-10
     assert -u[3 * tip + 2, 0].mean() == pytest.approx(gold["tip_deflection_mm"], abs=5e-8)
+
+
+def test_first_frequency_known_answer():
+    """Documented known answer (docs/topology_optimization.md, Numerische Pruefungen): first eigenfrequency
+    867.609 Hz of the clamped 48 x 6 x 6 mm beam, 24 x 4 x 4 cells, E = 4430 MPa, nu = 0.3, 1.09 g/cm3
+    (tests/test_topology_optimization.py:56-68)."""
+    shape, h = (24, 4, 4), (2.0, 1.5, 1.5)
+    ke, me, _ = hex8.hex8(h, hex8.isotropic_d(0.3))
+    pts, conn, dofs = hex8.grid(shape, h)
+    fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    ne = int(np.prod(shape))
+    values, vectors = modal.eigenpairs(dofs, conn, ke, me, np.full(ne, 4430.0), np.full(ne, 1.09e-9), fixed,
+                                       len(pts), 3)
+    assert modal.frequencies(values)[0] == pytest.approx(867.609, abs=6e-4)
+    mass = hex8.assemble(dofs, me, np.full(ne, 1.09e-9), 3 * len(pts))
+    assert np.allclose(vectors.T @ (mass @ vectors), np.eye(3), atol=1e-9)
+
+
+def _modal_case(density, lumped_tip=0.2e-6):
+    shape, h = (8, 2, 3), (1.0, 1.0, 1.0)
+    ke, me, _ = hex8.hex8(h, hex8.isotropic_d(0.3))
+    pts, conn, dofs = hex8.grid(shape, h)
+    fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    tip = np.flatnonzero(np.abs(pts[:, 0] - shape[0]) < 1e-9)
+    lumped = np.zeros(3 * len(pts))
+    lumped[problems.clamp_dofs(tip)] = lumped_tip / len(tip)
+    young, rho_g, cutoff = 2000.0, 1.2, 0.1
+    moduli = young * (1e-6 + (1 - 1e-6) * density ** 3)
+    mass = rho_g * 1e-9 * modal.interpolation(density, cutoff)[0]
+    values, vectors = modal.eigenpairs(dofs, conn, ke, me, moduli, mass, fixed, len(pts), 3, lumped)
+    target = (2 * np.pi * 2000.0) ** 2
+    modes = [(values[i], vectors[:, i]) for i in range(2)]
+    return modal.aggregate(dofs, ke, me, modes, density, young, rho_g, target, 40.0, cutoff)
+
+
+def test_modal_aggregate_gradient_matches_finite_difference():
+    """Eigenvalue sensitivity v^T (K' - lambda M') v of M-normalised modes through the KS soft minimum,
+    against central differences of eigsh eigenvalues, on both branches of the mass interpolation
+    (as tests/test_topology_optimization.py:617-629)."""
+    rng = np.random.default_rng(4)
+    density = rng.uniform(0.3, 1.0, 8 * 2 * 3)
+    density[5] = 0.06  # below the mass cutoff: rho^6 / c^5 branch
+    violation, gradient, f1, ratio = _modal_case(density)
+    assert violation == pytest.approx(1 - ratio) and f1 > 0
+    for e in (0, 5, 17, 40):
+        step = np.zeros_like(density)
+        step[e] = 1e-6
+        fd = (_modal_case(density + step)[0] - _modal_case(density - step)[0]) / 2e-6
+        assert fd == pytest.approx(gradient[e], rel=2e-5, abs=1e-9 * np.abs(gradient).max())
 
 
 def test_covariance_cantilever_golden_flexibility():
