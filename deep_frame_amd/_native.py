@@ -41,6 +41,9 @@ SIGNATURES = {
     "dfmi_set_mass": ([c_void_p, POINTER(c_double)], c_int32),
     "dfmi_mass_apply": ([c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32], c_int32),
     "dfmi_mask": ([c_void_p, c_int32, c_void_p, c_int32], c_int32),
+    "dfmi_build_hierarchy_kernel": ([c_void_p, c_int32, c_void_p, POINTER(c_double), c_int32, POINTER(c_double)],
+                                    c_int32),
+    "dfmi_kernel_component": ([c_void_p, c_int32, c_void_p, c_int32, POINTER(c_int32), POINTER(c_double)], c_int32),
     "dfmi_modal_sensitivity": ([c_void_p, c_void_p, c_int32, POINTER(c_double), POINTER(c_double), c_void_p, c_void_p,
                                 c_double, c_void_p], c_int32),
     "dfmi_element_energy": ([c_void_p, c_void_p, c_int32, c_void_p], c_int32),

@@ -63,13 +63,14 @@ __global__ __launch_bounds__(GRAM_NT) void k_gram(const double* __restrict__ a, 
     }
 }
 
-// out[e] = sum over blocks of partial[block][e], in block order
-__global__ void k_sum_blocks(const double* __restrict__ partial, int nblocks, int count, double* __restrict__ out) {
+// out[e] = scale * (sum over blocks of partial[block][e], in block order)
+__global__ void k_sum_blocks(const double* __restrict__ partial, int nblocks, int count, double scale,
+                             double* __restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= count) return;
   double s = 0.0;
   for (int blk = 0; blk < nblocks; ++blk) s += partial[(long long)blk * count + e];
-  out[e] = s;
+  out[e] = scale * s;
 }
 
 constexpr int COMB_M = 16;  // output columns per pass
@@ -103,6 +104,32 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ u, l
   }
 }
 
+void gram_device(const double* a, const double* b, long long n, int ka, int kb, double* out, double* work,
+                 int work_blocks, double scale, hipStream_t s) {
+  if (n < 1 || ka < 1 || kb < 1 || ka > GRAM_MAXK || kb > GRAM_MAXK)
+    throw std::invalid_argument("gram: need n >= 1 and 1 <= ka, kb <= 64");
+  if (((ka + GRAM_T - 1) / GRAM_T) * ((kb + GRAM_T - 1) / GRAM_T) > GRAM_NT)
+    throw std::invalid_argument("gram: too many output tiles");
+  if (work_blocks < 1) throw std::invalid_argument("gram: work_blocks must be >= 1");
+  const long long chunks = (n + GRAM_CH - 1) / GRAM_CH;
+  const int blocks = (int)std::min<long long>(chunks, work_blocks);
+  hipLaunchKernelGGL(k_gram, dim3(blocks), dim3(GRAM_NT), 0, s, a, b, n, ka, kb, work);
+  DFMI_HIP(hipGetLastError());
+  const int count = ka * kb;
+  hipLaunchKernelGGL(k_sum_blocks, dim3((count + 255) / 256), dim3(256), 0, s, work, blocks, count, scale, out);
+  DFMI_HIP(hipGetLastError());
+}
+
+void combine_device(const double* u, long long n, int k, const double* m, int mcols, double* y, int accumulate,
+                    hipStream_t s) {
+  if (n < 1 || k < 1 || k > GRAM_MAXK || mcols < 1) throw std::invalid_argument("combine: need n >= 1, 1 <= k <= 64");
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
+  for (int m0 = 0; m0 < mcols; m0 += COMB_M) {
+    hipLaunchKernelGGL(k_combine, dim3(blocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate);
+    DFMI_HIP(hipGetLastError());
+  }
+}
+
 }  // namespace dfmi
 
 using namespace dfmi;
@@ -123,35 +150,13 @@ extern "C" {
 
 int dfmi_gram(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, int32_t kb, double* out_dev,
               double* work_dev, int32_t work_blocks, void* hip_stream) {
-  DFMI_GUARD3({
-    if (n < 1 || ka < 1 || kb < 1 || ka > GRAM_MAXK || kb > GRAM_MAXK)
-      throw std::invalid_argument("gram: need n >= 1 and 1 <= ka, kb <= 64");
-    if (((ka + GRAM_T - 1) / GRAM_T) * ((kb + GRAM_T - 1) / GRAM_T) > GRAM_NT)
-      throw std::invalid_argument("gram: too many output tiles");
-    if (work_blocks < 1) throw std::invalid_argument("gram: work_blocks must be >= 1");
-    hipStream_t s = (hipStream_t)hip_stream;
-    const long long chunks = (n + GRAM_CH - 1) / GRAM_CH;
-    const int blocks = (int)std::min<long long>(chunks, work_blocks);
-    hipLaunchKernelGGL(k_gram, dim3(blocks), dim3(GRAM_NT), 0, s, a_dev, b_dev, (long long)n, ka, kb, work_dev);
-    DFMI_HIP(hipGetLastError());
-    const int count = ka * kb;
-    hipLaunchKernelGGL(k_sum_blocks, dim3((count + 255) / 256), dim3(256), 0, s, work_dev, blocks, count, out_dev);
-    DFMI_HIP(hipGetLastError());
-  })
+  DFMI_GUARD3({ gram_device(a_dev, b_dev, (long long)n, ka, kb, out_dev, work_dev, work_blocks, 1.0,
+                            (hipStream_t)hip_stream); })
 }
 
 int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev, int32_t mcols, double* y_dev,
                  int32_t accumulate, void* hip_stream) {
-  DFMI_GUARD3({
-    if (n < 1 || k < 1 || k > GRAM_MAXK || mcols < 1) throw std::invalid_argument("combine: need n >= 1, 1 <= k <= 64");
-    hipStream_t s = (hipStream_t)hip_stream;
-    const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
-    for (int m0 = 0; m0 < mcols; m0 += COMB_M) {
-      hipLaunchKernelGGL(k_combine, dim3(blocks), dim3(256), 0, s, u_dev, (long long)n, k, m_dev, mcols, m0, y_dev,
-                         accumulate);
-      DFMI_HIP(hipGetLastError());
-    }
-  })
+  DFMI_GUARD3({ combine_device(u_dev, (long long)n, k, m_dev, mcols, y_dev, accumulate, (hipStream_t)hip_stream); })
 }
 
 }  // extern "C"

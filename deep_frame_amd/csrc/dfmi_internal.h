@@ -206,6 +206,23 @@ template <typename T>
 void to_work(const double* r, const double* scale, T* w, long long n, int k, const int* active, hipStream_t s);
 template <typename T>
 void from_work(const T* w, const double* scale, double* z, long long n, int k, const int* active, hipStream_t s);
+// out = scale * A B^T for row blocks A (ka x n), B (kb x n), 1 <= ka, kb <= 64 (partial sums per workgroup
+// in `work`, work_blocks * ka * kb doubles, summed in a fixed order)
+void gram_device(const double* a, const double* b, long long n, int ka, int kb, double* out, double* work,
+                 int work_blocks, double scale, hipStream_t s);
+// y[m] (+)= sum_r m[r][m] u[r] for k (<= 64) rows u of length n and an mcols-column device matrix m
+void combine_device(const double* u, long long n, int k, const double* m, int mcols, double* y, int accumulate,
+                    hipStream_t s);
+// rigid-body kernel of floating fixtures (rigid.hip); coef is 6 x nk row-major over the rigid modes
+// (three translations, rotations about x, y, z) about `center`
+void fine_kernel(const Geom& g, const double h[3], const double center[3], const double* coef, int nk,
+                 const uint8_t* mask, double* out, double* tmp, double* work, int work_blocks, double* small,
+                 hipStream_t s);
+void coarsest_kernel(const Geom& g, const double h[3], const double center[3], const double* coef, int nk,
+                     const std::vector<int>& live, std::vector<double>& kernel, std::vector<int>& pins);
+void pin_dense(double* a, int n, const int* pins_dev, int np, hipStream_t s);
+void project_dense_inverse(double* a, int n, const int* pins_dev, int np, const std::vector<double>& kernel, int nk,
+                           hipStream_t s);
 void flat_to_grid(const Geom& pad, int ux, int uy, int uz, const double* flat, double* grid, int k, hipStream_t s);
 void grid_to_flat(const Geom& pad, int ux, int uy, int uz, const double* grid, double* flat, int k, hipStream_t s);
 void pad_moduli(const Geom& pad, int ux, int uy, int uz, const double* moduli, double* m64, float* m32,

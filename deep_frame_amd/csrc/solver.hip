@@ -78,7 +78,13 @@ struct Hierarchy {
   int coarsest_n = 0;
   DevBuf live_dofs, ainv, work_in, work_out, work_split;
   double setup_s = 0.0;
+  // rigid-body kernel left free by the Dirichlet DOFs (floating fixtures): nk orthonormal fine-level
+  // vectors, the Gram work buffer and the nk x k projection coefficients
+  int nk = 0;
+  DevBuf kern, kwork, kcoef;
 };
+
+constexpr int KERNEL_GRAM_BLOCKS = 512;
 
 struct Handle {
   int ux = 0, uy = 0, uz = 0;
@@ -572,7 +578,7 @@ static double spectrum(Handle* h, Hierarchy& H, int idx) {
   return std::sqrt(out);
 }
 
-static void build(Handle* h, int slot, const uint8_t* fixed_dev) {
+static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* coef = nullptr, int nk = 0) {
   if (!h->have_moduli) throw std::runtime_error("update(moduli) must precede hierarchy construction");
   auto t0 = std::chrono::steady_clock::now();
   Hierarchy& H = h->slots[slot];
@@ -597,6 +603,19 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev) {
   const long long n0 = F.g.nodes() * 3;
   hipLaunchKernelGGL(k_inverse_diag, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, diag.as<double>(),
                      F.mask.as<uint8_t>(), F.dinv64.as<double>(), F.dinv32.as<float>(), F.g.nodes());
+  // rigid kernel of a floating fixture: masked modes about the centre of the design grid, orthonormal
+  const double center[3] = {0.5 * h->ux * h->spacing[0], 0.5 * h->uy * h->spacing[1], 0.5 * h->uz * h->spacing[2]};
+  if (nk > 0) {
+    if (nk > 6 || !coef) throw std::invalid_argument("rigid kernel needs 1 to 6 coefficient columns");
+    H.nk = nk;
+    H.kern.ensure(sizeof(double) * (size_t)nk * n0);
+    H.kwork.ensure(sizeof(double) * (size_t)KERNEL_GRAM_BLOCKS * nk * 64);
+    H.kcoef.ensure(sizeof(double) * (size_t)nk * 64);
+    DevBuf tmp;
+    tmp.ensure(sizeof(double) * (size_t)nk * n0);
+    fine_kernel(F.g, h->spacing, center, coef, nk, F.mask.as<uint8_t>(), H.kern.as<double>(), tmp.as<double>(),
+                H.kwork.as<double>(), KERNEL_GRAM_BLOCKS, H.kcoef.as<double>(), s);
+  }
   // Galerkin levels
   DevBuf fst;
   fst.ensure(sizeof(double) * 243 * (size_t)F.g.nodes());
@@ -650,7 +669,23 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev) {
     const double* cst = h->nlev > 1 ? C.st64.as<double>() : nullptr;
     if (!cst) throw std::runtime_error("multigrid needs at least two levels");
     coarsest_assemble(C.g, cst, idx.as<int>(), n, H.ainv.as<double>(), s);
-    dense_sweep_inverse(H.ainv.as<double>(), n, s);
+    if (nk > 0) {
+      // floating: pin nk DOFs, invert the rest, and fold the kernel projection into the inverse
+      // (topology_multigrid.py:233-246, 267-283)
+      const double scale = (double)(1 << (h->nlev - 1));
+      const double hc[3] = {h->spacing[0] * scale, h->spacing[1] * scale, h->spacing[2] * scale};
+      std::vector<double> kernel;
+      std::vector<int> pins;
+      coarsest_kernel(C.g, hc, center, coef, nk, live, kernel, pins);
+      DevBuf pd;
+      pd.ensure(sizeof(int) * pins.size());
+      DFMI_HIP(hipMemcpyAsync(pd.p, pins.data(), sizeof(int) * pins.size(), hipMemcpyHostToDevice, s));
+      pin_dense(H.ainv.as<double>(), n, pd.as<int>(), (int)pins.size(), s);
+      dense_sweep_inverse(H.ainv.as<double>(), n, s);
+      project_dense_inverse(H.ainv.as<double>(), n, pd.as<int>(), (int)pins.size(), kernel, nk, s);
+    } else {
+      dense_sweep_inverse(H.ainv.as<double>(), n, s);
+    }
     sync(h);
   }
   H.built = true;
@@ -658,6 +693,20 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev) {
 }
 
 // ---- PCG ----------------------------------------------------------------------------------
+// x -= B (B^T x) for k fine-level columns, B the hierarchy's orthonormal rigid kernel
+// (topology_multigrid.py:224-228)
+static void remove_kernel(Handle* h, Hierarchy& H, double* x, int k) {
+  if (H.nk == 0) return;
+  const long long n = col(h->pad);
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int kc = std::min(64, k - c0);
+    double* xc = x + (size_t)c0 * n;
+    gram_device(H.kern.as<double>(), xc, n, H.nk, kc, H.kcoef.as<double>(), H.kwork.as<double>(), KERNEL_GRAM_BLOCKS,
+                -1.0, h->stream);
+    combine_device(H.kern.as<double>(), n, H.nk, H.kcoef.as<double>(), kc, xc, 1, h->stream);
+  }
+}
+
 template <typename T>
 static void precondition(Handle* h, Hierarchy& H, const double* r, double* z, int k, const int* active) {
   hipStream_t s = h->stream;
@@ -685,6 +734,7 @@ static void precond_any(Handle* h, Hierarchy& H, const double* r, double* z, int
     precondition<float>(h, H, r, z, k, active);
   else
     precondition<double>(h, H, r, z, k, active);
+  remove_kernel(h, H, z, k);
 }
 
 static void fine_fp64(Handle* h, Hierarchy& H, int mode, const double* x, double* y, const double* b, int k,
@@ -741,6 +791,7 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
 
   flat_to_grid(h->pad, h->ux, h->uy, h->uz, rhs_flat, b, k, s);
   mask_vector(F.mask.as<uint8_t>(), b, h->pad.nodes(), k, s);
+  remove_kernel(h, H, b, k);
   std::vector<double> nb;
   column_norms(h, b, n, k, nullptr, nb);
   std::vector<int> active(k);
@@ -755,6 +806,7 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
     flat_to_grid(h->pad, h->ux, h->uy, h->uz, x_flat, x, k, s);
     mask_vector(F.mask.as<uint8_t>(), x, h->pad.nodes(), k, s);
     fine_fp64(h, H, MODE_RESID, x, r, b, k, nullptr, nullptr, nullptr);
+    remove_kernel(h, H, r, k);
   } else {
     fill_zero<double>(x, (long long)k * n, s);
     copy_columns<double>(b, r, n, k, nullptr, s);
@@ -793,6 +845,7 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
       // true residual confirmation (topology_multigrid.py:431-437)
       DFMI_HIP(hipMemcpyAsync(act, check.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
       fine_fp64(h, H, MODE_RESID, x, tmp, b, k, act, nullptr, nullptr);
+      remove_kernel(h, H, tmp, k);
       std::vector<double> tn;
       column_norms(h, tmp, n, k, act, tn);
       std::vector<int> replace(k, 0);
@@ -824,6 +877,7 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
   }
   if (any()) {
     fine_fp64(h, H, MODE_RESID, x, tmp, b, k, act, nullptr, nullptr);
+    remove_kernel(h, H, tmp, k);
     std::vector<double> tn;
     column_norms(h, tmp, n, k, act, tn);
     for (int c = 0; c < k; ++c)
@@ -1145,6 +1199,47 @@ int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed, double
   })
 }
 
+int dfmi_build_hierarchy_kernel(dfmi_mg* mg, int32_t slot, const uint8_t* fixed, const double* coefficients,
+                                int32_t nk, double* setup_seconds) {
+  DFMI_GUARD({
+    if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
+    if (nk < 0 || nk > 6 || (nk > 0 && !coefficients)) throw std::invalid_argument("nk must be in [0, 6]");
+    build(&mg->h, slot, fixed, coefficients, nk);
+    if (setup_seconds) *setup_seconds = mg->h.slots[slot].setup_s;
+  })
+}
+
+int dfmi_kernel_component(dfmi_mg* mg, int32_t slot, const double* rhs, int32_t k, int32_t* nk, double* component) {
+  DFMI_GUARD({
+    Handle& h = mg->h;
+    if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
+    Hierarchy& H = h.slots[slot];
+    if (nk) *nk = H.nk;
+    if (!rhs || k < 1 || !component) return 0;
+    ensure_workspace(&h, k);
+    const long long n = col(h.pad);
+    flat_to_grid(h.pad, h.ux, h.uy, h.uz, rhs, h.b.as<double>(), k, h.stream);
+    mask_vector(H.levels[0]->mask.as<uint8_t>(), h.b.as<double>(), h.pad.nodes(), k, h.stream);
+    std::vector<double> nb;
+    column_norms(&h, h.b.as<double>(), n, k, nullptr, nb);
+    for (int c = 0; c < k; ++c) component[c] = 0.0;
+    if (H.nk == 0) return 0;
+    for (int c0 = 0; c0 < k; c0 += 64) {
+      const int kc = std::min(64, k - c0);
+      gram_device(H.kern.as<double>(), h.b.as<double>() + (size_t)c0 * n, n, H.nk, kc, H.kcoef.as<double>(),
+                  H.kwork.as<double>(), KERNEL_GRAM_BLOCKS, 1.0, h.stream);
+      std::vector<double> cf((size_t)H.nk * kc);
+      DFMI_HIP(hipMemcpyAsync(cf.data(), H.kcoef.p, sizeof(double) * cf.size(), hipMemcpyDeviceToHost, h.stream));
+      sync(&h);
+      for (int c = 0; c < kc; ++c) {
+        double sq = 0.0;
+        for (int a = 0; a < H.nk; ++a) sq += cf[(size_t)a * kc + c] * cf[(size_t)a * kc + c];
+        component[c0 + c] = std::sqrt(sq) / std::max(nb[c0 + c], 1e-300);
+      }
+    }
+  })
+}
+
 // Retry of failed columns in FP64 (topology_multigrid.py:376-406, _retried / _precise): up to `retries`
 // times, the columns whose relative residual is not below the tolerance are solved again by flexible
 // PCG with the FP64 V-cycle, from their previous iterate when its relative residual is below 1 and from
@@ -1200,7 +1295,7 @@ int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t 
   DFMI_GUARD({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     if (k < 1 || !rhs || !x || !report) throw std::invalid_argument("solve needs k >= 1 and buffers");
-    if (mg->h.st.refine)
+    if (mg->h.st.refine && mg->h.slots[slot].nk == 0)
       solve_refine(&mg->h, slot, rhs, x, k, report);
     else
       solve(&mg->h, slot, rhs, x, k, report);

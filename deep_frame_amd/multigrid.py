@@ -18,7 +18,7 @@ from deep_frame_amd import _native
 MULTIGRID = {"coarse": "galerkin", "coarsest_dofs": 80000, "max_levels": 8, "smoother": "chebyshev", "sweeps": 2,
              "damping": 1.0, "chebyshev_degree": 3, "chebyshev_ratio": 20.0, "growth": 2.0, "cycle": "V",
              "power_iterations": 20, "precision": "float32", "tolerance": 1e-8, "max_iterations": 2000,
-             "batch": 64, "projection": True, "dead_ratio": 1e-12, "zero_rhs": 1e-12, "levels": 0, "seed": 0,
+             "batch": 64, "projection": True, "coarsest_projection": True, "dead_ratio": 1e-12, "zero_rhs": 1e-12, "levels": 0, "seed": 0,
              "refine": False, "inner_tolerance": 1e-2, "device": "cuda", "fine_smoother": "vector", "retries": 2,
              "eigen_tolerance": 1e-8, "eigen_iterations": 300, "eigen_guard": 2, "eigen_refresh": 10,
              "eigen_drop": 1e-12, "eigen_seed": 0}
@@ -36,6 +36,20 @@ def _torch():
     return torch
 
 
+def rigid_modes(points, center):
+    """The six rigid modes at node positions (n, 3) about ``center`` -> (6, 3n): translations x, y, z and
+    rotations (0, -z, y), (z, 0, -x), (-y, x, 0) (topology_multigrid.py:29-38)."""
+    d = np.asarray(points, float) - np.asarray(center, float)
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    modes = np.zeros((6, len(d), 3))
+    for axis in range(3):
+        modes[axis, :, axis] = 1.0
+    modes[3, :, 1], modes[3, :, 2] = -z, y
+    modes[4, :, 0], modes[4, :, 2] = z, -x
+    modes[5, :, 0], modes[5, :, 1] = -y, x
+    return modes.reshape(6, -1)
+
+
 class GeometricMultigrid:
     def __init__(self, shape, spacing, ke, settings=None, me=None):
         torch = self.torch = _torch()
@@ -51,6 +65,7 @@ class GeometricMultigrid:
         self.spacing = np.asarray(spacing, dtype=float)
         self.ke64 = np.ascontiguousarray(np.asarray(ke, dtype=float).reshape(24, 24))
         self.nodes = tuple(n + 1 for n in self.shape)
+        self.center = np.asarray(self.shape) * self.spacing / 2
         self.ndof = 3 * int(np.prod(self.nodes))
         self.nelem = int(np.prod(self.shape))
         self.native = self._settings()
@@ -116,16 +131,62 @@ class GeometricMultigrid:
             raise ValueError("fixed DOF index out of range")
         flags[fixed] = 1
         dev = self._tensor(flags, self.torch.uint8)
+        coefficients = self.rigid_coefficients(fixed)
+        nk = coefficients.shape[1] if self.settings["projection"] else 0
         self._stream()
         seconds = c_double()
-        _native.call("dfmi_build_hierarchy", self.handle, slot, _native.ptr(dev), byref(seconds))
+        if nk:
+            if not self.settings["coarsest_projection"]:
+                raise ValueError("a floating hierarchy needs coarsest_projection (the coarsest operator is singular)")
+            coef = np.ascontiguousarray(coefficients)
+            _native.call("dfmi_build_hierarchy_kernel", self.handle, slot, _native.ptr(dev),
+                         coef.ctypes.data_as(ctypes.POINTER(c_double)), nk, byref(seconds))
+        else:
+            _native.call("dfmi_build_hierarchy", self.handle, slot, _native.ptr(dev), byref(seconds))
         lam = (c_double * max(1, self.level_count - 1))()
         coarsest = c_int32()
         _native.call("dfmi_diagnostics", self.handle, slot, lam, byref(coarsest))
         entry = {"slot": slot, "setup_s": seconds.value, "lambda_max": list(lam)[:self.level_count - 1],
-                 "coarsest_dofs": coarsest.value, "fixed": fixed}
+                 "coarsest_dofs": coarsest.value, "fixed": fixed, "coefficients": coefficients, "kernel": nk}
         self.hierarchies[key] = entry
         return entry
+
+    def _points(self):
+        axes = [np.arange(n) * h for n, h in zip(self.nodes, self.spacing)]
+        return np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
+
+    def rigid_coefficients(self, fixed):
+        """Combinations (6, nk) of the rigid modes that vanish on the fixed DOFs of active nodes
+        (topology_multigrid.py:151-159): null space of their Gram matrix over those DOFs."""
+        fixed = np.asarray(fixed, dtype=np.int64)
+        nodes = np.unique(fixed // 3)
+        # a node is active when one of its (up to eight) elements has a positive modulus
+        ijk = np.stack(np.unravel_index(nodes, self.nodes), axis=1)
+        corners = np.array([(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)])
+        elems = ijk[:, None, :] - corners[None]
+        inside = np.all((elems >= 0) & (elems < np.asarray(self.shape)), axis=2)
+        flat = np.ravel_multi_index(tuple(np.clip(elems, 0, np.asarray(self.shape) - 1).transpose(2, 0, 1)),
+                                    self.shape)
+        gathered = self.moduli[self._tensor(flat.ravel(), self.torch.int64)].cpu().numpy().reshape(flat.shape)
+        active_nodes = nodes[np.any(inside & (gathered > 0), axis=1)]
+        constrained = fixed[np.isin(fixed // 3, active_nodes)]
+        if not constrained.size:
+            return np.eye(6)
+        nodes = np.unique(constrained // 3)
+        modes = rigid_modes(self._points()[nodes], self.center)
+        local = np.searchsorted(nodes, constrained // 3) * 3 + constrained % 3
+        fixed_modes = modes[:, local]
+        values, vectors = np.linalg.eigh(fixed_modes @ fixed_modes.T)
+        return vectors[:, values <= 1e-10 * max(values.max(), 1.0)]
+
+    def rigid_flat(self, key):
+        """Kernel modes of a hierarchy as masked flat DOF columns (ndof, nk), host."""
+        entry = self.hierarchies[key]
+        rigid = (rigid_modes(self._points(), self.center).T @ entry["coefficients"])
+        ones = self.torch.ones((1, self.ndof), dtype=self.torch.float64, device=self.device)
+        self._stream()
+        _native.call("dfmi_mask", self.handle, entry["slot"], _native.ptr(ones), 1)
+        return rigid * ones.cpu().numpy()[0][:, None]
 
     def solve_device(self, key, fixed, rhs):
         """rhs: (k, ndof) FP64 device tensor -> (x (k, ndof) device tensor, per-column reports
@@ -141,30 +202,53 @@ class GeometricMultigrid:
         return x, [(r.iterations, r.relative_residual, bool(r.converged), r.retried) for r in reports]
 
     def solve(self, key, fixed, forces, support=None):
-        """Reference signature (topology_multigrid.py:336-375); forces (ndof, k) host FP64."""
-        if support is not None:
-            raise NotImplementedError("floating (inertia-relief) supports are not part of this build")
+        """Reference signature (topology_multigrid.py:336-375); forces (ndof, k) host FP64.
+
+        With ``support`` (and ``projection``) the fixture floats: only fixed DOFs outside the support are
+        Dirichlet, the support DOFs take the least-squares reactions that remove the forces' rigid-mode
+        components, and the solution is shifted by the rigid motion that best fits zero displacement on the
+        support (topology_multigrid.py:340-349, 368-370)."""
         started = perf_counter()
-        entry = self.hierarchy(key, fixed)
+        floating = support is not None and bool(self.settings["projection"])
+        dirichlet = np.setdiff1d(fixed, support) if floating else np.asarray(fixed)
+        key = (key, floating)
+        entry = self.hierarchy(key, dirichlet)
+        fixed = dirichlet
         forces = np.array(forces, dtype=float).reshape(self.ndof, -1)
+        if floating:
+            rigid = self.rigid_flat(key)
+            support = np.asarray(support)
+            forces[support] += np.linalg.lstsq(rigid[support].T, -(rigid.T @ forces), rcond=None)[0]
         norms = np.linalg.norm(forces, axis=0)
         live = np.flatnonzero(norms > self.settings["zero_rhs"] * max(norms.max(initial=0.0), 1e-300))
         flat = np.zeros_like(forces)
         iterations, relative = np.zeros(forces.shape[1], dtype=int), np.zeros(forces.shape[1])
         batches = retried = 0
+        consistency = []
         for start in range(0, len(live), self.settings["batch"]):
             cols = live[start:start + self.settings["batch"]]
-            x, reps = self.solve_device(key, fixed, self._tensor(forces[:, cols].T))
+            rhs = self._tensor(forces[:, cols].T)
+            if entry["kernel"]:
+                component = (c_double * len(cols))()
+                self._stream()
+                _native.call("dfmi_kernel_component", self.handle, entry["slot"], _native.ptr(rhs), len(cols),
+                             None, component)
+                consistency += list(component)
+            x, reps = self.solve_device(key, fixed, rhs)
             flat[:, cols] = x.cpu().numpy().T
             iterations[cols] = [r[0] for r in reps]
             relative[cols] = [r[1] for r in reps]
             retried += sum(r[3] for r in reps)
             batches += 1
         converged = bool(np.all(relative[live] < self.settings["tolerance"])) if len(live) else True
+        if floating:
+            shift = np.linalg.lstsq(rigid[support], flat[support], rcond=None)[0]
+            flat = flat - rigid @ shift
         report = {"iterations": iterations.tolist(), "relative_residual": relative.tolist(), "converged": converged,
                   "batches": batches, "retried_columns": int(retried), "zero_rhs_columns": int(forces.shape[1] - len(live)),
-                  "seconds": perf_counter() - started, "setup_s": entry.pop("setup_s", 0.0), "floating": False,
-                  "levels": self.level_count, "coarsest_dofs": entry["coarsest_dofs"], "kernel": 0,
+                  "seconds": perf_counter() - started, "setup_s": entry.pop("setup_s", 0.0), "floating": floating,
+                  "levels": self.level_count, "coarsest_dofs": entry["coarsest_dofs"],
+                  "kernel": int(entry["kernel"]) if floating else 0, "rhs_kernel_component": consistency or None,
                   "lambda_max": entry["lambda_max"]}
         self.statistics.append(report)
         return flat, report

@@ -35,6 +35,26 @@ def patch_nodes(shape, centre, layer, size=4):
     return element.node_index(shape)(ii.ravel(), jj.ravel(), np.full(ii.size, layer))
 
 
+def relief_support(points, candidates):
+    """Statically determinate (3-2-1) support DOFs of an inertia-relief fixture on candidate nodes
+    (HexElasticity._relief_support, deep_frame/topology_optimization.py:511-525): node a (least y, then z)
+    holds all three DOFs, node b (farthest from a) the two DOFs across a-b, node c (largest area with a-b)
+    the DOF along the plane normal."""
+    candidates = np.asarray(candidates, dtype=np.int64)
+    pts = np.asarray(points, float)[candidates]
+    a = int(np.argmin(pts[:, 1] + 1e-3 * pts[:, 2]))
+    b = int(np.argmax(np.linalg.norm(pts - pts[a], axis=1)))
+    normal = np.cross(pts - pts[a], pts[b] - pts[a])
+    c = int(np.argmax(np.linalg.norm(normal, axis=1)))
+    if np.linalg.norm(normal[c]) < 1e-9:
+        raise ValueError("Inertia-relief support requires three non-collinear nodes")
+    along, normal = np.abs(pts[b] - pts[a]), np.abs(np.cross(pts[c] - pts[a], pts[b] - pts[a]))
+    nodes = candidates[[a, b, c]]
+    dofs = [3 * nodes[0] + np.arange(3), 3 * nodes[1] + np.delete(np.arange(3), np.argmax(along)),
+            [3 * nodes[2] + np.argmax(normal)]]
+    return np.sort(np.concatenate(dofs))
+
+
 def min_norm_wrenches(points, nodes, reference):
     """(3 * len(nodes), 6): minimum-norm nodal forces of the six unit wrenches about ``reference``.
 

@@ -82,6 +82,20 @@ int dfmi_update(dfmi_mg* mg, const double* moduli_dev);
  * plays the role of the reference's cache key.  setup_seconds may be NULL. */
 int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed_dev, double* setup_seconds);
 
+/* Hierarchy of a fixture whose Dirichlet DOFs leave rigid modes free (topology_multigrid.py:151-195,
+ * 233-246): coefficients (6 x nk row-major, 1 <= nk <= 6; nk = 0 is dfmi_build_hierarchy) combine the
+ * rigid modes about the centre shape * spacing / 2 (translations x, y, z; rotations (0, -z, y), (z, 0, -x),
+ * (-y, x, 0)) into the free kernel.  The solves then project it out of right-hand sides, true residuals and
+ * V-cycle outputs, and the coarsest level pins nk DOFs (pivoted QR) and applies P A^-1 P; such slots solve
+ * with the flexible FP64 PCG (refine is ignored). */
+int dfmi_build_hierarchy_kernel(dfmi_mg* mg, int32_t slot, const uint8_t* fixed_dev, const double* coefficients,
+                                int32_t nk, double* setup_seconds);
+
+/* rhs_kernel_component of solve (topology_multigrid.py:355-357): nk of the slot, and for k flat FP64 columns
+ * (rhs_dev may be NULL) component[r] = |B^T M b_r| / |M b_r| with B the orthonormal fine kernel. */
+int dfmi_kernel_component(dfmi_mg* mg, int32_t slot, const double* rhs_dev, int32_t k, int32_t* nk,
+                          double* component);
+
 /* GeometricMultigrid.solve(key, fixed, forces) for a built slot — topology_multigrid.py:336-375
  * (batched flexible PCG :407-454, FP64 retries of failed columns :376-406).  rhs_dev, x_dev: k flat FP64
  * columns.  report: host array of k. */

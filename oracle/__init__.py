@@ -19,4 +19,8 @@ Parity status per module:
   loads     pinned (sum f = F, sum r x f = M exactly; min-norm property)
   covariance pinned (golden load_mean / load_worst of the covariance cantilever;
             central differences of the mean and KS-worst gradients)
+  modal     pinned (documented first frequency 867.609 Hz of the clamped beam;
+            central differences of the KS aggregate on both mass branches)
+  floating  pinned (3-2-1 support: equals the pinned solve of the unadjusted forces;
+            line fixture: equilibrium on the free DOFs and orthogonality to the kernel)
 """

@@ -9,7 +9,7 @@ import pytest
 from scipy.sparse.linalg import spsolve
 
 from deep_frame_amd import element, problems
-from oracle import covariance, filters, gmg, hex8, loads, mma, modal, oc, system
+from oracle import covariance, filters, floating, gmg, hex8, loads, mma, modal, oc, system
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -102,6 +102,50 @@ def test_modal_aggregate_gradient_matches_finite_difference():
         step[e] = 1e-6
         fd = (_modal_case(density + step)[0] - _modal_case(density - step)[0]) / 2e-6
         assert fd == pytest.approx(gradient[e], rel=2e-5, abs=1e-9 * np.abs(gradient).max())
+
+
+def _floating_case(shape=(6, 3, 4), seed=0):
+    h = (1.0, 0.9, 1.1)
+    ke, _, _ = hex8.hex8(h, hex8.isotropic_d(0.3))
+    pts, conn, dofs = hex8.grid(shape, h)
+    rng = np.random.default_rng(seed)
+    mod = 100.0 * (1e-6 + (1 - 1e-6) * rng.uniform(0.3, 1.0, int(np.prod(shape))) ** 3)
+    center = np.asarray(shape) * np.asarray(h) / 2
+    return h, ke, pts, conn, dofs, mod, center
+
+
+def test_floating_321_support_equals_pinned_solve():
+    """With a statically determinate support the floating solve (kernel-orthogonal solution, support
+    reactions, rigid shift) must equal the plain solve with the support DOFs clamped, for any forces."""
+    h, ke, pts, conn, dofs, mod, center = _floating_case()
+    support = problems.relief_support(pts, np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    f = np.random.default_rng(1).normal(size=(3 * len(pts), 3))
+    u, nk, adjusted = floating.solve(dofs, conn, pts, ke, mod, support, support, f, center)
+    pinned, _ = system.solve(dofs, conn, ke, mod, support, f, len(pts))
+    assert nk == 6 and len(support) == 6
+    assert np.allclose(floating.rigid_modes(pts, center).T @ adjusted, 0, atol=1e-9 * np.abs(f).sum())
+    assert np.max(np.abs(u - pinned)) < 1e-9 * np.max(np.abs(pinned))
+
+
+def test_floating_line_fixture_kernel_and_equilibrium():
+    """Dirichlet DOFs on one edge line leave the rotation about it free (nk = 1); for forces without a
+    moment about the line the solution is in equilibrium on the free DOFs and orthogonal to the kernel."""
+    h, ke, pts, conn, dofs, mod, center = _floating_case(seed=2)
+    line = np.flatnonzero((np.abs(pts[:, 1]) < 1e-9) & (np.abs(pts[:, 2]) < 1e-9))
+    fixed = problems.clamp_dofs(line)
+    coef = floating.kernel_coefficients(pts, conn, mod, fixed, center)
+    assert coef.shape == (6, 1)
+    mode = (floating.rigid_modes(pts, center) @ coef)[:, 0]
+    assert np.allclose(mode[fixed], 0, atol=1e-10) and np.abs(mode[3 * line[0] + 1]) < 1e-10
+    free = system.free_dofs(conn, mod, fixed, len(pts))
+    f = np.random.default_rng(3).normal(size=3 * len(pts))
+    f[np.setdiff1d(np.arange(len(f)), free)] = 0
+    f -= mode * (mode @ f) / (mode @ mode)
+    u, nk, _ = floating.solve(dofs, conn, pts, ke, mod, fixed, np.zeros(0, dtype=np.int64), f, center)
+    k = hex8.assemble(dofs, ke, mod, 3 * len(pts))
+    assert nk == 1
+    assert np.linalg.norm((k @ u[:, 0] - f)[free]) < 1e-10 * np.linalg.norm(f)
+    assert abs(mode @ u[:, 0]) < 1e-10 * np.linalg.norm(mode) * np.linalg.norm(u)
 
 
 def test_covariance_cantilever_golden_flexibility():
