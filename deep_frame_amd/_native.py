@@ -57,6 +57,8 @@ SIGNATURES = {
     "dfmi_diagnostics": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
     "dfmi_gram": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p], c_int32),
+    "dfmi_project": ([c_void_p, c_void_p, c_int64, c_double, POINTER(c_double), c_int32, c_int32, c_void_p, c_void_p,
+                      c_void_p], c_int32),
     "dfmi_simp": ([c_void_p, c_void_p, c_int64, c_double, c_double, c_double, c_void_p, c_void_p, c_void_p], c_int32),
     "dfmi_filter_create": ([POINTER(c_int32), POINTER(c_double), c_double, POINTER(c_uint8), POINTER(c_uint8),
                             POINTER(c_void_p)], c_int32),

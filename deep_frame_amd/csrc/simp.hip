@@ -43,6 +43,51 @@ __global__ void k_simp(const double* rho, const uint8_t* active, long long n, do
   if (dmoduli) dmoduli[i] = on ? young * (1.0 - eps) * p * pow(r, p - 1.0) : 0.0;
 }
 
+// Heaviside projection of the filtered field (DensityMap.project, topology_optimization.py:1152-1171) at
+// up to three thresholds in one pass: physical[f][i], derivative[f][i].  flags bit 0: preserve (physical
+// 1), bit 1: forbidden (physical 0), bit 2: free (derivative kept).  The robust form writes
+// sech^2 as 4 e^(-2|s|) / (1 + e^(-2|s|))^2, the single form as 1 - tanh^2, as the reference does.
+struct ProjectArgs {
+  double beta;
+  double eta[3];
+  int count;
+  int robust;
+};
+
+__global__ void k_project(long long n, const double* __restrict__ filtered, const uint8_t* __restrict__ flags,
+                          ProjectArgs pa, double* __restrict__ physical, double* __restrict__ derivative) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double f = filtered[i];
+  const uint8_t fl = flags[i];
+  const double b = pa.beta;
+  for (int q = 0; q < pa.count; ++q) {
+    const double eta = pa.eta[q];
+    double phys, der;
+    if (b != 0.0) {
+      const double tb = tanh(b * eta);
+      const double den = tb + tanh(b * (1.0 - eta));
+      const double shifted = b * (f - eta);
+      const double th = tanh(shifted);
+      phys = (tb + th) / den;
+      if (pa.robust) {
+        const double decay = exp(-2.0 * fabs(shifted));
+        der = b * 4.0 * decay / ((1.0 + decay) * (1.0 + decay)) / den;
+      } else {
+        der = b * (1.0 - th * th) / den;
+      }
+    } else {
+      phys = f;
+      der = 1.0;
+    }
+    if (fl & 1) phys = 1.0;
+    if (fl & 2) phys = 0.0;
+    if (!(fl & 4)) der = 0.0;
+    physical[(long long)q * n + i] = fmin(fmax(phys, 0.0), 1.0);
+    derivative[(long long)q * n + i] = der;
+  }
+}
+
 // sensitivity[e] = -dmod[e] * factor * sum_r energy[r][e]
 __global__ void k_sensitivity(const double* energy, long long ne, int k, const double* dmod, double factor,
                               double* sens) {
@@ -496,6 +541,26 @@ int dfmi_simp(const double* rho, const uint8_t* active, int64_t n, double young,
     if (penal < 1 || !(eps > 0 && eps < 1)) throw std::invalid_argument("Invalid SIMP interpolation settings");
     hipLaunchKernelGGL(k_simp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, rho, active,
                        (long long)n, young, penal, eps, moduli, dmoduli);
+    DFMI_HIP(hipGetLastError());
+  })
+}
+
+int dfmi_project(const double* filtered, const uint8_t* flags, int64_t n, double beta, const double* eta,
+                 int32_t count, int32_t robust, double* physical, double* derivative, void* stream) {
+  DFMI_GUARD2({
+    if (n < 1 || count < 1 || count > 3 || !eta || !filtered || !flags || !physical || !derivative)
+      throw std::invalid_argument("project needs n >= 1, 1 to 3 thresholds and buffers");
+    if (!(beta >= 0) || !std::isfinite(beta)) throw std::invalid_argument("beta must be finite and >= 0");
+    ProjectArgs pa{};
+    pa.beta = beta;
+    pa.count = count;
+    pa.robust = robust ? 1 : 0;
+    for (int q = 0; q < count; ++q) {
+      if (!(eta[q] > 0 && eta[q] < 1)) throw std::invalid_argument("projection thresholds must lie within (0, 1)");
+      pa.eta[q] = eta[q];
+    }
+    hipLaunchKernelGGL(k_project, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, (long long)n,
+                       filtered, flags, pa, physical, derivative);
     DFMI_HIP(hipGetLastError());
   })
 }

@@ -153,6 +153,10 @@ static void hex8_blocks(const double* ke_bits, KeBlocks& kb) {
       }
 }
 
+// The coarsest level is solved by an explicit dense inverse (n^2 FP64); above this many DOFs a further
+// level is added instead (the reference's sparse direct coarsest solve takes coarsest_dofs up to 80000).
+constexpr int DENSE_COARSEST_MAX = 40000;
+
 static int level_count(const int shape[3], const dfmi_settings& s, int padded[3]) {
   int count = 2;
   if (s.levels > 0) {
@@ -164,7 +168,7 @@ static int level_count(const int shape[3], const dfmi_settings& s, int padded[3]
         if (std::ceil(shape[a] / std::pow(2.0, cnt)) < 1) return false;
         coarse *= std::ceil(shape[a] / std::pow(2.0, cnt - 1)) + 1;
       }
-      return coarse * 3 > s.coarsest_dofs;
+      return coarse * 3 > std::min(s.coarsest_dofs, DENSE_COARSEST_MAX);
     };
     while (count < s.max_levels && ok(count)) ++count;
   }
@@ -657,7 +661,8 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
       live.push_back((int)i);
     }
   H.coarsest_n = (int)live.size();
-  if (H.coarsest_n > 40000) throw std::runtime_error("coarsest level too large for the dense solve; add levels");
+  if (H.coarsest_n > DENSE_COARSEST_MAX)
+    throw std::runtime_error("coarsest level too large for the dense solve; add levels");
   if (H.coarsest_n > 0) {
     DevBuf idx;
     idx.ensure(sizeof(int) * nc * 3);

@@ -190,6 +190,14 @@ int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev,
 int dfmi_simp(const double* rho_dev, const uint8_t* active_dev, int64_t n, double young, double penal, double eps,
               double* moduli_dev, double* dmoduli_dev, void* hip_stream);
 
+/* DensityMap.project / fields (deep_frame/topology_optimization.py:1145-1171): Heaviside projection of the
+ * filtered field at count (1..3) thresholds eta (host array; robust: eroded eta + delta, intermediate eta,
+ * dilated eta - delta) -> physical_dev[q * n + i] (clipped to [0, 1]) and derivative_dev[q * n + i];
+ * beta = 0 is the identity; flags_dev bit 0 preserve (physical 1), bit 1 forbidden (physical 0), bit 2
+ * free (derivative kept, else 0). */
+int dfmi_project(const double* filtered_dev, const uint8_t* flags_dev, int64_t n, double beta, const double* eta,
+                 int32_t count, int32_t robust, double* physical_dev, double* derivative_dev, void* hip_stream);
+
 /* ConeFilter / DensityMap.filtered / pullback — topology_optimization.py:1082-1092, 1135-1142,
  * 1171-1178.  allowed/preserve: uint8 per element (host). */
 int dfmi_filter_create(const int32_t shape[3], const double spacing[3], double radius, const uint8_t* allowed,

@@ -4,7 +4,8 @@ Restates ``deep_frame/topology_optimization.py``:
   * cone weights max(r - |o h|, 0) over |o_a| <= floor(r / h_a) (:1082-1092, :1094-1105),
   * ``DensityMap.filtered`` x~ = (W (x a)) a / sums, sums = (W a) a, 1 on
     forbidden cells (:1123, :1130, :1135-1142),
-  * ``DensityMap.project`` single tanh projection (:1152-1170),
+  * ``DensityMap.project`` single and robust tanh projections with the preserve / forbidden / free
+    masks (:1152-1170) and ``fields`` at the eroded / intermediate / dilated thresholds (:1114-1116, :1145-1147),
   * ``DensityMap.pullback`` (:1171-1178).
 Implemented as an explicit shifted-sum over the offset box (not a convolution
 library call).  Test infrastructure only.
@@ -66,3 +67,35 @@ def project(filtered, beta, eta=0.5):
     den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
     t = np.tanh(beta * (x - eta))
     return np.clip((np.tanh(beta * eta) + t) / den, 0, 1), beta * (1 - t ** 2) / den
+
+
+def project_masked(filtered, beta, eta, robust, preserve=None, forbidden=None, free=None):
+    """(physical, derivative) of DensityMap.project (topology_optimization.py:1152-1170): the robust form
+    writes the derivative as beta 4 e^(-2|s|) / (1 + e^(-2|s|))^2 / den, the single form as
+    beta (1 - tanh^2 s) / den; preserve -> 1, forbidden -> 0, derivative 0 off the free cells, clip."""
+    x = np.asarray(filtered, float)
+    n = x.size
+    if beta and robust:
+        den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
+        shifted = beta * (x - eta)
+        decay = np.exp(-2 * np.abs(shifted))
+        physical = (np.tanh(beta * eta) + np.tanh(shifted)) / den
+        derivative = beta * 4 * decay / (1 + decay) ** 2 / den
+    elif beta:
+        den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
+        t = np.tanh(beta * (x - eta))
+        physical, derivative = (np.tanh(beta * eta) + t) / den, beta * (1 - t ** 2) / den
+    else:
+        physical, derivative = x.copy(), np.ones(n)
+    if preserve is not None:
+        physical[np.asarray(preserve, bool)] = 1
+    if forbidden is not None:
+        physical[np.asarray(forbidden, bool)] = 0
+    if free is not None:
+        derivative[~np.asarray(free, bool)] = 0
+    return np.clip(physical, 0, 1), derivative
+
+
+def thresholds(eta, delta, robust):
+    """Projection thresholds of DensityMap (topology_optimization.py:1114-1116)."""
+    return {"eroded": eta + delta, "intermediate": eta, "dilated": eta - delta} if robust else {"intermediate": eta}

@@ -5,10 +5,14 @@ Reads (as JSON/text data, nothing is imported or executed):
       32 x 12 x 12 orthotropic cantilever (construction tools/formulation_study.py:405-428)
   deep_frame/config.py:710-716 PRINT_MATERIAL orthotropic constants (copied as numbers below)
   docs/topology_optimization.md "Numerische Pruefungen": Euler-Bernoulli tip deflection
+  exports/handoff/neural_r3_v05/density_half.npz   an optimised half-frame density (51 x 96 x 24),
+      loaded with allow_pickle=False and rewritten unchanged (key "density", float64)
 Run from the repository root with the reference mounted at /root/reference.
 """
 import json
 from pathlib import Path
+
+import numpy as np
 
 REF = Path("/root/reference")
 OUT = Path(__file__).resolve().parent
@@ -37,6 +41,10 @@ def main():
             "spacing_mm": [2.0, 1.5, 1.5], "young_mpa": 4430.0, "poisson": 0.3, "force_n": -1.0,
             "tip_deflection_mm": 0.0730389, "euler_bernoulli_mm": 0.0770504}
     (OUT / "euler_bernoulli.json").write_text(json.dumps(beam, indent=1))
+    with np.load(REF / "exports/handoff/neural_r3_v05/density_half.npz", allow_pickle=False) as data:
+        density = np.asarray(data["density"])
+    assert density.shape == (51, 96, 24) and density.dtype == np.float64
+    np.savez_compressed(OUT / "density_half_neural_r3_v05.npz", density=density)
 
 
 if __name__ == "__main__":

@@ -148,6 +148,32 @@ def test_floating_line_fixture_kernel_and_equilibrium():
     assert abs(mode @ u[:, 0]) < 1e-10 * np.linalg.norm(mode) * np.linalg.norm(u)
 
 
+def test_projection_closed_forms_and_derivative():
+    """Heaviside projection: phys(0) = 0, phys(1) = 1, phys(eta) = tanh(b eta) / den (closed forms); the
+    robust derivative equals the single form (sech^2 = 1 - tanh^2) and central differences; masks."""
+    x = np.linspace(0, 1, 101)
+    for beta, eta in ((1.0, 0.5), (8.0, 0.75), (32.0, 0.25)):
+        den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
+        for robust in (False, True):
+            p, d = filters.project_masked(x, beta, eta, robust)
+            assert p[0] == pytest.approx(0, abs=1e-15) and p[-1] == pytest.approx(1, abs=1e-15)
+            assert filters.project_masked(np.array([eta]), beta, eta, robust)[0][0] == pytest.approx(np.tanh(beta * eta) / den)
+            h = 1e-6
+            fd = (filters.project_masked(x + h, beta, eta, robust)[0] - filters.project_masked(x - h, beta, eta, robust)[0]) / (2 * h)
+            inside = (x > 1e-5) & (x < 1 - 1e-5)
+            assert np.allclose(fd[inside], d[inside], rtol=1e-6, atol=1e-9 * beta)
+        # same function; 1 - tanh^2 cancels where tanh -> 1, the robust form does not
+        assert np.allclose(filters.project_masked(x, beta, eta, True)[1], filters.project_masked(x, beta, eta, False)[1],
+                           rtol=1e-6, atol=1e-14 * beta)
+    p, d = filters.project_masked(x, 0.0, 0.5, True)
+    assert np.array_equal(p, x) and np.all(d == 1)
+    pres, forb = np.zeros(101, bool), np.zeros(101, bool)
+    pres[3], forb[7] = True, True
+    p, d = filters.project_masked(x, 4.0, 0.5, True, pres, forb, ~(pres | forb))
+    assert p[3] == 1 and p[7] == 0 and d[3] == 0 and d[7] == 0
+    assert filters.thresholds(0.5, 0.25, True) == {"eroded": 0.75, "intermediate": 0.5, "dilated": 0.25}
+
+
 def test_covariance_cantilever_golden_flexibility():
     gold = json.loads((GOLDEN / "covariance_cantilever.json").read_text())
     d = hex8.orthotropic_d(gold["orthotropic"]) / gold["orthotropic"]["e_xy_mpa"]
