@@ -4,17 +4,21 @@
 // corner offset bits) transformed per component by the 2x2x2 Hadamard matrix H,
 // Ke = H^T B H / 64 where B is block diagonal with eight 3x3 blocks (one per
 // reflection-parity class p = m xor e_comp).  Per element and right-hand side
-// that is 72 add/sub (forward butterfly) + 72 FMA (blocks) + 72 add/sub
+// that is 72 add/sub (forward butterfly) + 45 FMA (blocks) + 72 add/sub
 // (inverse butterfly) instead of 576 FMA for the dense 24x24 product, and the
 // rigid translations are annihilated exactly (their Hadamard mode is 0).
 //
 // Work decomposition: one workgroup owns a (TY-1) x (TZ-1) tile of node columns
 // in (y, z) (z contiguous in memory, so loads coalesce) and marches along x.  Each
-// step stages node plane i+1 of the operand in LDS, every thread evaluates one
-// element of layer i (the tile plus a one-element halo at the low side), the 8
-// corner results go through LDS, and each thread sums the four elements of layer
-// i and carries the four of layer i+1 in registers.  Sums are in a fixed order,
-// so results are deterministic.
+// step stages node plane i+1 of the operand in LDS and every thread evaluates one
+// element of layer i (the tile plus a one-element halo at the low side).  H is
+// factored into a (y, z) face transform and an x stage: the face transform of
+// plane i+1 is reused by the next element along x (carried in registers), and the
+// inverse x stage leaves a plane-i+1 face that is added to the next element's
+// plane-i face before one inverse face transform (48 add/sub forward where the face
+// is carried, 60 back, instead of 72 each way).  The three off-corner results
+// of the plane-i face go through LDS and each thread sums the four faces of its
+// node.  Sums are in a fixed order, so results are deterministic.
 #include <cmath>
 #include <stdexcept>
 
@@ -42,6 +46,19 @@ __device__ __forceinline__ void butterfly(T (&u)[8][3]) {
         }
       }
     }
+  }
+}
+
+// 4-point Hadamard transform over the (y, z) corners of an element face: f[m], m = dy + 2 dz.
+template <typename T>
+__device__ __forceinline__ void face_butterfly(T (&f)[4][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T a = f[0][c] + f[1][c], b = f[0][c] - f[1][c], d = f[2][c] + f[3][c], e = f[2][c] - f[3][c];
+    f[0][c] = a + d;
+    f[2][c] = a - d;
+    f[1][c] = b + e;
+    f[3][c] = b - e;
   }
 }
 
@@ -140,7 +157,7 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
   constexpr int NT = FineCfg<T>::NT;
   constexpr int MAXNP = FineCfg<T>::MAXNP;
   __shared__ T U[2][3][MAXNP];  // node planes i and i+1 of the operand (masked)
-  __shared__ T OUT[6][3][NT];   // element corner results q = 2..7
+  __shared__ T OUT[3][3][NT];   // face corner results m = 1..3 (dy, dz, dy dz) at plane i
   __shared__ double red[NT / 64];
 
   const int r = blockIdx.y;
@@ -226,7 +243,13 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     return ok ? e : T(0);
   };
 
-  T carry[3] = {T(0), T(0), T(0)};
+  // The element's 8-point Hadamard transform factors into a (y, z) face transform and an x stage.  The
+  // face transform of plane i+1 is the next element's plane-i transform, so it is computed once and
+  // carried (fc); on the way back the x stage splits w into its plane-i and plane-i+1 faces, and the
+  // plane-i+1 face (gc) is added to the next element's plane-i face before one inverse face transform.
+  T fc[4][3], gc[4][3];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) fc[m][0] = fc[m][1] = fc[m][2] = gc[m][0] = gc[m][1] = gc[m][2] = T(0);
   double dot = 0.0;
   fetch(i0 - 1);
   stage((i0 - 1) & 1);
@@ -270,17 +293,38 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     xs[0] = U[b0][0][self];
     xs[1] = U[b0][1][self];
     xs[2] = U[b0][2][self];
-    T y[8][3];
-    {  // every element is evaluated (E = 0 gives zero contributions)
+    // modes with epilogue operands in registers recompute the plane-i face instead of carrying it
+    constexpr bool carry_face = !(MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1);
+    if (!carry_face || i == i0 - 1) {  // face transform of plane i
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int buf = (q & 1) ? b1 : b0;
-        const int v = self + ((q >> 1) & 1) * (TZ + 1) + ((q >> 2) & 1);
-        y[q][0] = U[buf][0][v];
-        y[q][1] = U[buf][1][v];
-        y[q][2] = U[buf][2][v];
+      for (int m = 0; m < 4; ++m) {
+        const int v = self + (m & 1) * (TZ + 1) + (m >> 1);
+        fc[m][0] = U[b0][0][v];
+        fc[m][1] = U[b0][1][v];
+        fc[m][2] = U[b0][2][v];
       }
-      butterfly(y);
+      face_butterfly(fc);
+    }
+    T g[4][3];
+    {  // every element is evaluated (E = 0 gives zero contributions)
+      T fn[4][3];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int v = self + (m & 1) * (TZ + 1) + (m >> 1);
+        fn[m][0] = U[b1][0][v];
+        fn[m][1] = U[b1][1][v];
+        fn[m][2] = U[b1][2][v];
+      }
+      face_butterfly(fn);
+      T y[8][3];  // mode q = x + 2 m
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          y[2 * m][c] = fc[m][c] + fn[m][c];
+          y[2 * m + 1][c] = fc[m][c] - fn[m][c];
+          if (carry_face) fc[m][c] = fn[m][c];
+        }
       T w[8][3];
       if constexpr (MODE == MODE_MASS) {  // the mass is diagonal in the Hadamard basis, every mode included
 #pragma unroll
@@ -299,30 +343,29 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
         }
         sym_mul(ks, y, w);
       }
-      butterfly(w);
+      // x stage of the inverse transform: plane-i face (this element) plus the previous element's
+      // plane-i face, then one inverse face transform gives the four corner sums at plane i
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        y[q][0] = w[q][0];
-        y[q][1] = w[q][1];
-        y[q][2] = w[q][2];
-      }
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          g[m][c] = gc[m][c] + (w[2 * m][c] + w[2 * m + 1][c]);
+          gc[m][c] = w[2 * m][c] - w[2 * m + 1][c];
+        }
+      face_butterfly(g);
     }
 #pragma unroll
-    for (int q = 2; q < 8; ++q) {
-      OUT[q - 2][0][t] = y[q][0];
-      OUT[q - 2][1][t] = y[q][1];
-      OUT[q - 2][2][t] = y[q][2];
+    for (int m = 1; m < 4; ++m) {
+      OUT[m - 1][0][t] = g[m][0];
+      OUT[m - 1][1][t] = g[m][1];
+      OUT[m - 1][2][t] = g[m][2];
     }
     __syncthreads();
     if (own) {
       const int tb = t - TZ, tc = t - 1, td = t - TZ - 1;  // elements (ty-1,tz), (ty,tz-1), (ty-1,tz-1)
-      T s0[3], s1[3];
+      T s0[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        s0[c] = carry[c] + y[0][c] + OUT[0][c][tb] + OUT[2][c][tc] + OUT[4][c][td];
-        s1[c] = y[1][c] + OUT[1][c][tb] + OUT[3][c][tc] + OUT[5][c][td];
-        carry[c] = s1[c];
-      }
+      for (int c = 0; c < 3; ++c) s0[c] = g[0][c] + OUT[0][c][tb] + OUT[1][c][tc] + OUT[2][c][td];
       if (out) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {

@@ -61,11 +61,13 @@ def test_first_frequency_known_answer(torch_gpu):
     moduli, mass = np.full(ne, 4430.0), np.full(ne, 1.09e-9)
     mg = GeometricMultigrid(shape, h, ke, {"coarsest_dofs": 200}, me)
     mg.update(moduli)
-    values, vectors, report = mg.eigenpairs(b"beam", fixed, mass, None, 3)
+    # the square section pairs the bending modes (1-2, 3-4): ask for whole pairs, so that no tracked mode
+    # shares its eigenvalue with a guard vector
+    values, vectors, report = mg.eigenpairs(b"beam", fixed, mass, None, 4)
     print("lobpcg", report["iterations"], report["relative_residual"], flush=True)
     assert report["converged"] and report["levels"] > 1
     assert omodal.frequencies(values)[0] == pytest.approx(867.609, abs=6e-4)
-    exact, _ = omodal.eigenpairs(dofs, conn, ke, me, moduli, mass, fixed, len(pts), 3)
+    exact, _ = omodal.eigenpairs(dofs, conn, ke, me, moduli, mass, fixed, len(pts), 4)
     assert np.max(np.abs(values - exact) / exact) < 1e-9
     v = vectors.cpu().numpy()
     m = hex8.assemble(dofs, me, mass, 3 * len(pts))

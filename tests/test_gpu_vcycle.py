@@ -49,10 +49,12 @@ def test_vcycle_matches_oracle(shape, spacing, precision, torch_gpu):
     print(f"vcycle shape={shape} precision={precision}", flush=True)
     smoother = "matrix" if precision.endswith("matrix") else "vector"
     mg, entry, levels, fixed = case(shape, spacing, 1, precision.split("-")[0], torch_gpu, smoother)
-    r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], 2)) * levels[0]["mask"][:, None]
-    ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c]) for c in range(2)])
-    z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
-    tol = 1e-9 if precision == "float64" else 1e-4
-    for c in range(2):
-        assert np.linalg.norm(z[:, c] - ref[:, c]) < tol * np.linalg.norm(ref[:, c]), c
+    # 2 columns: one column per stencil thread; 7 columns: column pairs per stencil thread and an odd last one
+    for cols in (2, 7):
+        r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
+        ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c]) for c in range(cols)])
+        z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
+        tol = 1e-9 if precision == "float64" else 1e-4
+        for c in range(cols):
+            assert np.linalg.norm(z[:, c] - ref[:, c]) < tol * np.linalg.norm(ref[:, c]), (cols, c)
     mg.close()
