@@ -24,6 +24,10 @@ CONFIGS = {
     "cantilever": {"shape": (48, 24, 24), "rhs": 1, "levels": 3},
     # 64 seeded uneven frame variants, nx, ny in [48, 160], nz in [12, 32], 6-42 RHS each
     "variants": {"count": 64},
+    # the callers either side of the solve: covariance measures and robust projection on the 256 x 256 x 48
+    # frame, LOBPCG modes on a 91 x 170 x 48 frame (the 0.75 mm half-domain size), a floating 3-2-1 support
+    # on the 96 x 96 x 16 frame
+    "rows": {},
 }
 HBM_PEAK_GBS = 8000.0
 TRAFFIC = {1: ROOT / "profiles" / "traffic_k_fine_apply_fp32_frame_256x256x48_42rhs.txt",
@@ -100,6 +104,8 @@ def main():
     cfg = CONFIGS[args.config]
     if args.config == "variants":
         return run_variants(args, cfg)
+    if args.config == "rows":
+        return run_rows(args)
     prob = problems.frame(cfg["shape"], seed=0, rhs=cfg["rhs"]) if cfg["rhs"] > 1 else problems.cantilever(seed=0)
     ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
     mod, dmod = problems.moduli(prob["density"])
@@ -217,6 +223,128 @@ def main():
         "cpu_baseline": baseline,
     }
     print(json.dumps(out))
+
+
+def _events_ms(torch, fn, reps=5):
+    """Mean milliseconds of fn() between HIP events on the current stream (the library launches there)."""
+    fn()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def run_rows(args):
+    """One JSON line with a measurement of each caller around the solve (covariance, modal, floating, density)."""
+    import numpy as np
+    import torch
+    from deep_frame_amd import element, problems
+    from deep_frame_amd.covariance import InterfaceCovariance
+    from deep_frame_amd.density import DensityMap
+    from deep_frame_amd.modal import MultigridModal
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    dev = torch.device("cuda:0")
+    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
+    rows = {}
+
+    # interface covariance on the 42 solutions of the 256 x 256 x 48 frame
+    shape = CONFIGS["frame"]["shape"]
+    prob = problems.frame(shape, seed=0, rhs=42)
+    mod, dmod = problems.moduli(prob["density"])
+    mg = GeometricMultigrid(shape, (1, 1, 1), ke, {"precision": "float32", "refine": True, "levels": 5})
+    mg.update(torch.as_tensor(mod, device=dev))
+    f = torch.as_tensor(np.ascontiguousarray(prob["forces"].T), device=dev)
+    u, _ = mg.solve_device(b"cov", prob["fixed"], f)
+    g = np.random.default_rng(0).normal(size=(42, 42))
+    cov = InterfaceCovariance(g @ g.T / 42 + np.eye(42))
+    dm = torch.as_tensor(dmod, device=dev)
+    gram_ms = _events_ms(torch, lambda: cov.unit_flexibility(u, f))
+    limits = {"mean_n_mm": 1.0, "worst_n_mm": 1.0}
+    cov.measure(mg, u, f, dm, limits)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        cov.measure(mg, u, f, dm, limits)
+    torch.cuda.synchronize()
+    gram_bytes = 2.0 * f.numel() * 8
+    rows["covariance"] = {"workload": "InterfaceCovariance.measure, 42 unit-wrench solutions, 9,709,203 DOF",
+                          "measure_ms": (time.perf_counter() - t0) / 3 * 1e3, "gram_ms": gram_ms,
+                          "gram_roofline": {"bound": "hbm", "achieved": gram_bytes / (gram_ms * 1e-3) / 1e9,
+                                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                            "frac": gram_bytes / (gram_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    print(f"# covariance {rows['covariance']}", file=sys.stderr, flush=True)
+
+    # robust projection and pullback on the same grid
+    n = int(np.prod(shape))
+    allowed = np.ones(n, bool)
+    dmap = DensityMap(shape, (1, 1, 1), allowed, np.zeros(n, bool), ~allowed,
+                      {"projection": "robust", "projection_beta": 8.0, "filter_radius_mm": 1.5})
+    design = torch.as_tensor(prob["density"].ravel(), device=dev)
+    filtered = dmap.filtered(design)
+    etas = list(dmap.thresholds.values())
+    project_ms = _events_ms(torch, lambda: dmap._project(filtered, etas))
+    fields, _ = dmap.fields(design)
+    sens = torch.as_tensor(np.random.default_rng(1).normal(size=n), device=dev)
+    fields_ms = _events_ms(torch, lambda: dmap.fields(design))
+    pull_ms = _events_ms(torch, lambda: dmap.pullback(sens, fields["eroded"][1]))
+    project_bytes = n * (8.0 + 1.0 + 2 * 8.0 * len(etas))
+    rows["density"] = {"workload": "DensityMap robust fields (3 thresholds) and pullback, 3,145,728 cells, r = 1.5",
+                       "fields_ms": fields_ms, "pullback_ms": pull_ms, "project_ms": project_ms,
+                       "project_roofline": {"bound": "hbm", "achieved": project_bytes / (project_ms * 1e-3) / 1e9,
+                                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                            "frac": project_bytes / (project_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    dmap.close()
+    mg.close()
+    del u, f
+    print(f"# density {rows['density']}", file=sys.stderr, flush=True)
+
+    # LOBPCG modes on the 0.75 mm half-domain size
+    shape = (91, 170, 48)
+    prob = problems.frame(shape, seed=0, rhs=6)
+    rho = prob["density"].ravel()
+    mg = GeometricMultigrid(shape, (1, 1, 1), ke, {"precision": "float32"}, element.mass((1, 1, 1)))
+    modal = MultigridModal(mg, [{"key": b"modes", "fixed": prob["fixed"]}], 1.0, 1.0,
+                           settings={"modes": 6, "tracked": 3, "f1_min_hz": 1.0})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    modal.measure(rho)
+    cold = time.perf_counter() - t0
+    first = modal.reports[0]
+    t0 = time.perf_counter()
+    modal.measure(np.clip(rho * 1.01, 0, 1))
+    warm = time.perf_counter() - t0
+    rows["modal"] = {"workload": f"MultigridModal.measure, 6 modes, frame {shape[0]}x{shape[1]}x{shape[2]} "
+                                 f"({mg.ndof} DOF), cold then warm-started on a perturbed density",
+                     "cold_s": cold, "cold_iterations": first["iterations"], "warm_s": warm,
+                     "warm_iterations": modal.reports[0]["iterations"],
+                     "converged": bool(first["converged"] and modal.reports[0]["converged"])}
+    mg.close()
+    print(f"# modal {rows['modal']}", file=sys.stderr, flush=True)
+
+    # floating 3-2-1 support against the clamped pads on the 96 x 96 x 16 frame
+    shape = CONFIGS["frame_small"]["shape"]
+    prob = problems.frame(shape, seed=0, rhs=42)
+    mod, _ = problems.moduli(prob["density"])
+    points = element.node_points(shape)
+    support = problems.relief_support(points, np.flatnonzero(points[:, 2] == 0))
+    result = {}
+    for name, fixed, sup in (("clamped", prob["fixed"], None), ("floating", support, support)):
+        mg = GeometricMultigrid(shape, (1, 1, 1), ke, {"precision": "float32", "levels": 4})
+        mg.update(mod)
+        mg.solve(name, fixed, prob["forces"], sup)
+        t0 = time.perf_counter()
+        _, rep = mg.solve(name, fixed, prob["forces"], sup)
+        result[name] = {"s": time.perf_counter() - t0, "iterations": max(rep["iterations"]),
+                        "converged": rep["converged"], "kernel": rep["kernel"]}
+        mg.close()
+    rows["floating"] = {"workload": "GeometricMultigrid.solve with a 3-2-1 support vs the clamped pads, frame "
+                                    "96x96x16, 42 RHS (host arrays in and out)", **result}
+    print(f"# floating {rows['floating']}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "callers around the solve", "n_gpus": 1, "data": "synthetic", "rows": rows}))
 
 
 def run_variants(args, cfg):

@@ -13,36 +13,53 @@ constexpr int GRAM_NT = 256;
 constexpr int GRAM_CH = 64;     // DOFs staged per pass
 constexpr int GRAM_MAXK = 64;   // columns of each operand
 constexpr int GRAM_T = 4;       // each thread owns a 4 x 4 tile of output pairs
+static_assert(GRAM_NT * GRAM_T * GRAM_T <= GRAM_MAXK * (GRAM_CH + 1), "lane sums must fit the staging buffer");
 
-// Per-block partial sums of C[i][j] = sum_n A[i][n] B[j][n] over the block's DOF chunks.
+// Per-block partial sums of C[i][j] = sum_n A[i][n] B[j][n] over the block's DOF chunks.  The output is
+// cut into 4 x 4 tiles and every tile gets 256 / tiles threads, which split the chunk's DOFs between them
+// (so small Gram matrices keep all threads busy); the tile's lanes are summed through LDS in a fixed order
+// at the end.  partial is [output element][block], so the block sums read contiguous runs.
 __global__ __launch_bounds__(GRAM_NT) void k_gram(const double* __restrict__ a, const double* __restrict__ b,
                                                    long long n, int ka, int kb, double* __restrict__ partial) {
   __shared__ double As[GRAM_MAXK][GRAM_CH + 1];
   __shared__ double Bs[GRAM_MAXK][GRAM_CH + 1];
   const int t = threadIdx.x;
   const int tiles_j = (kb + GRAM_T - 1) / GRAM_T;
-  const int ti = t / tiles_j, tj = t - (t / tiles_j) * tiles_j;
-  const bool owner = ti * GRAM_T < ka;
+  const int tiles = ((ka + GRAM_T - 1) / GRAM_T) * tiles_j;
+  const int lanes = max(1, GRAM_NT / tiles);
+  const int tile = t / lanes, lane = t - (t / lanes) * lanes;
+  const bool owner = tile < tiles;
+  const int ti = owner ? tile / tiles_j : 0, tj = owner ? tile - (tile / tiles_j) * tiles_j : 0;
+  int ra[GRAM_T], rb[GRAM_T];  // operand rows of this tile, clamped into the staged rows
+#pragma unroll
+  for (int x = 0; x < GRAM_T; ++x) {
+    ra[x] = min(ti * GRAM_T + x, ka - 1);
+    rb[x] = min(tj * GRAM_T + x, kb - 1);
+  }
   double acc[GRAM_T][GRAM_T];
 #pragma unroll
   for (int x = 0; x < GRAM_T; ++x)
 #pragma unroll
     for (int y = 0; y < GRAM_T; ++y) acc[x][y] = 0.0;
   for (long long c0 = (long long)blockIdx.x * GRAM_CH; c0 < n; c0 += (long long)gridDim.x * GRAM_CH) {
-    for (int v = t; v < GRAM_MAXK * GRAM_CH; v += GRAM_NT) {
+    for (int v = t; v < ka * GRAM_CH; v += GRAM_NT) {
       const int r = v / GRAM_CH, c = v - (v / GRAM_CH) * GRAM_CH;
       const long long idx = c0 + c;
-      As[r][c] = (r < ka && idx < n) ? a[(long long)r * n + idx] : 0.0;
-      Bs[r][c] = (r < kb && idx < n) ? b[(long long)r * n + idx] : 0.0;
+      As[r][c] = idx < n ? a[(long long)r * n + idx] : 0.0;
+    }
+    for (int v = t; v < kb * GRAM_CH; v += GRAM_NT) {
+      const int r = v / GRAM_CH, c = v - (v / GRAM_CH) * GRAM_CH;
+      const long long idx = c0 + c;
+      Bs[r][c] = idx < n ? b[(long long)r * n + idx] : 0.0;
     }
     __syncthreads();
     if (owner) {
-      for (int c = 0; c < GRAM_CH; ++c) {
+      for (int c = lane; c < GRAM_CH; c += lanes) {
         double av[GRAM_T], bv[GRAM_T];
 #pragma unroll
         for (int x = 0; x < GRAM_T; ++x) {
-          av[x] = As[min(ti * GRAM_T + x, GRAM_MAXK - 1)][c];
-          bv[x] = Bs[min(tj * GRAM_T + x, GRAM_MAXK - 1)][c];
+          av[x] = As[ra[x]][c];
+          bv[x] = Bs[rb[x]][c];
         }
 #pragma unroll
         for (int x = 0; x < GRAM_T; ++x)
@@ -52,25 +69,38 @@ __global__ __launch_bounds__(GRAM_NT) void k_gram(const double* __restrict__ a, 
     }
     __syncthreads();
   }
-  if (!owner) return;
-  double* out = partial + (long long)blockIdx.x * ka * kb;
+  // lanes of a tile -> one partial per output element, through the (now free) staging buffer
+  double* red = &As[0][0];  // [GRAM_NT][GRAM_T * GRAM_T] fits in As
 #pragma unroll
   for (int x = 0; x < GRAM_T; ++x)
 #pragma unroll
-    for (int y = 0; y < GRAM_T; ++y) {
-      const int i = ti * GRAM_T + x, j = tj * GRAM_T + y;
-      if (i < ka && j < kb) out[i * kb + j] = acc[x][y];
-    }
+    for (int y = 0; y < GRAM_T; ++y) red[t * GRAM_T * GRAM_T + x * GRAM_T + y] = acc[x][y];
+  __syncthreads();
+  for (int q = t; q < tiles * GRAM_T * GRAM_T; q += GRAM_NT) {
+    const int tt = q / (GRAM_T * GRAM_T), e = q - tt * (GRAM_T * GRAM_T);
+    const int i = (tt / tiles_j) * GRAM_T + e / GRAM_T, j = (tt - (tt / tiles_j) * tiles_j) * GRAM_T + e % GRAM_T;
+    if (i >= ka || j >= kb) continue;
+    double sum = 0.0;
+    for (int l = 0; l < lanes; ++l) sum += red[(tt * lanes + l) * GRAM_T * GRAM_T + e];
+    partial[(long long)(i * kb + j) * gridDim.x + blockIdx.x] = sum;
+  }
 }
 
-// out[e] = scale * (sum over blocks of partial[block][e], in block order)
-__global__ void k_sum_blocks(const double* __restrict__ partial, int nblocks, int count, double scale,
-                             double* __restrict__ out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// out[e] = scale * (sum over blocks of partial[e][block]): one workgroup per element, fixed-order tree
+__global__ __launch_bounds__(256) void k_sum_blocks(const double* __restrict__ partial, int nblocks, int count,
+                                                   double scale, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int e = blockIdx.x, t = threadIdx.x;
   if (e >= count) return;
   double s = 0.0;
-  for (int blk = 0; blk < nblocks; ++blk) s += partial[(long long)blk * count + e];
-  out[e] = scale * s;
+  for (int blk = t; blk < nblocks; blk += 256) s += partial[(long long)e * nblocks + blk];
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[e] = scale * red[0];
 }
 
 constexpr int COMB_M = 16;  // output columns per pass
@@ -116,7 +146,7 @@ void gram_device(const double* a, const double* b, long long n, int ka, int kb, 
   hipLaunchKernelGGL(k_gram, dim3(blocks), dim3(GRAM_NT), 0, s, a, b, n, ka, kb, work);
   DFMI_HIP(hipGetLastError());
   const int count = ka * kb;
-  hipLaunchKernelGGL(k_sum_blocks, dim3((count + 255) / 256), dim3(256), 0, s, work, blocks, count, scale, out);
+  hipLaunchKernelGGL(k_sum_blocks, dim3(count), dim3(256), 0, s, work, blocks, count, scale, out);
   DFMI_HIP(hipGetLastError());
 }
 

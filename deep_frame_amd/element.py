@@ -1,8 +1,8 @@
 """Hex8 element matrices and regular-grid numbering (host set-up, FP64 NumPy).
 
 Same definitions as deep_frame/topology_optimization.py: corner order (:246-249),
-isotropic / orthotropic constitutive matrices (:251-267), 2x2x2 Gauss stiffness
-(:269-299) and C-order node/DOF numbering (:301-319).  These run once per problem
+isotropic / orthotropic constitutive matrices (:251-267), 2x2x2 Gauss stiffness and the
+consistent mass (:269-299) and C-order node/DOF numbering (:301-319).  These run once per problem
 on the host; the per-iteration work is in libdfmi.
 """
 import numpy as np
@@ -50,6 +50,17 @@ def stiffness(spacing, d):
         b[:, row, :, comp] = grad[:, :, axis]
     b = b.reshape(8, 6, 24)
     return np.einsum("gsi,st,gtj->ij", b, d, b) * np.prod(h) / 8.0
+
+
+def mass(spacing):
+    """24 x 24 consistent mass of a unit-density brick, DOF = 3 * corner + component: the exact integral of
+    N_i N_j, which for trilinear shape functions is V / 216 * prod over the axes of (2 if corners i and j share
+    that coordinate else 1), times the 3 x 3 identity (the matrix hexahedron_matrices forms by 2x2x2 Gauss
+    quadrature, topology_optimization.py:269-299)."""
+    volume = float(np.prod(np.asarray(spacing, dtype=float)))
+    same = (CORNERS[:, None, :] == CORNERS[None, :, :]).astype(float) + 1.0    # (8, 8, 3)
+    m8 = volume / 216.0 * np.prod(same, axis=2)
+    return np.kron(m8, np.eye(3))
 
 
 def node_index(shape):

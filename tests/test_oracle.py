@@ -55,6 +55,15 @@ def test_euler_bernoulli_known_answer():
     assert -u[3 * tip + 2, 0].mean() == pytest.approx(gold["tip_deflection_mm"], abs=5e-8)
 
 
+def test_element_mass_closed_form_matches_quadrature():
+    """Closed-form consistent mass (element.mass) against the oracle's Gauss quadrature and the total mass."""
+    for h in ((1.0, 1.0, 1.0), (2.0, 1.5, 0.75)):
+        _, me, _ = hex8.hex8(h, hex8.isotropic_d(0.3))
+        m = element.mass(h)
+        assert np.allclose(m, me, rtol=1e-13, atol=1e-15)
+        assert m[0::3, 0::3].sum() == pytest.approx(np.prod(h))
+
+
 def test_first_frequency_known_answer():
     """Documented known answer (docs/topology_optimization.md, Numerische Pruefungen): first eigenfrequency
     867.609 Hz of the clamped 48 x 6 x 6 mm beam, 24 x 4 x 4 cells, E = 4430 MPa, nu = 0.3, 1.09 g/cm3
