@@ -12,7 +12,9 @@ __host__ __device__ inline bool live_bit(uint8_t m, int c) { return (m >> c) & 1
 __device__ inline double weight1(int t) { return t == 0 ? 1.0 : ((t == 1 || t == -1) ? 0.5 : 0.0); }
 
 // Thread per (node, pair of right-hand sides), right-hand sides fastest: with the interleaved layout the
-// threads of one node read one contiguous run per neighbour.  The node's 27 x 9 stencil entries are
+// threads of one node read one contiguous run per neighbour.  Thread g of a node takes columns g and
+// g + groups, so each load instruction of the node's threads covers one dense run of columns (no
+// interleaved gaps that would make the address unit visit every cache line twice).  The node's 27 x 9 stencil entries are
 // staged once per workgroup in LDS and broadcast to its threads.  No bounds or mask tests in the
 // gather: stencil entries towards missing or dead neighbours are zero (Galerkin product and
 // stencil_mask) and level vectors are zero on dead DOFs, so a neighbour index that runs off a grid
@@ -61,10 +63,13 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
   const int j = t / groups, grp = t - (t / groups) * groups;
   const int n = n0 + j;
   if (j >= nodes_per_block || n >= N) return;
-  const int r0 = grp * RP;
+  int col[RP];
   bool on[RP];
 #pragma unroll
-  for (int q = 0; q < RP; ++q) on[q] = r0 + q < a.k && (!a.active || a.active[r0 + q]);
+  for (int q = 0; q < RP; ++q) {
+    col[q] = grp + q * groups;
+    on[q] = col[q] < a.k && (!a.active || a.active[col[q]]);
+  }
   bool any = false;
 #pragma unroll
   for (int q = 0; q < RP; ++q) any = any || on[q];
@@ -83,7 +88,7 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
   const int nsb = ns * (int)sizeof(T);
   unsigned cb[RP];  // byte offset of each of this thread's columns within node n
 #pragma unroll
-  for (int q = 0; q < RP; ++q) cb[q] = (unsigned)(n * nsb + min(r0 + q, a.k - 1) * rs * (int)sizeof(T));
+  for (int q = 0; q < RP; ++q) cb[q] = (unsigned)(n * nsb + min(col[q], a.k - 1) * rs * (int)sizeof(T));
 #pragma unroll 3
   for (int d = 0; d < 27; ++d) {
     const int od = ((d / 9 - 1) * px + ((d / 3) % 3 - 1) * py + (d % 3 - 1)) * nsb;  // uniform
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
     if (!on[q]) continue;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const int o = n * ns + (r0 + q) * rs + c;
+      const int o = n * ns + col[q] * rs + c;
       const bool lv = live_bit(mn, c);
       if (MODE == MODE_APPLY) {
         Y[o] = lv ? acc[q][c] : T(0);
