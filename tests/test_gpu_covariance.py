@@ -30,6 +30,30 @@ def test_gram_and_combine_match_numpy(torch_gpu):
     assert np.max(np.abs(y - m.T @ b)) < 1e-12 * np.max(np.abs(m.T @ b))
 
 
+@pytest.mark.parametrize("ka,kb,n", [(1, 1, 300), (8, 8, 100003), (7, 42, 777), (64, 3, 4099), (64, 64, 20000)])
+def test_gram_shapes_and_accumulating_combine(torch_gpu, ka, kb, n):
+    """dfmi_gram over the tile/lane splits (one thread per tile up to 64 lanes per tile) and dfmi_combine
+    with accumulate = 1, against NumPy."""
+    from deep_frame_amd import _native
+    torch = torch_gpu
+    rng = np.random.default_rng(ka * 100 + kb)
+    a, b = rng.normal(size=(ka, n)), rng.normal(size=(kb, n))
+    ta, tb = torch.as_tensor(a, device="cuda"), torch.as_tensor(b, device="cuda")
+    out = torch.empty((ka, kb), dtype=torch.float64, device="cuda")
+    work = torch.empty(37 * ka * kb, dtype=torch.float64, device="cuda")
+    _native.call("dfmi_gram", _native.ptr(ta), _native.ptr(tb), n, ka, kb, _native.ptr(out), _native.ptr(work), 37,
+                 _native.stream_handle(torch))
+    ref = a @ b.T
+    assert np.max(np.abs(out.cpu().numpy() - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref))) * np.sqrt(n)
+    m = rng.normal(size=(ka, 5))
+    y0 = rng.normal(size=(5, n))
+    y = torch.as_tensor(y0, device="cuda")
+    _native.call("dfmi_combine", _native.ptr(ta), n, ka, _native.ptr(torch.as_tensor(m, device="cuda")), 5,
+                 _native.ptr(y), 1, _native.stream_handle(torch))
+    expect = y0 + m.T @ a
+    assert np.max(np.abs(y.cpu().numpy() - expect)) < 1e-12 * np.max(np.abs(expect)) * ka
+
+
 def test_golden_cantilever_mean_and_worst(torch_gpu):
     """Golden docs/validation/formulation_covariance_cantilever.json: load_mean and load_worst of the
     dense full model from GPU solves of its three unit loads."""
