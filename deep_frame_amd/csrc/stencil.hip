@@ -41,6 +41,22 @@ __device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, unsigned byte, d
 // buffer descriptor word 3 for raw 32-bit loads on gfx9 (data format 32)
 constexpr int kBufferWord3 = 0x00020000;
 
+// Three consecutive entries written through one (FP32) or two (FP64) buffer stores.
+__device__ __forceinline__ void store3(__amdgpu_buffer_rsrc_t r, unsigned byte, float a, float b, float c) {
+  __attribute__((ext_vector_type(3))) unsigned v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c)};
+  __builtin_amdgcn_raw_buffer_store_b96(v, r, byte, 0, 0);
+}
+__device__ __forceinline__ void store3(__amdgpu_buffer_rsrc_t r, unsigned byte, double a, double b, double c) {
+  __attribute__((ext_vector_type(4))) unsigned v = {(unsigned)__double2loint(a), (unsigned)__double2hiint(a),
+                                                    (unsigned)__double2loint(b), (unsigned)__double2hiint(b)};
+  __attribute__((ext_vector_type(2))) unsigned w = {(unsigned)__double2loint(c), (unsigned)__double2hiint(c)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, byte, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, byte + 16, 0, 0);
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer(const void* p, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kBufferWord3);
+}
+
 template <typename T, int RP, int MODE>
 __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restrict__ st,
                                                         const uint8_t* __restrict__ mask, ApplyArgs a, VLay lay,
@@ -103,27 +119,36 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
       acc[q][2] = fma(s6, x0, fma(s7, x1, fma(s8, x2, acc[q][2])));
     }
   }
+  // epilogue: the node's three components of a column are one load or store per operand (the result,
+  // b and x_prev share the operand's layout and extent)
   const uint8_t mn = mask[n];
-  T* Y = reinterpret_cast<T*>(a.y);
+  const __amdgpu_buffer_rsrc_t yr = buffer(a.y, extent);
+  T dv[3] = {T(0), T(0), T(0)};
+  if (MODE == MODE_CHEB) load3(buffer(a.dinv, (long long)N * 3 * (long long)sizeof(T)), (unsigned)(n * 3 * (int)sizeof(T)),
+                               dv[0], dv[1], dv[2]);
 #pragma unroll
   for (int q = 0; q < RP; ++q) {
     if (!on[q]) continue;
+    const unsigned ob = (unsigned)((n * ns + col[q] * rs) * (int)sizeof(T));
+    T y[3];
+    if (MODE == MODE_APPLY) {
+      y[0] = acc[q][0];
+      y[1] = acc[q][1];
+      y[2] = acc[q][2];
+    } else if (MODE == MODE_RESID) {
+      T b[3];
+      load3(buffer(a.b, extent), ob, b[0], b[1], b[2]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int o = n * ns + col[q] * rs + c;
-      const bool lv = live_bit(mn, c);
-      if (MODE == MODE_APPLY) {
-        Y[o] = lv ? acc[q][c] : T(0);
-      } else if (MODE == MODE_RESID) {
-        Y[o] = lv ? reinterpret_cast<const T*>(a.b)[o] - acc[q][c] : T(0);
-      } else if (MODE == MODE_CHEB) {
-        const T xv = X[o];
-        const T xpv = a.xp ? reinterpret_cast<const T*>(a.xp)[o] : T(0);
-        const T bv = reinterpret_cast<const T*>(a.b)[o];
-        const T dv = reinterpret_cast<const T*>(a.dinv)[n * 3 + c];
-        Y[o] = lv ? xv + (T)a.c1 * (xv - xpv) + (T)a.c2 * dv * (bv - acc[q][c]) : T(0);
-      }
+      for (int c = 0; c < 3; ++c) y[c] = b[c] - acc[q][c];
+    } else if (MODE == MODE_CHEB) {
+      T x[3], xp[3] = {T(0), T(0), T(0)}, b[3];
+      load3(xr, ob, x[0], x[1], x[2]);
+      if (a.xp) load3(buffer(a.xp, extent), ob, xp[0], xp[1], xp[2]);
+      load3(buffer(a.b, extent), ob, b[0], b[1], b[2]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) y[c] = x[c] + (T)a.c1 * (x[c] - xp[c]) + (T)a.c2 * dv[c] * (b[c] - acc[q][c]);
     }
+    store3(yr, ob, live_bit(mn, 0) ? y[0] : T(0), live_bit(mn, 1) ? y[1] : T(0), live_bit(mn, 2) ? y[2] : T(0));
   }
 }
 
