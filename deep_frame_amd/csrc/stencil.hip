@@ -334,19 +334,24 @@ __device__ inline void prolong_node(const Geom& gc, const T* __restrict__ C, int
   node_xyz(n, gf, i, j, kz);
   acc[0] = acc[1] = acc[2] = T(0);
   const int i0 = i >> 1, j0 = j >> 1, k0 = kz >> 1;
-  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (kz & 1) ? 2 : 1;
-  const T w = T(((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((kz & 1) ? 0.5 : 1.0));
-  for (int a = 0; a < ni; ++a)
-    for (int b = 0; b < nj; ++b)
-      for (int d = 0; d < nk; ++d) {
-        const T* v = C + (((i0 + a) * gc.ny + j0 + b) * gc.nz + k0 + d) * cns;
+  // branch-free: always the eight parents of the coarse cell; an even coordinate takes its own parent with
+  // weight 1 twice at half weight (the second read repeats the first), an odd one both neighbours at 1/2
+  const int i1 = i0 + (i & 1), j1 = j0 + (j & 1), k1 = k0 + (kz & 1);
+  const int pi[2] = {i0, i1}, pj[2] = {j0, j1}, pk[2] = {k0, k1};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const T* v = C + (((long long)pi[a] * gc.ny + pj[b]) * gc.nz + pk[d]) * cns;
         acc[0] += v[0];
         acc[1] += v[1];
         acc[2] += v[2];
       }
-  acc[0] *= w;
-  acc[1] *= w;
-  acc[2] *= w;
+  acc[0] *= T(0.125);
+  acc[1] *= T(0.125);
+  acc[2] *= T(0.125);
 }
 
 template <typename T>
@@ -406,8 +411,11 @@ __global__ void k_prolong_add(Geom gc, const T* __restrict__ c, VLay cl, Geom gf
   prolong_node<T>(gc, c + (long long)r * cl.rs, (int)cl.ns, gf, n, acc);
   const uint8_t m = fmask[n];
   T* F = f + (long long)n * fl.ns + (long long)r * fl.rs;
-  for (int q = 0; q < 3; ++q)
-    if (live_bit(m, q)) F[q] = F[q] + acc[q];
+  // the three components are read and written unconditionally (one 3-word access each way)
+  const T f0 = F[0], f1 = F[1], f2 = F[2];
+  F[0] = live_bit(m, 0) ? f0 + acc[0] : f0;
+  F[1] = live_bit(m, 1) ? f1 + acc[1] : f1;
+  F[2] = live_bit(m, 2) ? f2 + acc[2] : f2;
 }
 
 template <typename T>
