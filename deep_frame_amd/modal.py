@@ -206,7 +206,8 @@ def eigenpairs(mg, key, fixed, mass, lumped, count, initial=None):
     report = {"iterations": step, "relative_residual": relative[:count].tolist(),
               "converged": bool((relative[:count] < tolerance).all()), "history": history,
               "seconds": perf_counter() - started, "setup_s": entry.pop("setup_s", 0.0), "levels": mg.level_count,
-              "coarsest_dofs": entry["coarsest_dofs"], "kernel": None, "lambda_max": entry["lambda_max"]}
+              "coarsest_dofs": entry["coarsest_dofs"], "kernel": entry.get("kernel") or None,
+              "lambda_max": entry["lambda_max"]}
     mg.statistics.append(report)
     return values[:count], current[0], report
 

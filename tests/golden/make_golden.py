@@ -7,6 +7,8 @@ Reads (as JSON/text data, nothing is imported or executed):
   docs/topology_optimization.md "Numerische Pruefungen": Euler-Bernoulli tip deflection
   exports/handoff/neural_r3_v05/density_half.npz   an optimised half-frame density (51 x 96 x 24),
       loaded with allow_pickle=False and rewritten unchanged (key "density", float64)
+  exports/handoff/dgx_075mm/run3_r3_v05/density_half.npz   the optimised 0.75 mm half-frame density
+      (91 x 170 x 48), same treatment
 Run from the repository root with the reference mounted at /root/reference.
 """
 import json
@@ -45,6 +47,10 @@ def main():
         density = np.asarray(data["density"])
     assert density.shape == (51, 96, 24) and density.dtype == np.float64
     np.savez_compressed(OUT / "density_half_neural_r3_v05.npz", density=density)
+    with np.load(REF / "exports/handoff/dgx_075mm/run3_r3_v05/density_half.npz", allow_pickle=False) as data:
+        density = np.asarray(data["density"])
+    assert density.shape == (91, 170, 48) and density.dtype == np.float64
+    np.savez_compressed(OUT / "density_half_075mm_run3.npz", density=density)
 
 
 if __name__ == "__main__":
