@@ -5,6 +5,11 @@ One step = update(moduli) + Galerkin hierarchy set-up + 42-RHS PCG solve to a
 true FP64 relative residual < 1e-8 per column, with the density field, fixture
 and the 42 unit-wrench right-hand sides already resident in HBM.
 
+The mixed solver is FP32 PCG with reliable FP64 updates around an FP32 V-cycle
+whose level-1 correction is two flexible-CG steps preconditioned by the level-1
+V-cycle (coarse_krylov 2), with Chebyshev degree 3 on every level (growth 1.0);
+the FP64 solve is the reference's scheme (FP64 V-cycle, degree 3 * 2^level).
+
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 [--config frame|frame_small|cantilever] [--no-cpu-baseline] [--no-fp64]
 """
@@ -63,6 +68,11 @@ def parse():
                     help="mixed solver: FP32 PCG inside FP64 iterative refinement, or FP64 PCG with an FP32 V-cycle")
     ap.add_argument("--inner-tolerance", type=float, default=None, help="refine scheme: inner relative tolerance")
     ap.add_argument("--apply-reps", type=int, default=20)
+    ap.add_argument("--coarse-krylov", type=int, default=2, choices=[0, 1, 2],
+                    help="mixed solver: flexible-CG steps of the level-1 solve inside the fine V-cycle (0: V-cycle)")
+    ap.add_argument("--degree", type=int, default=3, help="mixed solver: Chebyshev degree on the fine level")
+    ap.add_argument("--growth", type=float, default=1.0,
+                    help="mixed solver: Chebyshev degree growth per level (degree * growth^level)")
     ap.add_argument("--fine-smoother", default="vector", choices=["matrix", "vector"],
                     help="fine level of the fp32 V-cycle on the matrix cores (split bf16) or the FP32 vector units")
     ap.add_argument("--fp64-only", action="store_true", help="only time the plain FP64 solve (for kernel traces)")
@@ -170,7 +180,8 @@ def main():
         _, t64, rep64, _ = run({"precision": "float64"}, args.steps, args.warmup, "fp64")
         print(json.dumps({"fp64_s": float(np.mean(t64)), "iterations": max(r[0] for r in rep64)}))
         return
-    mixed = {"precision": "float32", "refine": args.scheme == "refine", "fine_smoother": args.fine_smoother}
+    mixed = {"precision": "float32", "refine": args.scheme == "refine", "fine_smoother": args.fine_smoother,
+             "coarse_krylov": args.coarse_krylov, "growth": args.growth, "chebyshev_degree": args.degree}
     if args.inner_tolerance:
         mixed["inner_tolerance"] = args.inner_tolerance
     mg, times, reports, x = run(mixed, args.steps, args.warmup, f"mixed-{args.scheme}")
@@ -212,7 +223,8 @@ def main():
                                f"{ndof} DOF, {k} RHS, {cfg['levels']}-level GMG", "scheme": args.scheme,
                    "mixed_s": t_mixed, "fp64_s": t_fp64, "iterations": max(r[0] for r in reports),
                    "max_rel_res": max(r[1] for r in reports), "fine_smoother": args.fine_smoother,
-                   "fine_tile": list(tile[:2]) if tile else None},
+                   "coarse_krylov": args.coarse_krylov, "growth": args.growth,
+                   "chebyshev_degree": args.degree, "fine_tile": list(tile[:2]) if tile else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, main_precision),
                      "traffic_source": str(TRAFFIC[main_precision].relative_to(ROOT)),

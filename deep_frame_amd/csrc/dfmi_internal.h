@@ -197,6 +197,24 @@ void pcg_update(T* x, T* r, const T* p, const T* ap, const double* alpha, long l
 template <typename T>
 void pcg_direction(T* p, const T* z, const double* beta, long long n, int k, const int* active, hipStream_t s);
 void add_scaled(double* x, const float* d, const double* scale, long long n, int k, const int* active, hipStream_t s);
+// Krylov coarse solve on interleaved level vectors [node][r][3]: dots of np (2 or 4) pairs a[i].b[i] per
+// column into partial[(i * k + r) * nblocks + block]; scalar steps on the rows of ks (9 k doubles); y -= f x;
+// x = a x + b p
+template <typename T>
+struct LevelPairs {
+  const T* a[4];
+  const T* b[4];
+};
+int level_dot_blocks(long long nodes);
+template <typename T>
+void level_dots(const LevelPairs<T>& p, int np, long long nodes, int k, const int* active, double* partial,
+                int nblocks, hipStream_t s);
+void krylov_scalars(double* ks, int k, const int* active, int step, hipStream_t s);
+template <typename T>
+void level_axpy(T* y, const T* x, const double* f, long long nodes, int k, const int* active, hipStream_t s);
+template <typename T>
+void level_combine(T* x, const T* p, const double* a, const double* b, long long nodes, int k, const int* active,
+                   hipStream_t s);
 template <typename T>
 void scale_columns(T* v, long long n, int k, const double* f, const int* select, hipStream_t s);
 // dst[c] = src[c] (square_scale false) or dst[c] *= src[c]^2 (true) for selected columns

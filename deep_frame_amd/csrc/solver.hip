@@ -70,6 +70,9 @@ struct Level {
   double lambda_max = 0.0;
   // V-cycle work vectors (precision of the V-cycle), capacity k_cap columns
   DevBuf rhs, x, xa, res;
+  // level 1 with a Krylov coarse solve: first direction, its product, the second product (interleaved
+  // vectors), the step scalars and the dot partials
+  DevBuf kp, kq, kaz, kscal, kpart;
 };
 
 struct Hierarchy {
@@ -380,6 +383,51 @@ static void learned_correction(VArgs& v, const T* b) {
   }
 }
 
+template <typename T>
+static void vcycle(VArgs& v, int idx, const T* b);
+
+// Level-1 solve of the fine level's coarse-grid correction by m = coarse_krylov flexible-CG steps, each
+// preconditioned by the level-1 V-cycle (a K-cycle on the first coarse level): z0 = V(g), q = A z0,
+// alpha0 = g.z0 / z0.q; for m = 2 then r1 = g - alpha0 q, z1 = V(r1), beta = (r1.z1 - r1.z0) / g.z0 and
+// p1 = z1 + beta z0, whose product A p1 = A z1 + beta q needs one more operator apply, and
+// x = alpha0 z0 + alpha1 p1 with alpha1 = r1.z1 / p1.A p1.  g (level 1's rhs buffer) is overwritten by r1;
+// the result is left in level 1's x, as the V-cycle leaves it.
+template <typename T>
+static void krylov_coarse(VArgs& v, T* g) {
+  Level& C = *v.H->levels[1];
+  hipStream_t s = v.h->stream;
+  const int k = v.k, m = v.h->st.coarse_krylov;
+  const long long nodes = C.g.nodes();
+  double* ks = C.kscal.as<double>();
+  double* part = C.kpart.as<double>();
+  const int nb = level_dot_blocks(nodes);
+  vcycle<T>(v, 1, g);
+  std::swap(C.x.p, C.kp.p), std::swap(C.x.bytes, C.kp.bytes);  // kp = z0
+  const T* z0 = C.kp.as<T>();
+  ApplyArgs a{};
+  a.x = z0;
+  a.y = C.kq.as<T>();
+  level_apply<T>(v, 1, MODE_APPLY, a);
+  LevelPairs<T> d1{{g, z0, nullptr, nullptr}, {z0, C.kq.as<T>(), nullptr, nullptr}};
+  level_dots<T>(d1, 2, nodes, k, v.active, part, nb, s);
+  reduce_partials(part, nb, 2 * k, ks, s);
+  krylov_scalars(ks, k, v.active, 1, s);
+  if (m >= 2) {
+    level_axpy<T>(g, C.kq.as<T>(), ks + 6 * (size_t)k, nodes, k, v.active, s);  // r1
+    vcycle<T>(v, 1, g);                                                         // z1 in C.x
+    ApplyArgs b{};
+    b.x = C.x.as<T>();
+    b.y = C.kaz.as<T>();
+    level_apply<T>(v, 1, MODE_APPLY, b);
+    const T* z1 = C.x.as<T>();
+    LevelPairs<T> d2{{g, g, z1, z1}, {z1, z0, C.kaz.as<T>(), C.kq.as<T>()}};
+    level_dots<T>(d2, 4, nodes, k, v.active, part, nb, s);
+    reduce_partials(part, nb, 4 * k, ks + 2 * (size_t)k, s);
+    krylov_scalars(ks, k, v.active, 2, s);
+  }
+  level_combine<T>(C.x.as<T>(), z0, ks + 7 * (size_t)k, ks + 8 * (size_t)k, nodes, k, v.active, s);
+}
+
 // V-cycle on level idx for right-hand side b (device, T); result in levels[idx].x
 template <typename T>
 static void vcycle(VArgs& v, int idx, const T* b) {
@@ -412,7 +460,10 @@ static void vcycle(VArgs& v, int idx, const T* b) {
     restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
                      level_layout(C, idx + 1, v.k), v.k, v.active, s);
   }
-  vcycle<T>(v, idx + 1, C.rhs.as<T>());
+  if (idx == 0 && v.h->st.coarse_krylov > 0 && H.levels.size() > 2)
+    krylov_coarse<T>(v, C.rhs.as<T>());
+  else
+    vcycle<T>(v, idx + 1, C.rhs.as<T>());
   if (change) {
     relayout<T>(C.x.as<T>(), C.res.as<T>(), C.g.nodes(), v.k, true, v.active, s);
     prolong_add<T>(C.g, C.res.as<T>(), column_major(C.g), L.g, L.mask.as<uint8_t>(), L.x.as<T>(), column_major(L.g),
@@ -437,6 +488,15 @@ static void ensure_level_vectors(Handle* h, Hierarchy& H, int k) {
     L.x.ensure(n);
     L.xa.ensure(n);
     L.res.ensure(n);
+  }
+  if (h->st.coarse_krylov > 0 && H.levels.size() > 2) {
+    Level& C = *H.levels[1];
+    const size_t n = (size_t)k * col(C.g) * el;
+    C.kp.ensure(n);
+    C.kq.ensure(n);
+    C.kaz.ensure(n);
+    C.kscal.ensure(sizeof(double) * 9 * (size_t)k);
+    C.kpart.ensure(sizeof(double) * 4 * (size_t)k * level_dot_blocks(C.g.nodes()));
   }
   if (H.coarsest_n > 0) {
     H.work_in.ensure(sizeof(double) * (size_t)k * H.coarsest_n);
@@ -1113,6 +1173,7 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->inner_tolerance = 1e-2;
   o->fine_smoother = 0;
   o->retries = 2;
+  o->coarse_krylov = 0;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
@@ -1131,6 +1192,7 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
     if (st.precision != 0 && st.precision != 1) throw std::invalid_argument("precision must be 0 or 1");
     if (st.fine_smoother != 0 && st.fine_smoother != 1) throw std::invalid_argument("fine_smoother must be 0 or 1");
     if (st.retries < 0) throw std::invalid_argument("retries must be >= 0");
+    if (st.coarse_krylov < 0 || st.coarse_krylov > 2) throw std::invalid_argument("coarse_krylov must be 0, 1 or 2");
     if (st.chebyshev_degree < 1 || st.max_iterations < 1 || !(st.tolerance > 0))
       throw std::invalid_argument("invalid multigrid settings");
     h.st = st;

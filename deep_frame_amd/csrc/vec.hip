@@ -3,6 +3,8 @@
 // results are deterministic), AXPY-type updates, precision conversion and the
 // flat <-> padded-grid layout changes.  Column r of a multi-vector is the
 // contiguous slice v[r * n .. (r + 1) * n).
+#include <stdexcept>
+
 #include "dfmi_internal.h"
 
 namespace dfmi {
@@ -455,5 +457,153 @@ void mask_vector(const uint8_t* mask, double* v, long long nodes, int k, hipStre
   hipLaunchKernelGGL(k_mask_vector, dim3((unsigned)((nodes * 3 + 255) / 256), k), dim3(256), 0, s, mask, v, nodes);
   DFMI_HIP(hipGetLastError());
 }
+
+// ---- Krylov coarse solve on interleaved level vectors [node][r][3] ---------------------------
+// Per-column dot products of up to four vector pairs in one pass over the vectors.  A workgroup owns a
+// contiguous range of nodes; its threads form slots of 3 kc lanes (one lane per stored value of the kc
+// columns c0 .. c0 + kc - 1 of a node), slot q taking nodes n0 + q, n0 + q + slots, ...  Each lane keeps
+// its own sum; a column's 3 x slots lane sums are then added in a fixed order, so the partials are
+// deterministic.  partial[(i * k + r) * gridDim.x + block] holds pair i, column r.
+constexpr int KDOT_NT = 256;
+constexpr int KDOT_COLS = KDOT_NT / 3;  // columns per launch (one slot of lanes per node at least)
+
+template <typename T, int NP>
+__global__ __launch_bounds__(KDOT_NT) void k_level_dots(LevelPairs<T> p, long long nodes, int k, int c0, int kc,
+                                                        const int* active, double* partial) {
+  __shared__ double lane[NP][KDOT_NT];
+  const int w = 3 * kc, slots = KDOT_NT / w;
+  const int t = threadIdx.x, q = t / w, j = t - q * w;
+  double acc[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) acc[i] = 0.0;
+  if (q < slots && (!active || active[c0 + j / 3])) {
+    const long long per = (nodes + gridDim.x - 1) / gridDim.x;
+    const long long n0 = (long long)blockIdx.x * per, n1 = min(nodes, n0 + per);
+    for (long long n = n0 + q; n < n1; n += slots) {
+      const long long o = n * 3LL * k + 3LL * c0 + j;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) acc[i] += (double)p.a[i][o] * (double)p.b[i][o];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) lane[i][t] = acc[i];
+  __syncthreads();
+  for (int e = t; e < NP * kc; e += KDOT_NT) {
+    const int i = e / kc, r = e - i * kc;
+    double sum = 0.0;
+    for (int qq = 0; qq < slots; ++qq)
+      for (int c = 0; c < 3; ++c) sum += lane[i][qq * w + 3 * r + c];
+    partial[((long long)i * k + c0 + r) * gridDim.x + blockIdx.x] = sum;
+  }
+}
+
+int level_dot_blocks(long long nodes) { return (int)std::max(1LL, std::min(2048LL, (nodes + 127) / 128)); }
+
+template <typename T>
+void level_dots(const LevelPairs<T>& p, int np, long long nodes, int k, const int* active, double* partial,
+                int nblocks, hipStream_t s) {
+  for (int c0 = 0; c0 < k; c0 += KDOT_COLS) {
+    const int kc = std::min(KDOT_COLS, k - c0);
+    if (np == 2)
+      hipLaunchKernelGGL((k_level_dots<T, 2>), dim3(nblocks), dim3(KDOT_NT), 0, s, p, nodes, k, c0, kc, active,
+                         partial);
+    else if (np == 4)
+      hipLaunchKernelGGL((k_level_dots<T, 4>), dim3(nblocks), dim3(KDOT_NT), 0, s, p, nodes, k, c0, kc, active,
+                         partial);
+    else
+      throw std::invalid_argument("level_dots: 2 or 4 vector pairs");
+    DFMI_HIP(hipGetLastError());
+  }
+}
+template void level_dots<float>(const LevelPairs<float>&, int, long long, int, const int*, double*, int, hipStream_t);
+template void level_dots<double>(const LevelPairs<double>&, int, long long, int, const int*, double*, int,
+                                 hipStream_t);
+
+// ks rows (k doubles each): 0 rho0 = g.z0, 1 z0.A z0, 2 rho1 = r1.z1, 3 r1.z0, 4 z1.A z1, 5 z1.A z0,
+// 6 alpha0, 7 coefficient of z1, 8 coefficient of z0 in the result.
+__global__ void k_krylov_alpha(double* ks, int k, const int* active) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const double rho = ks[r], zaz = ks[k + r];
+  const bool ok = (!active || active[r]) && rho > 0.0 && zaz > 0.0;
+  ks[6 * k + r] = ok ? rho / zaz : 0.0;
+  ks[7 * k + r] = 0.0;
+  ks[8 * k + r] = ok ? rho / zaz : 0.0;
+}
+
+// second flexible-CG step: beta = (r1.z1 - r1.z0) / rho0, p1 = z1 + beta z0 with
+// p1.A p1 = z1.A z1 + 2 beta z1.A z0 + beta^2 z0.A z0, alpha1 = r1.z1 / p1.A p1, and the solution
+// alpha0 z0 + alpha1 p1 = alpha1 z1 + (alpha0 + alpha1 beta) z0
+__global__ void k_krylov_coefficients(double* ks, int k, const int* active) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= k) return;
+  const double rho0 = ks[r], zaz0 = ks[k + r], rho1 = ks[2 * k + r], rz0 = ks[3 * k + r];
+  const double zaz1 = ks[4 * k + r], zaz10 = ks[5 * k + r], a0 = ks[6 * k + r];
+  double cz = 0.0, cp = a0;
+  if ((!active || active[r]) && a0 > 0.0) {
+    const double beta = (rho1 - rz0) / rho0;
+    const double pap = zaz1 + 2.0 * beta * zaz10 + beta * beta * zaz0;
+    if (pap > 0.0 && rho1 > 0.0) {
+      const double a1 = rho1 / pap;
+      cz = a1;
+      cp = a0 + a1 * beta;
+    }
+  }
+  ks[7 * k + r] = cz;
+  ks[8 * k + r] = cp;
+}
+
+void krylov_scalars(double* ks, int k, const int* active, int step, hipStream_t s) {
+  if (step == 1)
+    hipLaunchKernelGGL(k_krylov_alpha, dim3((k + 63) / 64), dim3(64), 0, s, ks, k, active);
+  else
+    hipLaunchKernelGGL(k_krylov_coefficients, dim3((k + 63) / 64), dim3(64), 0, s, ks, k, active);
+  DFMI_HIP(hipGetLastError());
+}
+
+// y -= f[r] x over interleaved level vectors (columns with f = 0 or inactive are left alone)
+template <typename T>
+__global__ void k_level_axpy(T* __restrict__ y, const T* __restrict__ x, const double* f, long long n, int k,
+                             const int* active) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int r = (int)((e / 3) % k);
+  if (active && !active[r]) return;
+  y[e] -= (T)f[r] * x[e];
+}
+
+// x = a[r] x + b[r] p over interleaved level vectors (x is only read where a[r] != 0)
+template <typename T>
+__global__ void k_level_combine(T* __restrict__ x, const T* __restrict__ p, const double* a, const double* b,
+                                long long n, int k, const int* active) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int r = (int)((e / 3) % k);
+  if (active && !active[r]) return;
+  const double ca = a[r];
+  x[e] = (ca != 0.0 ? (T)ca * x[e] : T(0)) + (T)b[r] * p[e];
+}
+
+template <typename T>
+void level_axpy(T* y, const T* x, const double* f, long long nodes, int k, const int* active, hipStream_t s) {
+  const long long n = nodes * 3 * k;
+  hipLaunchKernelGGL(k_level_axpy<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, x, f, n, k, active);
+  DFMI_HIP(hipGetLastError());
+}
+template void level_axpy<float>(float*, const float*, const double*, long long, int, const int*, hipStream_t);
+template void level_axpy<double>(double*, const double*, const double*, long long, int, const int*, hipStream_t);
+
+template <typename T>
+void level_combine(T* x, const T* p, const double* a, const double* b, long long nodes, int k, const int* active,
+                   hipStream_t s) {
+  const long long n = nodes * 3 * k;
+  hipLaunchKernelGGL(k_level_combine<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, p, a, b, n, k,
+                     active);
+  DFMI_HIP(hipGetLastError());
+}
+template void level_combine<float>(float*, const float*, const double*, const double*, long long, int, const int*,
+                                   hipStream_t);
+template void level_combine<double>(double*, const double*, const double*, const double*, long long, int,
+                                    const int*, hipStream_t);
 
 }  // namespace dfmi

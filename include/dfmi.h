@@ -50,6 +50,11 @@ typedef struct dfmi_settings {
                                 operator is always the FP32 / FP64 vector kernel */
   int32_t retries;           /* 2: rounds of FP64 re-solves of columns that did not converge (warm start when
                                 their relative residual is below 1), topology_multigrid.py:376-406 */
+  int32_t coarse_krylov;     /* 0 (default, the reference's V-cycle) or 1 or 2: the fine level's coarse-grid
+                                correction solves the level-1 system by that many flexible-CG steps, each
+                                preconditioned by the level-1 V-cycle, instead of one level-1 V-cycle (a
+                                Krylov "K-cycle"; needs three or more levels, and a flexible outer PCG, which
+                                both solve schemes are) */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */

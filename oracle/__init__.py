@@ -10,7 +10,8 @@ Parity status per module:
   hex8      pinned (rigid modes, patch energy, Euler-Bernoulli KAT, dense
             flexibility golden vector of the covariance cantilever)
   system    pinned (same golden vectors; SuperLU residual <= 1e-12)
-  gmg       pinned (R = P^T, Galerkin = R A P by brute-force dense products)
+  gmg       pinned (R = P^T, Galerkin = R A P by brute-force dense products; fcg: two steps
+            equal the Galerkin projection onto span{Vg, VAVg}, many steps the exact level solve)
   filters   pinned (brute-force neighbour sums, adjoint identity, FD)
   oc        pinned (bisection invariants, volume target, move limits)
   mma       parity unpinned against mmapy 0.3.1 (absent); pinned against the

@@ -10,6 +10,9 @@ Restates ``deep_frame/topology_multigrid.py``:
   * Chebyshev smoother with Jacobi scaling, degree 3 * 2^level, bounds
     [1.1 lmax / 20, 1.1 lmax] (:284-305),
   * V-cycle (:306-317) with an exact coarsest solve,
+  * (no counterpart in the reference) the K-cycle variant of the fine level's V-cycle: the level-1
+    system of the coarse-grid correction is solved by a few flexible-CG steps preconditioned by the
+    level-1 V-cycle (``fcg``, ``kcycle``),
   * flexible PCG with Polak-Ribiere beta and true-residual confirmation (:407-454).
 Matrices are SciPy sparse over padded node grids with DOF = 3*node + comp.
 Test infrastructure only.
@@ -139,14 +142,58 @@ def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0):
     return chebyshev(lv, rhs, x, count, lmax[index])
 
 
-def pcg(levels, lmax, rhs, tol=1e-8, maxit=2000):
-    """Flexible PCG, Polak-Ribiere beta, per-column; returns (x, iterations, rel. residual)."""
+def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0):
+    """``steps`` iterations of flexible CG on A_index x = rhs from x = 0, each preconditioned by the V-cycle
+    from level ``index``: alpha = r.z / p.A p, beta = r_new.(z_new - z_old) / r_old.z_old."""
+    a = levels[index]["A"]
+    x = np.zeros_like(rhs)
+    r = rhs.copy()
+    z = vcycle(levels, lmax, r, index, degree, growth)
+    p = z.copy()
+    rz = r @ z
+    for step in range(steps):
+        ap = a @ p
+        pap = p @ ap
+        if not (rz > 0 and pap > 0):
+            break
+        alpha = rz / pap
+        x += alpha * p
+        if step == steps - 1:
+            break
+        r = r - alpha * ap
+        znew = vcycle(levels, lmax, r, index, degree, growth)
+        beta = (r @ (znew - z)) / rz
+        rz = r @ znew
+        p = znew + beta * p
+        z = znew
+    return x
+
+
+def kcycle(levels, lmax, rhs, steps=2, degree=3, growth=2.0):
+    """The fine level's V-cycle with its level-1 correction solved by ``fcg`` (``steps`` flexible-CG
+    steps) instead of one level-1 V-cycle; with two levels the level-1 solve is the exact coarsest one."""
+    if len(levels) < 3:
+        return vcycle(levels, lmax, rhs, 0, degree, growth)
+    lv = levels[0]
+    x = chebyshev(lv, rhs, None, degree, lmax[0])
+    res = (rhs - lv["A"] @ x) * lv["mask"]
+    p = levels[1]["P"]
+    corr = fcg(levels, lmax, 1, (p.T @ res) * levels[1]["mask"], steps, degree, growth)
+    x = x + (p @ corr) * lv["mask"]
+    return chebyshev(lv, rhs, x, degree, lmax[0])
+
+
+def pcg(levels, lmax, rhs, tol=1e-8, maxit=2000, precondition=None):
+    """Flexible PCG, Polak-Ribiere beta, per-column; returns (x, iterations, rel. residual).
+    ``precondition(r)`` defaults to the V-cycle."""
+    if precondition is None:
+        precondition = lambda r: vcycle(levels, lmax, r)
     a = levels[0]["A"]
     b = rhs * levels[0]["mask"]
     nb = np.linalg.norm(b)
     x = np.zeros_like(b)
     r = b.copy()
-    z = vcycle(levels, lmax, r)
+    z = precondition(r)
     p = z.copy()
     rz = r @ z
     for it in range(1, maxit + 1):
@@ -161,7 +208,7 @@ def pcg(levels, lmax, rhs, tol=1e-8, maxit=2000):
             if rel < tol:
                 return x, it, rel
             r = true
-        znew = vcycle(levels, lmax, r)
+        znew = precondition(r)
         beta = (r @ (znew - z)) / rz
         rz = r @ znew
         p = znew + beta * p

@@ -9,7 +9,7 @@ from oracle import gmg
 pytestmark = pytest.mark.gpu
 
 
-def case(shape, spacing, seed, precision, torch, fine_smoother="vector"):
+def case(shape, spacing, seed, precision, torch, fine_smoother="vector", coarse_krylov=0):
     from deep_frame_amd.multigrid import GeometricMultigrid
     ke = element.stiffness(spacing, element.isotropic(0.3))
     rho = np.clip(np.random.default_rng(seed).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
@@ -17,7 +17,7 @@ def case(shape, spacing, seed, precision, torch, fine_smoother="vector"):
     pts = element.node_points(shape, spacing)
     fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
     mg = GeometricMultigrid(shape, spacing, ke, {"coarsest_dofs": 200, "precision": precision,
-                                                 "fine_smoother": fine_smoother})
+                                                 "fine_smoother": fine_smoother, "coarse_krylov": coarse_krylov})
     mg.update(mod)
     entry = mg.hierarchy("v", fixed)
     levels = gmg.hierarchy(shape, ke, mod, fixed, gmg.level_count(shape, 200))
@@ -53,6 +53,26 @@ def test_vcycle_matches_oracle(shape, spacing, precision, torch_gpu):
     for cols in (2, 7):
         r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
         ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c]) for c in range(cols)])
+        z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
+        tol = 1e-9 if precision == "float64" else 1e-4
+        for c in range(cols):
+            assert np.linalg.norm(z[:, c] - ref[:, c]) < tol * np.linalg.norm(ref[:, c]), (cols, c)
+    mg.close()
+
+
+@pytest.mark.parametrize("shape,spacing", [((16, 8, 8), (1.0, 1.0, 1.0)), ((16, 8, 40), (1.0, 1.0, 0.5))])
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+@pytest.mark.parametrize("steps", [1, 2])
+def test_kcycle_matches_oracle(shape, spacing, precision, steps, torch_gpu):
+    """Fine V-cycle whose level-1 correction is solved by one or two flexible-CG steps (coarse_krylov)
+    against oracle.gmg.kcycle: FP64 to 1e-9, FP32 to 1e-4; 1, 2 and 7 columns (single-column layout,
+    column pairs, an odd last column)."""
+    print(f"kcycle shape={shape} precision={precision} steps={steps}", flush=True)
+    mg, entry, levels, fixed = case(shape, spacing, 1, precision, torch_gpu, coarse_krylov=steps)
+    assert len(levels) >= 3
+    for cols in (1, 2, 7):
+        r = np.random.default_rng(3).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
+        ref = np.column_stack([gmg.kcycle(levels, entry["lambda_max"], r[:, c], steps) for c in range(cols)])
         z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
         tol = 1e-9 if precision == "float64" else 1e-4
         for c in range(cols):

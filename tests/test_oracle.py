@@ -5,6 +5,7 @@ import json
 from pathlib import Path
 
 import numpy as np
+from scipy.sparse.linalg import splu
 import pytest
 from scipy.sparse.linalg import spsolve
 
@@ -309,6 +310,62 @@ def test_gmg_pcg_matches_direct_solve():
     x, it, rel = gmg.pcg(levels, lam, f)
     ref, _ = system.solve(dofs, conn, ke, mod, fixed, f, len(pts))
     assert rel < 1e-8 and it < 40
+    assert np.linalg.norm(x - ref[:, 0]) < 1e-7 * np.linalg.norm(ref)
+
+
+def _kcycle_case():
+    shape = (16, 8, 8)
+    ke, _, _ = hex8.hex8((1, 1, 1), hex8.isotropic_d(0.3))
+    rho = np.random.default_rng(4).uniform(0.3, 1.0, int(np.prod(shape)))
+    mod = 1e-6 + (1 - 1e-6) * rho ** 3
+    pts, conn, dofs = hex8.grid(shape)
+    fixed = problems.clamp_dofs(np.flatnonzero(pts[:, 0] == 0))
+    levels = gmg.hierarchy(shape, ke, mod, fixed, gmg.level_count(shape, 100))
+    assert len(levels) >= 3
+    lam = [gmg.spectrum(l) for l in levels[:-1]]
+    return levels, lam, pts, conn, dofs, ke, mod, fixed
+
+
+def test_gmg_two_fcg_steps_are_the_krylov_galerkin_projection():
+    """With a linear symmetric preconditioner V, two flexible-CG steps from zero give the A-orthogonal
+    projection of the solution onto span{V g, V A V g}: x = Z (Z^T A Z)^-1 Z^T g."""
+    levels, lam, *_ = _kcycle_case()
+    a, mask = levels[1]["A"], levels[1]["mask"]
+    g = np.random.default_rng(5).normal(size=a.shape[0]) * mask
+    v = lambda r: gmg.vcycle(levels, lam, r, 1)
+    z0 = v(g)
+    z = np.column_stack([z0, v(a @ z0)])
+    expect = z @ np.linalg.solve(z.T @ (a @ z), z.T @ g)
+    got = gmg.fcg(levels, lam, 1, g, 2)
+    assert np.linalg.norm(got - expect) < 1e-9 * np.linalg.norm(expect)
+    one = gmg.fcg(levels, lam, 1, g, 1)
+    assert np.linalg.norm(one - z0 * (g @ z0) / (z0 @ (a @ z0))) < 1e-12 * np.linalg.norm(one)
+
+
+def test_gmg_many_fcg_steps_solve_the_level_exactly_and_kcycle_pcg_converges():
+    """Enough flexible-CG steps reach the exact level-1 solve (sparse LU on the live DOFs), so the K-cycle
+    then equals the two-grid cycle; PCG with the two-step K-cycle meets 1e-8 in fewer iterations than with
+    the V-cycle and agrees with the direct solve."""
+    levels, lam, *_ = _kcycle_case()
+    a, mask = levels[1]["A"], levels[1]["mask"]
+    g = np.random.default_rng(6).normal(size=a.shape[0]) * mask
+    live = np.flatnonzero(mask > 0)
+    exact = np.zeros_like(g)
+    exact[live] = splu(a[live][:, live].tocsc()).solve(g[live])
+    got = gmg.fcg(levels, lam, 1, g, 40)
+    assert np.linalg.norm(got - exact) < 1e-8 * np.linalg.norm(exact)
+    shape = (32, 32, 8)  # frame fixture, 4 levels: the V-cycle leaves level-1 error that two FCG steps remove
+    prob = problems.frame(shape, seed=0, rhs=6)
+    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
+    mod, _ = problems.moduli(prob["density"])
+    levels = gmg.hierarchy(shape, ke, mod, prob["fixed"], (4, shape))
+    lam = [gmg.spectrum(l) for l in levels[:-1]]
+    f = prob["forces"][:, 2]
+    x, it_k, rel = gmg.pcg(levels, lam, f, precondition=lambda r: gmg.kcycle(levels, lam, r, 2))
+    _, it_v, _ = gmg.pcg(levels, lam, f)
+    pts, conn, dofs = hex8.grid(shape)
+    ref, _ = system.solve(dofs, conn, ke, mod, prob["fixed"], f, len(pts))
+    assert rel < 1e-8 and it_k < it_v
     assert np.linalg.norm(x - ref[:, 0]) < 1e-7 * np.linalg.norm(ref)
 
 
