@@ -561,32 +561,45 @@ void krylov_scalars(double* ks, int k, const int* active, int step, hipStream_t 
   DFMI_HIP(hipGetLastError());
 }
 
-// y -= f[r] x over interleaved level vectors (columns with f = 0 or inactive are left alone)
+// y -= f[r] x over interleaved level vectors, one thread per (node, column): the node's three values
+// (columns with f = 0 or inactive are left alone)
 template <typename T>
-__global__ void k_level_axpy(T* __restrict__ y, const T* __restrict__ x, const double* f, long long n, int k,
+__global__ void k_level_axpy(T* __restrict__ y, const T* __restrict__ x, const double* f, int slots, int k,
                              const int* active) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int r = (int)((e / 3) % k);
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= slots) return;
+  const int r = e % k;
   if (active && !active[r]) return;
-  y[e] -= (T)f[r] * x[e];
+  const T fr = (T)f[r];
+  const long long o = 3LL * e;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[o + c] -= fr * x[o + c];
 }
 
-// x = a[r] x + b[r] p over interleaved level vectors (x is only read where a[r] != 0)
+// x = a[r] x + b[r] p over interleaved level vectors, one thread per (node, column) (x is only read where
+// a[r] != 0)
 template <typename T>
 __global__ void k_level_combine(T* __restrict__ x, const T* __restrict__ p, const double* a, const double* b,
-                                long long n, int k, const int* active) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int r = (int)((e / 3) % k);
+                                int slots, int k, const int* active) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= slots) return;
+  const int r = e % k;
   if (active && !active[r]) return;
   const double ca = a[r];
-  x[e] = (ca != 0.0 ? (T)ca * x[e] : T(0)) + (T)b[r] * p[e];
+  const T cb = (T)b[r];
+  const long long o = 3LL * e;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) x[o + c] = (ca != 0.0 ? (T)ca * x[o + c] : T(0)) + cb * p[o + c];
+}
+
+static int level_slots(long long nodes, int k) {
+  if (nodes * (long long)k > (long long)INT32_MAX) throw std::runtime_error("level vectors too large");
+  return (int)(nodes * k);
 }
 
 template <typename T>
 void level_axpy(T* y, const T* x, const double* f, long long nodes, int k, const int* active, hipStream_t s) {
-  const long long n = nodes * 3 * k;
+  const int n = level_slots(nodes, k);
   hipLaunchKernelGGL(k_level_axpy<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, x, f, n, k, active);
   DFMI_HIP(hipGetLastError());
 }
@@ -596,7 +609,7 @@ template void level_axpy<double>(double*, const double*, const double*, long lon
 template <typename T>
 void level_combine(T* x, const T* p, const double* a, const double* b, long long nodes, int k, const int* active,
                    hipStream_t s) {
-  const long long n = nodes * 3 * k;
+  const int n = level_slots(nodes, k);
   hipLaunchKernelGGL(k_level_combine<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, p, a, b, n, k,
                      active);
   DFMI_HIP(hipGetLastError());

@@ -49,8 +49,9 @@ def test_vcycle_matches_oracle(shape, spacing, precision, torch_gpu):
     print(f"vcycle shape={shape} precision={precision}", flush=True)
     smoother = "matrix" if precision.endswith("matrix") else "vector"
     mg, entry, levels, fixed = case(shape, spacing, 1, precision.split("-")[0], torch_gpu, smoother)
-    # 2 columns: one column per stencil thread; 7 columns: column pairs per stencil thread and an odd last one
-    for cols in (2, 7):
+    # 2 columns: one column per stencil thread; 7 columns: column pairs per stencil thread and an odd last one;
+    # 20 columns: the LDS-tiled stencil form in two column chunks
+    for cols in (2, 7, 20):
         r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
         ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c]) for c in range(cols)])
         z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
@@ -65,12 +66,12 @@ def test_vcycle_matches_oracle(shape, spacing, precision, torch_gpu):
 @pytest.mark.parametrize("steps", [1, 2])
 def test_kcycle_matches_oracle(shape, spacing, precision, steps, torch_gpu):
     """Fine V-cycle whose level-1 correction is solved by one or two flexible-CG steps (coarse_krylov)
-    against oracle.gmg.kcycle: FP64 to 1e-9, FP32 to 1e-4; 1, 2 and 7 columns (single-column layout,
-    column pairs, an odd last column)."""
+    against oracle.gmg.kcycle: FP64 to 1e-9, FP32 to 1e-4; 1, 2, 7 and 20 columns (single-column layout,
+    column pairs, an odd last column, the LDS-tiled stencil form)."""
     print(f"kcycle shape={shape} precision={precision} steps={steps}", flush=True)
     mg, entry, levels, fixed = case(shape, spacing, 1, precision, torch_gpu, coarse_krylov=steps)
     assert len(levels) >= 3
-    for cols in (1, 2, 7):
+    for cols in (1, 2, 7, 20):
         r = np.random.default_rng(3).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
         ref = np.column_stack([gmg.kcycle(levels, entry["lambda_max"], r[:, c], steps) for c in range(cols)])
         z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
