@@ -102,7 +102,9 @@ struct ApplyArgs {
   const void* dinv;   // [nodes][3] inverse diagonal (masked), CHEB only
   double c1, c2;      // Chebyshev coefficients, CHEB only
   double c0;          // first Chebyshev iterate x1 = c0 Dinv b, CHEB0 / CHEB1 only
-  double* partial;    // [k][nblocks] partial dot products, DOT only
+  double* partial;    // [k][nblocks] partial dot products: x.y for DOT; b.y for FP32 fine CHEB when not null
+  const double* beta; // FP32 fine DOT with pout: per-column beta of the fused direction (nullptr: p = b)
+  void* pout;         // FP32 fine DOT: when not null the operand is p = b + beta x, written here for owned nodes
   const int* active;  // per-column flag (device), nullptr = all active
   int k;
   double mdiag[8];    // MASS only: unit-density element mass in the corner-Hadamard basis, divided by 64
@@ -113,6 +115,9 @@ template <typename T>
 void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlocks& kb, int mode,
                 const ApplyArgs& a, hipStream_t s, int* nblocks_out);
 int fine_blocks(const Geom& g);
+// workgroups per column of a fine_apply launch in `mode` for k columns (the stride of its partials)
+template <typename T>
+int fine_grid_blocks(const Geom& g, int mode, int k);
 // Fine level on the matrix cores (FP32 vectors, split-bf16 MFMA products): modes APPLY, RESID, CHEB,
 // CHEB0, CHEB1 on grids with at most 16 * FINE_MFMA_ZMAX elements along z.  atab: per-lane A operands
 // from fine_mfma_table (64 * 12 words, device).

@@ -148,14 +148,18 @@ struct FineCfg {
 
 // One workgroup owns a (TY-1) x (TZ-1) tile of node columns (TY * TZ <= NT element columns including
 // a one-element halo at the low side) and marches along x.  LDS is structure-of-arrays (component
-// planes), so consecutive lanes touch consecutive 4- or 8-byte words: no bank conflicts.
+// planes), so consecutive lanes touch consecutive 4- or 8-byte words: no bank conflicts.  The FP32 DOT
+// mode, which forms the fused PCG direction, is held to 80 VGPRs (six waves per SIMD, three workgroups per
+// CU).
 template <typename T, int MODE>
-__global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* __restrict__ moduli,
-                                                               const uint8_t* __restrict__ mask, KeSym<T> ks,
-                                                               ApplyArgs a, int TY, int TZ, int xchunk, int tiles_y,
-                                                               int tiles_z) {
+__global__ __launch_bounds__(FineCfg<T>::NT)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1))) void k_fine_apply(
+    Geom g, const T* __restrict__ moduli, const uint8_t* __restrict__ mask, KeSym<T> ks, ApplyArgs a, int TY, int TZ,
+    int xchunk, int tiles_y, int tiles_z) {
   constexpr int NT = FineCfg<T>::NT;
   constexpr int MAXNP = FineCfg<T>::MAXNP;
+  // the FP32 PCG's fusions (direction formed inside DOT, r.z partials from the last CHEB step)
+  constexpr bool kFusedPcg = sizeof(T) == sizeof(float);
   __shared__ T U[2][3][MAXNP];  // node planes i and i+1 of the operand (masked)
   __shared__ T OUT[3][3][NT];   // face corner results m = 1..3 (dy, dz, dy dz) at plane i
   __shared__ double red[NT / 64];
@@ -214,6 +218,18 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
           v0 = c0 * ds[0] * bs[0];
           v1 = c0 * ds[1] * bs[1];
           v2 = c0 * ds[2] * bs[2];
+        } else if (MODE == MODE_DOT && kFusedPcg && a.pout) {  // fused PCG direction: p = z + beta p_old
+          const T* zs = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * n;
+          v0 = zs[0];
+          v1 = zs[1];
+          v2 = zs[2];
+          if (a.beta) {
+            const T bt = (T)a.beta[r];
+            const T* src = X + 3LL * n;
+            v0 += bt * src[0];
+            v1 += bt * src[1];
+            v2 += bt * src[2];
+          }
         } else {
           const T* src = X + 3LL * n;
           v0 = src[0];
@@ -375,13 +391,16 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
           } else if (MODE == MODE_DOT) {
             const T yv = on ? s0[c] : T(0);
             Y[3LL * nn + c] = yv;
+            if (kFusedPcg && a.pout) reinterpret_cast<T*>(a.pout)[(long long)r * col + 3LL * nn + c] = xs[c];
             dot += (double)xs[c] * (double)yv;
           } else if (MODE == MODE_RESID) {
             Y[3LL * nn + c] = on ? bv[c] - s0[c] : T(0);
           } else if (MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
             const T c1 = (T)a.c1, c2 = (T)a.c2;
             const T prev = MODE == MODE_CHEB1 ? (T)a.c0 * dv[c] * bv[c] : xpv[c];
-            Y[3LL * nn + c] = on ? xs[c] + c1 * (xs[c] - prev) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
+            const T yv = on ? xs[c] + c1 * (xs[c] - prev) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
+            Y[3LL * nn + c] = yv;
+            if (MODE == MODE_CHEB && kFusedPcg && a.partial) dot += (double)bv[c] * (double)yv;
           } else if (MODE == MODE_MASS) {
             const T lump = a.dinv ? reinterpret_cast<const T*>(a.dinv)[3LL * nn + c] : T(0);
             Y[3LL * nn + c] = on ? s0[c] + lump * xs[c] : T(0);
@@ -392,7 +411,7 @@ __global__ __launch_bounds__(FineCfg<T>::NT) void k_fine_apply(Geom g, const T* 
     e_cur = e_next;
     (void)mk_next;
   }
-  if (MODE == MODE_DOT) {
+  if (MODE == MODE_DOT || (MODE == MODE_CHEB && kFusedPcg && a.partial)) {  // partials of x.y (DOT), b.y (CHEB)
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((t & 63) == 0) red[t >> 6] = dot;
     __syncthreads();
@@ -430,6 +449,14 @@ static Tiling choose_tiling(const Geom& g, int k, int nt) {
   best.chunks = (g.nx + best.xchunk - 1) / best.xchunk;
   return best;
 }
+
+template <typename T>
+int fine_grid_blocks(const Geom& g, int mode, int k) {
+  const Tiling tl = choose_tiling(g, mode == MODE_DOT ? 1 : k, FineCfg<T>::NT);
+  return tl.tiles_y * tl.tiles_z * tl.chunks;
+}
+template int fine_grid_blocks<float>(const Geom&, int, int);
+template int fine_grid_blocks<double>(const Geom&, int, int);
 
 int fine_blocks(const Geom& g) {
   Tiling a = choose_tiling(g, 1, FineCfg<float>::NT), b = choose_tiling(g, 1, FineCfg<double>::NT);

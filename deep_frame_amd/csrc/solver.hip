@@ -193,6 +193,10 @@ struct VArgs {
   Hierarchy* H;
   int k;
   const int* active;
+  // when set, the fine level's last post-smoothing step also writes per-workgroup partials of b.x (the
+  // V-cycle's input dotted with its output), k x (*final_blocks) values
+  double* final_dot = nullptr;
+  int* final_blocks = nullptr;
 };
 
 template <typename T>
@@ -247,7 +251,7 @@ static bool use_mfma(Handle* h, const Geom& g, int mode) {
 }
 
 template <typename T>
-static void level_apply(VArgs& v, int idx, int mode, ApplyArgs& a) {
+static void level_apply(VArgs& v, int idx, int mode, ApplyArgs& a, int* nblocks = nullptr) {
   Level& L = *v.H->levels[idx];
   a.k = v.k;
   a.active = v.active;
@@ -255,13 +259,20 @@ static void level_apply(VArgs& v, int idx, int mode, ApplyArgs& a) {
     fine_apply_mfma(L.g, v.h->m32.as<float>(), L.mask.as<uint8_t>(), v.h->atab.as<unsigned>(), mode, a,
                     fine_tile(v.h, v.k), v.h->stream);
   else if (idx == 0)
-    fine_apply<T>(L.g, level_moduli<T>(v.h), L.mask.as<uint8_t>(), v.h->kb, mode, a, v.h->stream, nullptr);
+    fine_apply<T>(L.g, level_moduli<T>(v.h), L.mask.as<uint8_t>(), v.h->kb, mode, a, v.h->stream, nblocks);
   else
     stencil_apply<T>(L.g, level_stencil<T>(L), L.mask.as<uint8_t>(), mode, a, interleaved(v.k), v.h->stream, nullptr);
 }
 
 // vector layout of a level inside the V-cycle: column-major on the fine level, interleaved below
 static VLay level_layout(const Level& L, int idx, int k) { return idx == 0 ? column_major(L.g) : interleaved(k); }
+
+// Requests the b.x partials on the fine level's last post-smoothing step when the caller asked for them
+// (VArgs::final_dot) and the step runs on the vector kernel, which forms them.
+static void final_dot(VArgs& v, int idx, bool last, ApplyArgs& a) {
+  if (!v.final_dot || idx != 0 || !last || use_mfma(v.h, v.H->levels[0]->g, MODE_CHEB)) return;
+  a.partial = v.final_dot;
+}
 
 // Chebyshev smoothing of A x = b (topology_multigrid.py:284-305) in the three-term form
 // x_{j+1} = x_j + rho_j rho_{j-1} (x_j - x_{j-1}) + (2 rho_j / delta) Dinv (b - A x_j).
@@ -319,7 +330,8 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
     a.dinv = dinv;
     a.c1 = 0.0;
     a.c2 = 1.0 / theta;
-    level_apply<T>(v, idx, MODE_CHEB, a);
+    final_dot(v, idx, count == 1, a);
+    level_apply<T>(v, idx, MODE_CHEB, a, v.final_blocks);
     std::swap(cur, other);
     prev_zero = false;
   }
@@ -333,7 +345,8 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
     a.dinv = dinv;
     a.c1 = nr * rho;
     a.c2 = 2.0 * nr / delta;
-    level_apply<T>(v, idx, MODE_CHEB, a);
+    final_dot(v, idx, !from_zero && j == count - 1, a);
+    level_apply<T>(v, idx, MODE_CHEB, a, v.final_blocks);
     std::swap(cur, other);
     prev_zero = false;
     rho = nr;
@@ -984,6 +997,7 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
   float* d32 = r32 + kn;
   float* p32 = h->zn.as<float>();
   float* ap32 = p32 + kn;
+  float* pn32 = reinterpret_cast<float*>(h->p.as<double>());  // next direction (formed inside the Krylov apply)
   double* sc = h->scal.as<double>();
   const size_t kc = (size_t)h->k_cap;
   double* alpha = sc;
@@ -1002,6 +1016,26 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
   VArgs v{h, &H, k, act};
   const int nbr = reduce_blocks(n);
   double* part = h->partial.as<double>();
+  // r.z from the V-cycle's last fine smoothing step (vector kernel) instead of a separate pass
+  const bool fused_rz = !use_mfma(h, F.g, MODE_CHEB);
+  int nbc = 0;
+  if (fused_rz) {
+    if ((size_t)fine_grid_blocks<float>(F.g, MODE_CHEB, k) * k * sizeof(double) > h->partial.bytes)
+      throw std::logic_error("partial-sum buffer too small for the fused r.z");
+    v.final_dot = part;
+    v.final_blocks = &nbc;
+  }
+  auto precondition_rz = [&](double* out) {  // z = V(r32), out = r32.z per column
+    vcycle<float>(v, 0, r32);
+    z = F.x.as<float>();
+    if (fused_rz) {
+      reduce_partials(part, nbc, k, out, s);
+    } else {
+      column_dot<float>(r32, z, n, k, part, nbr, act, s);
+      reduce_partials(part, nbr, k, out, s);
+    }
+  };
+  bool first = true;  // the first direction is z itself
 
   flat_to_grid(h->pad, h->ux, h->uy, h->uz, rhs_flat, b, k, s);
   mask_vector(F.mask.as<uint8_t>(), b, h->pad.nodes(), k, s);
@@ -1032,22 +1066,21 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
   upload(act, active);
   to_work<float>(b, scale, r32, n, k, act, s);
   int step = 0;
-  if (count(active)) {
-    vcycle<float>(v, 0, r32);
-    z = F.x.as<float>();
-    copy_columns<float>(z, p32, n, k, act, s);
-    column_dot<float>(r32, z, n, k, part, nbr, act, s);
-    reduce_partials(part, nbr, k, rz, s);
-  }
+  if (count(active)) precondition_rz(rz);
   while (count(active) && step < h->st.max_iterations) {
     int nbf = 0;
     ApplyArgs a{};
-    a.x = p32;
+    a.x = p32;  // previous direction (not read for the first one)
+    a.b = z;
+    a.beta = first ? nullptr : beta;
+    a.pout = pn32;  // p = z + beta p_prev, formed while staging the operand
     a.y = ap32;
     a.k = k;
     a.active = act;
     a.partial = part;
     fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_DOT, a, s, &nbf);
+    std::swap(p32, pn32);
+    first = false;
     reduce_partials(part, nbf, k, pap, s);
     pcg_scalars_alpha(rz, pap, alpha, k, act, s);
     // d += alpha p, r -= alpha Ap, |r|^2 and r_new . z_old in one pass
@@ -1103,12 +1136,8 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
       upload(act, active);
     }
     if (!count(active)) break;
-    vcycle<float>(v, 0, r32);
-    z = F.x.as<float>();
-    column_dot<float>(r32, z, n, k, part, nbr, act, s);
-    reduce_partials(part, nbr, k, rzn, s);
-    pcg_beta(rzn, rzc, rz, beta, k, act, s);
-    pcg_direction<float>(p32, z, beta, n, k, act, s);
+    precondition_rz(rzn);
+    pcg_beta(rzn, rzc, rz, beta, k, act, s);  // the direction itself is formed by the next Krylov apply
   }
   if (count(active)) {  // iteration limit: flush and report the true residual
     upload(act, active);
