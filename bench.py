@@ -7,7 +7,8 @@ and the 42 unit-wrench right-hand sides already resident in HBM.
 
 The mixed solver is FP32 PCG with reliable FP64 updates around an FP32 V-cycle
 whose level-1 correction is two flexible-CG steps preconditioned by the level-1
-V-cycle (coarse_krylov 2), with Chebyshev degree 3 on every level (growth 1.0);
+V-cycle (coarse_krylov 2), with Chebyshev degree 3 on every level (growth 1.0)
+and reliable updates after a 1e-4 drop of the FP32 residual (inner_tolerance);
 the FP64 solve is the reference's scheme (FP64 V-cycle, degree 3 * 2^level).
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -66,7 +67,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--scheme", default="refine", choices=["refine", "flexible"],
                     help="mixed solver: FP32 PCG inside FP64 iterative refinement, or FP64 PCG with an FP32 V-cycle")
-    ap.add_argument("--inner-tolerance", type=float, default=None, help="refine scheme: inner relative tolerance")
+    ap.add_argument("--inner-tolerance", type=float, default=1e-4,
+                    help="refine scheme: FP32 residual drop between reliable FP64 updates")
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--coarse-krylov", type=int, default=2, choices=[0, 1, 2],
                     help="mixed solver: flexible-CG steps of the level-1 solve inside the fine V-cycle (0: V-cycle)")
@@ -86,18 +88,19 @@ def cpu_baseline(seconds_budget=30.0):
     import numpy as np
     from deep_frame_amd import element, problems
     from oracle import hex8, system
-    shape = (40, 40, 8)  # about 20 s of single-core SuperLU work; fill-in grows ~N^2 beyond this
+    from threadpoolctl import threadpool_limits
+    shape = (48, 48, 8)  # 10-40 s of single-core SuperLU work depending on the host; fill-in grows ~N^2
     prob = problems.frame(shape, seed=0, rhs=42)
     ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
     mod, _ = problems.moduli(prob["density"])
     pts, conn, dofs = hex8.grid(shape)
-    t0 = time.perf_counter()
-    u, rel = system.solve(dofs, conn, ke, mod, prob["fixed"], prob["forces"], len(pts))
-    dt = time.perf_counter() - t0
+    with threadpool_limits(limits=1):  # one core: SuperLU itself is serial; keep its BLAS calls serial too
+        t0 = time.perf_counter()
+        u, rel = system.solve(dofs, conn, ke, mod, prob["fixed"], prob["forces"], len(pts))
+        dt = time.perf_counter() - t0
     return {"value": dt, "unit": "s", "cores": 1, "kind": "port",
             "sample": f"SuperLU (MMD_AT_PLUS_A) assemble+factor+42-RHS solve, frame {shape[0]}x{shape[1]}x{shape[2]} "
-                      f"({3 * len(pts)} DOF), max rel-res {float(np.max(rel)):.1e}, OMP/BLAS threads "
-                      f"{os.environ.get('OMP_NUM_THREADS', 'default')}"}
+                      f"({3 * len(pts)} DOF), max rel-res {float(np.max(rel)):.1e}, BLAS limited to one thread"}
 
 
 def main():
@@ -224,7 +227,7 @@ def main():
                    "mixed_s": t_mixed, "fp64_s": t_fp64, "iterations": max(r[0] for r in reports),
                    "max_rel_res": max(r[1] for r in reports), "fine_smoother": args.fine_smoother,
                    "coarse_krylov": args.coarse_krylov, "growth": args.growth,
-                   "chebyshev_degree": args.degree, "fine_tile": list(tile[:2]) if tile else None},
+                   "chebyshev_degree": args.degree, "inner_tolerance": args.inner_tolerance, "fine_tile": list(tile[:2]) if tile else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, main_precision),
                      "traffic_source": str(TRAFFIC[main_precision].relative_to(ROOT)),

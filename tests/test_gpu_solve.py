@@ -14,13 +14,16 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 # solver schemes: FP64 V-cycle, FP32 V-cycle inside FP64 PCG, FP32 PCG with reliable FP64 updates; the FP32
 # V-cycle's fine level runs on the FP32 vector units, or on the matrix cores (split bf16) for "-matrix";
-# "-krylov": the fine level's coarse-grid correction solves level 1 by two flexible-CG steps (K-cycle)
+# "-krylov": the fine level's coarse-grid correction solves level 1 by two flexible-CG steps (K-cycle);
+# "refine-bench": the benchmark's mixed configuration (K-cycle, degree 3 on every level, 1e-4 inner drop)
 MODES = {"float64": {"precision": "float64"}, "float32": {"precision": "float32"},
          "refine": {"precision": "float32", "refine": True},
          "float32-matrix": {"precision": "float32", "fine_smoother": "matrix"},
          "refine-matrix": {"precision": "float32", "refine": True, "fine_smoother": "matrix"},
          "float64-krylov": {"precision": "float64", "coarse_krylov": 2},
-         "refine-krylov": {"precision": "float32", "refine": True, "coarse_krylov": 2}}
+         "refine-krylov": {"precision": "float32", "refine": True, "coarse_krylov": 2},
+         "refine-bench": {"precision": "float32", "refine": True, "coarse_krylov": 2, "growth": 1.0,
+                          "inner_tolerance": 1e-4}}
 
 
 def make(shape, spacing, ke, moduli, settings=None):
@@ -99,7 +102,7 @@ def test_covariance_cantilever_dense_flexibility(torch_gpu):
     mg.close()
 
 
-@pytest.mark.parametrize("mode", ["float32", "refine", "refine-matrix"])
+@pytest.mark.parametrize("mode", ["float32", "refine", "refine-matrix", "refine-bench"])
 def test_cantilever_config_compliance_and_sensitivity(mode, torch_gpu):
     """48 x 24 x 24 cantilever: compliance 1e-6 and sensitivity 1e-5 against the FP64 oracle."""
     prob = problems.cantilever(seed=0)
@@ -167,7 +170,7 @@ def test_kcycle_solves_frame_wrenches_in_fewer_iterations(torch_gpu):
     torch = torch_gpu
     rhs = torch.as_tensor(np.ascontiguousarray(prob["forces"].T), device="cuda")
     out, its = {}, {}
-    for mode in ("float64", "float64-krylov", "refine", "refine-krylov"):
+    for mode in ("float64", "float64-krylov", "refine", "refine-krylov", "refine-bench"):
         mg = make((32, 32, 8), (1, 1, 1), ke, mod, MODES[mode])
         assert mg.level_count == 4
         x, reps = mg.solve_device(b"f", prob["fixed"], rhs)
@@ -177,7 +180,7 @@ def test_kcycle_solves_frame_wrenches_in_fewer_iterations(torch_gpu):
         out[mode] = x.cpu().numpy()
         mg.close()
     assert its["float64-krylov"] < its["float64"] and its["refine-krylov"] < its["refine"]
-    for mode in ("float64-krylov", "refine", "refine-krylov"):
+    for mode in ("float64-krylov", "refine", "refine-krylov", "refine-bench"):
         for c in range(42):
             a, b = out["float64"][c], out[mode][c]
             assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
