@@ -473,7 +473,8 @@ static void vcycle(VArgs& v, int idx, const T* b) {
     restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
                      level_layout(C, idx + 1, v.k), v.k, v.active, s);
   }
-  if (idx == 0 && v.h->st.coarse_krylov > 0 && H.levels.size() > 2)
+  // (slots with a free rigid kernel keep the V-cycle: their level-1 operator is singular)
+  if (idx == 0 && v.h->st.coarse_krylov > 0 && H.levels.size() > 2 && H.nk == 0)
     krylov_coarse<T>(v, C.rhs.as<T>());
   else
     vcycle<T>(v, idx + 1, C.rhs.as<T>());

@@ -54,7 +54,7 @@ typedef struct dfmi_settings {
                                 correction solves the level-1 system by that many flexible-CG steps, each
                                 preconditioned by the level-1 V-cycle, instead of one level-1 V-cycle (a
                                 Krylov "K-cycle"; needs three or more levels, and a flexible outer PCG, which
-                                both solve schemes are) */
+                                both solve schemes are; hierarchies with a free rigid kernel keep the V-cycle) */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */
