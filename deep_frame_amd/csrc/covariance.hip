@@ -1,7 +1,8 @@
 // Interface load-covariance measures (deep_frame/topology_problem.py:573-684) on device-resident
 // unit-wrench solutions: the unit flexibility F_w = W^T U (all pairs of load and solution columns)
 // and the combined displacement fields U M (load cases, eigen-directions) whose element energies
-// give the gradients.  FP64 throughout; HBM-bound streams over the k columns.
+// give the gradients.  FP64 throughout; HBM-bound streams over the k columns (the Gram on the FP64 matrix
+// cores for up to 48 x 48 outputs).
 #include <stdexcept>
 
 #include "dfmi_internal.h"
@@ -86,6 +87,104 @@ __global__ __launch_bounds__(GRAM_NT) void k_gram(const double* __restrict__ a, 
   }
 }
 
+// The same partial sums on the FP64 matrix cores, for ka, kb <= 48: the output is cut into 16 x 16 tiles
+// (TI x TJ of them) and every wave accumulates all tiles over its 16-DOF blocks with
+// v_mfma_f64_16x16x4f64.  In a 4-DOF step lane l supplies A[i][k] = a[16 ti + l % 16][d_k] and
+// B[k][j] = b[16 tj + l % 16][d_k] with k = l / 16; a lane reads four consecutive DOFs of its columns once
+// and feeds them to four steps, so the DOFs of a block are visited in a permuted but common order, which a
+// sum over DOFs does not see.  Each lane then holds C[l / 16 + 4 v][l % 16], v = 0..3, of every tile.
+// The four waves of a workgroup are added in a fixed order through LDS, so results are deterministic.
+typedef double gram_d4 __attribute__((ext_vector_type(4)));
+
+template <int TI, int TJ>
+__global__ __launch_bounds__(256) void k_gram_mfma(const double* __restrict__ a, const double* __restrict__ b,
+                                                   long long n, int ka, int kb, double* __restrict__ partial) {
+  __shared__ double red[TI * TJ * 4][64];
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int c16 = l & 15, kq = l >> 4;
+  long long oa[TI], ob[TJ];  // this lane's operand columns (clamped: extra columns only feed unused outputs)
+#pragma unroll
+  for (int t = 0; t < TI; ++t) oa[t] = (long long)min(16 * t + c16, ka - 1) * n;
+#pragma unroll
+  for (int t = 0; t < TJ; ++t) ob[t] = (long long)min(16 * t + c16, kb - 1) * n;
+  gram_d4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = gram_d4{0.0, 0.0, 0.0, 0.0};
+  const long long blocks16 = (n + 15) / 16;
+  for (long long blk = (long long)blockIdx.x * 4 + wave; blk < blocks16; blk += (long long)gridDim.x * 4) {
+    const long long d0 = blk * 16 + 4 * kq;
+    double av[TI][4], bv[TJ][4];
+    if (blk * 16 + 16 <= n) {
+#pragma unroll
+      for (int t = 0; t < TI; ++t)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) av[t][s] = a[oa[t] + d0 + s];
+#pragma unroll
+      for (int t = 0; t < TJ; ++t)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) bv[t][s] = b[ob[t] + d0 + s];
+    } else {  // last, partial block: DOFs past n contribute zeros
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const bool in = d0 + s < n;
+#pragma unroll
+        for (int t = 0; t < TI; ++t) av[t][s] = in ? a[oa[t] + d0 + s] : 0.0;
+#pragma unroll
+        for (int t = 0; t < TJ; ++t) bv[t][s] = in ? b[ob[t] + d0 + s] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][s], bv[j][s], acc[i][j], 0, 0, 0);
+  }
+  // waves 1, 2, 3 are added into wave 0's tiles in that order
+  for (int w = 1; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) red[(i * TJ + j) * 4 + v][l] = acc[i][j][v];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[i][j][v] += red[(i * TJ + j) * 4 + v][l];
+    }
+    __syncthreads();
+  }
+  if (wave != 0) return;
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * i + kq + 4 * v, c = 16 * j + c16;
+        if (r < ka && c < kb) partial[(long long)(r * kb + c) * gridDim.x + blockIdx.x] = acc[i][j][v];
+      }
+}
+
+template <int TI>
+static void launch_gram_mfma(int tj, int blocks, hipStream_t s, const double* a, const double* b, long long n,
+                             int ka, int kb, double* work) {
+  switch (tj) {
+    case 1: hipLaunchKernelGGL((k_gram_mfma<TI, 1>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
+    case 2: hipLaunchKernelGGL((k_gram_mfma<TI, 2>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
+    default: hipLaunchKernelGGL((k_gram_mfma<TI, 3>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
+  }
+}
+
 // out[e] = scale * (sum over blocks of partial[e][block]): one workgroup per element, fixed-order tree
 __global__ __launch_bounds__(256) void k_sum_blocks(const double* __restrict__ partial, int nblocks, int count,
                                                    double scale, double* __restrict__ out) {
@@ -143,7 +242,17 @@ void gram_device(const double* a, const double* b, long long n, int ka, int kb, 
   if (work_blocks < 1) throw std::invalid_argument("gram: work_blocks must be >= 1");
   const long long chunks = (n + GRAM_CH - 1) / GRAM_CH;
   const int blocks = (int)std::min<long long>(chunks, work_blocks);
-  hipLaunchKernelGGL(k_gram, dim3(blocks), dim3(GRAM_NT), 0, s, a, b, n, ka, kb, work);
+  if (ka <= 48 && kb <= 48) {  // matrix cores: 16 x 16 output tiles, 64 DOFs per workgroup pass
+    const int ti = (ka + 15) / 16, tj = (kb + 15) / 16;
+    if (ti == 1)
+      launch_gram_mfma<1>(tj, blocks, s, a, b, n, ka, kb, work);
+    else if (ti == 2)
+      launch_gram_mfma<2>(tj, blocks, s, a, b, n, ka, kb, work);
+    else
+      launch_gram_mfma<3>(tj, blocks, s, a, b, n, ka, kb, work);
+  } else {
+    hipLaunchKernelGGL(k_gram, dim3(blocks), dim3(GRAM_NT), 0, s, a, b, n, ka, kb, work);
+  }
   DFMI_HIP(hipGetLastError());
   const int count = ka * kb;
   hipLaunchKernelGGL(k_sum_blocks, dim3(count), dim3(256), 0, s, work, blocks, count, scale, out);

@@ -30,10 +30,13 @@ def test_gram_and_combine_match_numpy(torch_gpu):
     assert np.max(np.abs(y - m.T @ b)) < 1e-12 * np.max(np.abs(m.T @ b))
 
 
-@pytest.mark.parametrize("ka,kb,n", [(1, 1, 300), (8, 8, 100003), (7, 42, 777), (64, 3, 4099), (64, 64, 20000)])
+@pytest.mark.parametrize("ka,kb,n", [(1, 1, 300), (8, 8, 100003), (7, 42, 777), (64, 3, 4099), (64, 64, 20000),
+                                     (3, 17, 15), (16, 16, 16), (17, 33, 1001), (48, 48, 65539), (42, 42, 9999),
+                                     (33, 5, 4096)])
 def test_gram_shapes_and_accumulating_combine(torch_gpu, ka, kb, n):
-    """dfmi_gram over the tile/lane splits (one thread per tile up to 64 lanes per tile) and dfmi_combine
-    with accumulate = 1, against NumPy."""
+    """dfmi_gram against NumPy: on the matrix cores for ka, kb <= 48 (one to three 16-wide tiles per side,
+    DOF counts below, at and off a multiple of the 16-DOF block), and on the vector units beyond (tile/lane
+    splits from one thread per tile up to 64 lanes per tile); dfmi_combine with accumulate = 1."""
     from deep_frame_amd import _native
     torch = torch_gpu
     rng = np.random.default_rng(ka * 100 + kb)
