@@ -233,6 +233,56 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ u, l
   }
 }
 
+// Y[m0 + j][n] (+)= sum_k U[k][n] M[k][m0 + j] on the FP64 matrix cores, j < 16 TJ: every wave takes 16-DOF
+// blocks and forms the transposed tile Y^T = M^T U^T: in a step over four k lane l supplies
+// A[i][kk] = M[k0 + kk][m0 + 16 t + i] and B[kk][d] = U[k0 + kk][d0 + d] with i = d = l % 16, kk = l / 16, so
+// each entry of U is read by exactly one lane once; afterwards lane l holds output column 16 t + l / 16 + 4 v
+// at DOF d0 + l % 16, so the sixteen lanes of a column store one contiguous run.
+template <int TJ>
+__global__ __launch_bounds__(256) void k_combine_mfma(const double* __restrict__ u, long long n, int k,
+                                                      const double* __restrict__ mat, int mcols, int m0,
+                                                      double* __restrict__ y, int accumulate) {
+  __shared__ double Ms[GRAM_MAXK][16 * TJ];
+  for (int v = threadIdx.x; v < GRAM_MAXK * 16 * TJ; v += blockDim.x) {
+    const int r = v / (16 * TJ), c = v - r * (16 * TJ);
+    Ms[r][c] = (r < k && m0 + c < mcols) ? mat[(long long)r * mcols + m0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int c16 = l & 15, kq = l >> 4;
+  const long long blocks16 = (n + 15) / 16;
+  for (long long blk = (long long)blockIdx.x * 4 + wave; blk < blocks16; blk += (long long)gridDim.x * 4) {
+    const long long d0 = blk * 16;
+    const bool in = d0 + c16 < n;
+    gram_d4 acc[TJ];
+#pragma unroll
+    for (int t = 0; t < TJ; ++t) acc[t] = gram_d4{0.0, 0.0, 0.0, 0.0};
+    double av[GRAM_MAXK / 4];  // all of this lane's U entries of the block, requested before the products
+#pragma unroll
+    for (int q = 0; q < GRAM_MAXK / 4; ++q) {
+      const int kk = 4 * q + kq;
+      av[q] = (in && kk < k) ? u[(long long)kk * n + d0 + c16] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < GRAM_MAXK / 4; ++q) {
+      if (4 * q >= k) break;
+#pragma unroll
+      for (int t = 0; t < TJ; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ms[4 * q + kq][16 * t + c16], av[q], acc[t], 0, 0, 0);
+    }
+    if (!in) continue;
+#pragma unroll
+    for (int t = 0; t < TJ; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = m0 + 16 * t + kq + 4 * v;
+        if (col >= mcols) continue;
+        double* out = y + (long long)col * n + d0 + c16;
+        *out = accumulate ? *out + acc[t][v] : acc[t][v];
+      }
+  }
+}
+
 void gram_device(const double* a, const double* b, long long n, int ka, int kb, double* out, double* work,
                  int work_blocks, double scale, hipStream_t s) {
   if (n < 1 || ka < 1 || kb < 1 || ka > GRAM_MAXK || kb > GRAM_MAXK)
@@ -263,8 +313,23 @@ void combine_device(const double* u, long long n, int k, const double* m, int mc
                     hipStream_t s) {
   if (n < 1 || k < 1 || k > GRAM_MAXK || mcols < 1) throw std::invalid_argument("combine: need n >= 1, 1 <= k <= 64");
   const int blocks = (int)std::min<long long>((n + 255) / 256, 4096);
-  for (int m0 = 0; m0 < mcols; m0 += COMB_M) {
-    hipLaunchKernelGGL(k_combine, dim3(blocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate);
+  if (k < 32 || mcols <= 16) {  // few input or output columns: the vector kernel streams U and Y faster
+    for (int m0 = 0; m0 < mcols; m0 += COMB_M) {
+      hipLaunchKernelGGL(k_combine, dim3(blocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate);
+      DFMI_HIP(hipGetLastError());
+    }
+    return;
+  }
+  // matrix cores: up to 64 output columns per pass over U (16-DOF blocks, four waves per workgroup)
+  const int mblocks = (int)std::min<long long>((n + 63) / 64, 4096);
+  for (int m0 = 0; m0 < mcols; m0 += 64) {
+    const int tj = std::min(4, (mcols - m0 + 15) / 16);
+    switch (tj) {
+      case 1: hipLaunchKernelGGL(k_combine_mfma<1>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+      case 2: hipLaunchKernelGGL(k_combine_mfma<2>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+      case 3: hipLaunchKernelGGL(k_combine_mfma<3>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+      default: hipLaunchKernelGGL(k_combine_mfma<4>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+    }
     DFMI_HIP(hipGetLastError());
   }
 }

@@ -25,7 +25,7 @@ def test_gram_and_combine_match_numpy(torch_gpu):
     ta, tb = torch.as_tensor(a, device="cuda"), torch.as_tensor(b, device="cuda")
     gram = cov.unit_flexibility(tb, ta)
     assert np.max(np.abs(gram - a @ b.T)) < 1e-10 * np.max(np.abs(a @ b.T))
-    m = rng.normal(size=(k, 19))
+    m = rng.normal(size=(k, 70))  # two passes over the columns: 64 outputs, then 6
     y = cov.fields(tb, m).cpu().numpy()
     assert np.max(np.abs(y - m.T @ b)) < 1e-12 * np.max(np.abs(m.T @ b))
 
