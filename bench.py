@@ -73,6 +73,7 @@ def parse():
     ap.add_argument("--coarse-krylov", type=int, default=2, choices=[0, 1, 2],
                     help="mixed solver: flexible-CG steps of the level-1 solve inside the fine V-cycle (0: V-cycle)")
     ap.add_argument("--degree", type=int, default=3, help="mixed solver: Chebyshev degree on the fine level")
+    ap.add_argument("--block", action="store_true", help="mixed solver: one block PCG recurrence over the columns")
     ap.add_argument("--growth", type=float, default=1.0,
                     help="mixed solver: Chebyshev degree growth per level (degree * growth^level)")
     ap.add_argument("--fine-smoother", default="vector", choices=["matrix", "vector"],
@@ -184,7 +185,8 @@ def main():
         print(json.dumps({"fp64_s": float(np.mean(t64)), "iterations": max(r[0] for r in rep64)}))
         return
     mixed = {"precision": "float32", "refine": args.scheme == "refine", "fine_smoother": args.fine_smoother,
-             "coarse_krylov": args.coarse_krylov, "growth": args.growth, "chebyshev_degree": args.degree}
+             "coarse_krylov": args.coarse_krylov, "growth": args.growth, "chebyshev_degree": args.degree,
+             "block_krylov": args.block}
     if args.inner_tolerance:
         mixed["inner_tolerance"] = args.inner_tolerance
     mg, times, reports, x = run(mixed, args.steps, args.warmup, f"mixed-{args.scheme}")
@@ -227,7 +229,8 @@ def main():
                    "mixed_s": t_mixed, "fp64_s": t_fp64, "iterations": max(r[0] for r in reports),
                    "max_rel_res": max(r[1] for r in reports), "fine_smoother": args.fine_smoother,
                    "coarse_krylov": args.coarse_krylov, "growth": args.growth,
-                   "chebyshev_degree": args.degree, "inner_tolerance": args.inner_tolerance, "fine_tile": list(tile[:2]) if tile else None},
+                   "chebyshev_degree": args.degree, "inner_tolerance": args.inner_tolerance,
+                   "block_krylov": args.block, "fine_tile": list(tile[:2]) if tile else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, main_precision),
                      "traffic_source": str(TRAFFIC[main_precision].relative_to(ROOT)),

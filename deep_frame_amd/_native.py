@@ -19,7 +19,7 @@ class Settings(Structure):
                 ("power_iterations", c_int32), ("precision", c_int32), ("tolerance", c_double),
                 ("max_iterations", c_int32), ("dead_ratio", c_double), ("seed", c_int32), ("refine", c_int32),
                 ("inner_tolerance", c_double), ("fine_smoother", c_int32), ("retries", c_int32),
-                ("coarse_krylov", c_int32)]
+                ("coarse_krylov", c_int32), ("block_krylov", c_int32)]
 
 
 class Report(Structure):

@@ -96,9 +96,9 @@ __global__ __launch_bounds__(GRAM_NT) void k_gram(const double* __restrict__ a, 
 // The four waves of a workgroup are added in a fixed order through LDS, so results are deterministic.
 typedef double gram_d4 __attribute__((ext_vector_type(4)));
 
-template <int TI, int TJ>
-__global__ __launch_bounds__(256) void k_gram_mfma(const double* __restrict__ a, const double* __restrict__ b,
-                                                   long long n, int ka, int kb, double* __restrict__ partial) {
+template <int TI, int TJ, typename T>
+__global__ __launch_bounds__(256) void k_gram_mfma(const T* __restrict__ a, const T* __restrict__ b, long long n,
+                                                   int ka, int kb, double* __restrict__ partial) {
   __shared__ double red[TI * TJ * 4][64];
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int c16 = l & 15, kq = l >> 4;
@@ -120,19 +120,19 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const double* __restrict__ a,
 #pragma unroll
       for (int t = 0; t < TI; ++t)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) av[t][s] = a[oa[t] + d0 + s];
+        for (int s = 0; s < 4; ++s) av[t][s] = (double)a[oa[t] + d0 + s];
 #pragma unroll
       for (int t = 0; t < TJ; ++t)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) bv[t][s] = b[ob[t] + d0 + s];
+        for (int s = 0; s < 4; ++s) bv[t][s] = (double)b[ob[t] + d0 + s];
     } else {  // last, partial block: DOFs past n contribute zeros
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const bool in = d0 + s < n;
 #pragma unroll
-        for (int t = 0; t < TI; ++t) av[t][s] = in ? a[oa[t] + d0 + s] : 0.0;
+        for (int t = 0; t < TI; ++t) av[t][s] = in ? (double)a[oa[t] + d0 + s] : 0.0;
 #pragma unroll
-        for (int t = 0; t < TJ; ++t) bv[t][s] = in ? b[ob[t] + d0 + s] : 0.0;
+        for (int t = 0; t < TJ; ++t) bv[t][s] = in ? (double)b[ob[t] + d0 + s] : 0.0;
       }
     }
 #pragma unroll
@@ -175,14 +175,25 @@ __global__ __launch_bounds__(256) void k_gram_mfma(const double* __restrict__ a,
       }
 }
 
-template <int TI>
-static void launch_gram_mfma(int tj, int blocks, hipStream_t s, const double* a, const double* b, long long n,
-                             int ka, int kb, double* work) {
+template <int TI, typename T>
+static void launch_gram_mfma(int tj, int blocks, hipStream_t s, const T* a, const T* b, long long n, int ka, int kb,
+                             double* work) {
   switch (tj) {
-    case 1: hipLaunchKernelGGL((k_gram_mfma<TI, 1>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
-    case 2: hipLaunchKernelGGL((k_gram_mfma<TI, 2>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
-    default: hipLaunchKernelGGL((k_gram_mfma<TI, 3>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
+    case 1: hipLaunchKernelGGL((k_gram_mfma<TI, 1, T>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
+    case 2: hipLaunchKernelGGL((k_gram_mfma<TI, 2, T>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
+    default: hipLaunchKernelGGL((k_gram_mfma<TI, 3, T>), dim3(blocks), dim3(256), 0, s, a, b, n, ka, kb, work); break;
   }
+}
+
+template <typename T>
+static void gram_mfma(const T* a, const T* b, long long n, int ka, int kb, double* work, int blocks, hipStream_t s) {
+  const int ti = (ka + 15) / 16, tj = (kb + 15) / 16;
+  if (ti == 1)
+    launch_gram_mfma<1, T>(tj, blocks, s, a, b, n, ka, kb, work);
+  else if (ti == 2)
+    launch_gram_mfma<2, T>(tj, blocks, s, a, b, n, ka, kb, work);
+  else
+    launch_gram_mfma<3, T>(tj, blocks, s, a, b, n, ka, kb, work);
 }
 
 // out[e] = scale * (sum over blocks of partial[e][block]): one workgroup per element, fixed-order tree
@@ -238,10 +249,10 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ u, l
 // A[i][kk] = M[k0 + kk][m0 + 16 t + i] and B[kk][d] = U[k0 + kk][d0 + d] with i = d = l % 16, kk = l / 16, so
 // each entry of U is read by exactly one lane once; afterwards lane l holds output column 16 t + l / 16 + 4 v
 // at DOF d0 + l % 16, so the sixteen lanes of a column store one contiguous run.
-template <int TJ>
-__global__ __launch_bounds__(256) void k_combine_mfma(const double* __restrict__ u, long long n, int k,
+template <int TJ, typename T>
+__global__ __launch_bounds__(256) void k_combine_mfma(const T* __restrict__ u, long long n, int k,
                                                       const double* __restrict__ mat, int mcols, int m0,
-                                                      double* __restrict__ y, int accumulate) {
+                                                      T* __restrict__ y, int accumulate) {
   __shared__ double Ms[GRAM_MAXK][16 * TJ];
   for (int v = threadIdx.x; v < GRAM_MAXK * 16 * TJ; v += blockDim.x) {
     const int r = v / (16 * TJ), c = v - r * (16 * TJ);
@@ -261,7 +272,7 @@ __global__ __launch_bounds__(256) void k_combine_mfma(const double* __restrict__
 #pragma unroll
     for (int q = 0; q < GRAM_MAXK / 4; ++q) {
       const int kk = 4 * q + kq;
-      av[q] = (in && kk < k) ? u[(long long)kk * n + d0 + c16] : 0.0;
+      av[q] = (in && kk < k) ? (double)u[(long long)kk * n + d0 + c16] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < GRAM_MAXK / 4; ++q) {
@@ -277,10 +288,46 @@ __global__ __launch_bounds__(256) void k_combine_mfma(const double* __restrict__
       for (int v = 0; v < 4; ++v) {
         const int col = m0 + 16 * t + kq + 4 * v;
         if (col >= mcols) continue;
-        double* out = y + (long long)col * n + d0 + c16;
-        *out = accumulate ? *out + acc[t][v] : acc[t][v];
+        T* out = y + (long long)col * n + d0 + c16;
+        *out = accumulate ? (T)((double)*out + acc[t][v]) : (T)acc[t][v];
       }
   }
+}
+
+// matrix cores: up to 64 output columns per pass over U (16-DOF blocks, four waves per workgroup)
+template <typename T>
+static void combine_mfma(const T* u, long long n, int k, const double* m, int mcols, T* y, int accumulate,
+                         hipStream_t s) {
+  const int mblocks = (int)std::min<long long>((n + 63) / 64, 4096);
+  for (int m0 = 0; m0 < mcols; m0 += 64) {
+    const int tj = std::min(4, (mcols - m0 + 15) / 16);
+    switch (tj) {
+      case 1: hipLaunchKernelGGL((k_combine_mfma<1, T>), dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+      case 2: hipLaunchKernelGGL((k_combine_mfma<2, T>), dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+      case 3: hipLaunchKernelGGL((k_combine_mfma<3, T>), dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+      default: hipLaunchKernelGGL((k_combine_mfma<4, T>), dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
+    }
+    DFMI_HIP(hipGetLastError());
+  }
+}
+
+// FP32 operands, FP64 products and sums (the block Krylov solve): out = A^T B (ka, kb <= 48) and
+// Y (+)= U M (k <= 64)
+void gram32_device(const float* a, const float* b, long long n, int ka, int kb, double* out, double* work,
+                   int work_blocks, hipStream_t s) {
+  if (n < 1 || ka < 1 || kb < 1 || ka > 48 || kb > 48 || work_blocks < 1)
+    throw std::invalid_argument("gram32: need n >= 1 and 1 <= ka, kb <= 48");
+  const int blocks = (int)std::min<long long>((n + GRAM_CH - 1) / GRAM_CH, work_blocks);
+  gram_mfma<float>(a, b, n, ka, kb, work, blocks, s);
+  DFMI_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_blocks, dim3(ka * kb), dim3(256), 0, s, work, blocks, ka * kb, 1.0, out);
+  DFMI_HIP(hipGetLastError());
+}
+
+void combine32_device(const float* u, long long n, int k, const double* m, int mcols, float* y, int accumulate,
+                      hipStream_t s) {
+  if (n < 1 || k < 1 || k > GRAM_MAXK || mcols < 1) throw std::invalid_argument("combine32: need n >= 1, 1 <= k <= 64");
+  combine_mfma<float>(u, n, k, m, mcols, y, accumulate, s);
 }
 
 void gram_device(const double* a, const double* b, long long n, int ka, int kb, double* out, double* work,
@@ -293,13 +340,7 @@ void gram_device(const double* a, const double* b, long long n, int ka, int kb, 
   const long long chunks = (n + GRAM_CH - 1) / GRAM_CH;
   const int blocks = (int)std::min<long long>(chunks, work_blocks);
   if (ka <= 48 && kb <= 48) {  // matrix cores: 16 x 16 output tiles, 64 DOFs per workgroup pass
-    const int ti = (ka + 15) / 16, tj = (kb + 15) / 16;
-    if (ti == 1)
-      launch_gram_mfma<1>(tj, blocks, s, a, b, n, ka, kb, work);
-    else if (ti == 2)
-      launch_gram_mfma<2>(tj, blocks, s, a, b, n, ka, kb, work);
-    else
-      launch_gram_mfma<3>(tj, blocks, s, a, b, n, ka, kb, work);
+    gram_mfma<double>(a, b, n, ka, kb, work, blocks, s);
   } else {
     hipLaunchKernelGGL(k_gram, dim3(blocks), dim3(GRAM_NT), 0, s, a, b, n, ka, kb, work);
   }
@@ -320,18 +361,7 @@ void combine_device(const double* u, long long n, int k, const double* m, int mc
     }
     return;
   }
-  // matrix cores: up to 64 output columns per pass over U (16-DOF blocks, four waves per workgroup)
-  const int mblocks = (int)std::min<long long>((n + 63) / 64, 4096);
-  for (int m0 = 0; m0 < mcols; m0 += 64) {
-    const int tj = std::min(4, (mcols - m0 + 15) / 16);
-    switch (tj) {
-      case 1: hipLaunchKernelGGL(k_combine_mfma<1>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
-      case 2: hipLaunchKernelGGL(k_combine_mfma<2>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
-      case 3: hipLaunchKernelGGL(k_combine_mfma<3>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
-      default: hipLaunchKernelGGL(k_combine_mfma<4>, dim3(mblocks), dim3(256), 0, s, u, n, k, m, mcols, m0, y, accumulate); break;
-    }
-    DFMI_HIP(hipGetLastError());
-  }
+  combine_mfma<double>(u, n, k, m, mcols, y, accumulate, s);
 }
 
 }  // namespace dfmi

@@ -234,6 +234,10 @@ void from_work(const T* w, const double* scale, double* z, long long n, int k, c
 void gram_device(const double* a, const double* b, long long n, int ka, int kb, double* out, double* work,
                  int work_blocks, double scale, hipStream_t s);
 // y[m] (+)= sum_r m[r][m] u[r] for k (<= 64) rows u of length n and an mcols-column device matrix m
+void gram32_device(const float* a, const float* b, long long n, int ka, int kb, double* out, double* work,
+                   int work_blocks, hipStream_t s);
+void combine32_device(const float* u, long long n, int k, const double* m, int mcols, float* y, int accumulate,
+                      hipStream_t s);
 void combine_device(const double* u, long long n, int k, const double* m, int mcols, double* y, int accumulate,
                     hipStream_t s);
 // rigid-body kernel of floating fixtures (rigid.hip); coef is 6 x nk row-major over the rigid modes

@@ -111,6 +111,8 @@ struct Handle {
   // PCG workspace (padded grid, FP64)
   int k_cap = 0;
   DevBuf b, x, r, z, zn, p, ap, tmp, partial, scal, active;
+  // block Krylov solve: Gram partials and the small k x k matrices (T, alpha, beta)
+  DevBuf bk_work, bk_mat;
   std::vector<Geom> geoms;
   // learned coarse correction (not owned) and training-sample harvest (caller's device buffer)
   NetImpl* net = nullptr;
@@ -1156,6 +1158,264 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
   sync(h);
 }
 
+// ---- block Krylov solve -------------------------------------------------------------------
+// Symmetric eigendecomposition of a small n x n matrix (cyclic Jacobi, FP64, host): a is destroyed, w gets
+// the eigenvalues and v (row-major, column j = eigenvector j) the eigenvectors.
+static void jacobi_eigh(std::vector<double>& a, int n, std::vector<double>& w, std::vector<double>& v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) (i == j ? diag : off) += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+    if (off <= 1e-30 * diag) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int r = 0; r < n; ++r) {  // columns p, q
+          const double arp = a[(size_t)r * n + p], arq = a[(size_t)r * n + q];
+          a[(size_t)r * n + p] = c * arp - sn * arq;
+          a[(size_t)r * n + q] = sn * arp + c * arq;
+        }
+        for (int r = 0; r < n; ++r) {  // rows p, q
+          const double apr = a[(size_t)p * n + r], aqr = a[(size_t)q * n + r];
+          a[(size_t)p * n + r] = c * apr - sn * aqr;
+          a[(size_t)q * n + r] = sn * apr + c * aqr;
+        }
+        for (int r = 0; r < n; ++r) {
+          const double vrp = v[(size_t)r * n + p], vrq = v[(size_t)r * n + q];
+          v[(size_t)r * n + p] = c * vrp - sn * vrq;
+          v[(size_t)r * n + q] = sn * vrp + c * vrq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
+}
+
+__global__ void k_negate_copy(const double* in, double* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = -in[i];
+}
+
+constexpr int BLOCK_MAX_K = 48;
+constexpr int BLOCK_GRAM_BLOCKS = 512;
+
+// Breakdown-free block PCG over the k columns (FP32 vectors and V-cycle, reliable FP64 updates as in
+// solve_refine).  Each iteration makes the search block A-orthonormal: W = A V, the eigendecomposition of
+// V^T W (normalised by its diagonal, directions with relative energy below 1e-6 dropped) gives T, and
+// P = V T, Q = W T satisfy P^T A P = I.  Then alpha = P^T R, D += P alpha, R -= Q alpha, Z = M R,
+// beta = -Q^T Z, and the next block V = Z + P beta.  Products over the DOFs run on the FP64 matrix cores;
+// only the k x k eigenproblem is on the host.
+static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_flat, int k, dfmi_report* rep) {
+  if (h->st.precision != 1) throw std::invalid_argument("block Krylov needs the fp32 V-cycle precision");
+  if (k > BLOCK_MAX_K) throw std::invalid_argument("block Krylov takes at most 48 columns");
+  Hierarchy& H = h->slots[slot];
+  if (!H.built) throw std::runtime_error("hierarchy slot not built");
+  ensure_workspace(h, k);
+  ensure_level_vectors(h, H, k);
+  hipStream_t s = h->stream;
+  const long long n = col(h->pad);
+  const size_t kn = (size_t)k * n;
+  double* b = h->b.as<double>();
+  double* x = h->x.as<double>();
+  double* r = h->r.as<double>();
+  float* r32 = h->z.as<float>();
+  float* d32 = r32 + kn;
+  float* p32 = h->zn.as<float>();
+  float* q32 = p32 + kn;
+  float* w32 = reinterpret_cast<float*>(h->p.as<double>());
+  h->bk_work.ensure(sizeof(double) * BLOCK_MAX_K * BLOCK_MAX_K * BLOCK_GRAM_BLOCKS);
+  h->bk_mat.ensure(sizeof(double) * BLOCK_MAX_K * BLOCK_MAX_K * 4);
+  double* gmat = h->bk_mat.as<double>();            // k x k Gram
+  double* tmat = gmat + BLOCK_MAX_K * BLOCK_MAX_K;   // k x m
+  double* amat = tmat + BLOCK_MAX_K * BLOCK_MAX_K;   // m x k (alpha or beta)
+  double* nmat = amat + BLOCK_MAX_K * BLOCK_MAX_K;   // -alpha
+  double* work = h->bk_work.as<double>();
+  double* sc = h->scal.as<double>();
+  const size_t kc = (size_t)h->k_cap;
+  double* rr = sc + 6 * kc;
+  double* scale = sc + 8 * kc;
+  double* factor = sc + 9 * kc;
+  int* act = h->active.as<int>();
+  Level& F = *H.levels[0];
+  VArgs v{h, &H, k, act};
+  const int nbr = reduce_blocks(n);
+  double* part = h->partial.as<double>();
+
+  flat_to_grid(h->pad, h->ux, h->uy, h->uz, rhs_flat, b, k, s);
+  mask_vector(F.mask.as<uint8_t>(), b, h->pad.nodes(), k, s);
+  std::vector<double> nb;
+  column_norms(h, b, n, k, nullptr, nb);
+  std::vector<int> active(k), upd(k), inactive(k);
+  std::vector<double> sh(k, 1.0), fh(k, 0.0), rrh(k), tn;
+  for (int c = 0; c < k; ++c) {
+    active[c] = nb[c] > 0 ? 1 : 0;
+    sh[c] = nb[c] > 0 ? nb[c] : 1.0;
+    rep[c].iterations = 0;
+    rep[c].relative_residual = 0.0;
+    rep[c].converged = 1;
+  }
+  const double tol = h->st.tolerance;
+  const double delta = std::min(0.5, std::max(h->st.inner_tolerance, 1e-7));
+  auto count = [](const std::vector<int>& f) {
+    int m = 0;
+    for (int q : f) m += q;
+    return m;
+  };
+  auto upload = [&](int* dst, const std::vector<int>& src) {
+    DFMI_HIP(hipMemcpyAsync(dst, src.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
+  };
+  // zero the columns whose flag in `which` is set (factor 0 on the selected columns)
+  auto zero_columns = [&](float* vec, const std::vector<int>& which) {
+    if (!count(which)) return;
+    std::vector<double> zeros(k, 0.0);
+    DFMI_HIP(hipMemcpyAsync(factor, zeros.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    upload(act, which);
+    scale_columns<float>(vec, n, k, factor, act, s);
+    upload(act, active);
+  };
+  auto inactive_of = [&]() {
+    for (int c = 0; c < k; ++c) inactive[c] = active[c] ? 0 : 1;
+    return inactive;
+  };
+  fill_zero<double>(x, (long long)kn, s);
+  fill_zero<float>(d32, (long long)kn, s);
+  fill_zero<float>(r32, (long long)kn, s);
+  DFMI_HIP(hipMemcpyAsync(scale, sh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+  upload(act, active);
+  to_work<float>(b, scale, r32, n, k, act, s);
+  if (!count(active)) {
+    grid_to_flat(h->pad, h->ux, h->uy, h->uz, x, x_flat, k, s);
+    sync(h);
+    return;
+  }
+  // V = M R, with the columns of finished right-hand sides zero
+  auto precondition_block = [&]() -> float* {
+    vcycle<float>(v, 0, r32);
+    float* z = F.x.as<float>();
+    zero_columns(z, inactive_of());
+    return z;
+  };
+  float* vb = precondition_block();
+  int step = 0;
+  std::vector<double> g((size_t)k * k), w, evec, t;
+  while (count(active) && step < h->st.max_iterations) {
+    // W = A V over all k columns, G = V^T W
+    {
+      ApplyArgs a{};
+      a.x = vb;
+      a.y = w32;
+      a.k = k;
+      fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, s, nullptr);
+    }
+    gram32_device(vb, w32, n, k, k, gmat, work, BLOCK_GRAM_BLOCKS, s);
+    DFMI_HIP(hipMemcpyAsync(g.data(), gmat, sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    sync(h);
+    // T = D^-1 U Lambda^-1/2 over the kept directions of the diagonal-normalised V^T A V
+    std::vector<int> idx;
+    for (int c = 0; c < k; ++c)
+      if (g[(size_t)c * k + c] > 0 && std::isfinite(g[(size_t)c * k + c])) idx.push_back(c);
+    const int nz = (int)idx.size();
+    if (nz == 0) break;
+    std::vector<double> dn(nz), cm((size_t)nz * nz);
+    for (int i = 0; i < nz; ++i) dn[i] = std::sqrt(g[(size_t)idx[i] * k + idx[i]]);
+    for (int i = 0; i < nz; ++i)
+      for (int j = 0; j < nz; ++j) cm[(size_t)i * nz + j] = g[(size_t)idx[i] * k + idx[j]] / (dn[i] * dn[j]);
+    jacobi_eigh(cm, nz, w, evec);
+    double wmax = 0.0;
+    for (double e : w) wmax = std::max(wmax, e);
+    std::vector<int> keep;
+    for (int j = 0; j < nz; ++j)
+      if (w[j] > 1e-6 * wmax) keep.push_back(j);
+    const int m = (int)keep.size();
+    if (m == 0) break;
+    t.assign((size_t)k * m, 0.0);
+    for (int i = 0; i < nz; ++i)
+      for (int q = 0; q < m; ++q)
+        t[(size_t)idx[i] * m + q] = evec[(size_t)i * nz + keep[q]] / (dn[i] * std::sqrt(w[keep[q]]));
+    DFMI_HIP(hipMemcpyAsync(tmat, t.data(), sizeof(double) * k * m, hipMemcpyHostToDevice, s));
+    combine32_device(vb, n, k, tmat, m, p32, 0, s);   // P = V T
+    combine32_device(w32, n, k, tmat, m, q32, 0, s);  // Q = W T = A P
+    // alpha = P^T R; D += P alpha; R -= Q alpha
+    gram32_device(p32, r32, n, m, k, amat, work, BLOCK_GRAM_BLOCKS, s);
+    hipLaunchKernelGGL(k_negate_copy, dim3((m * k + 255) / 256), dim3(256), 0, s, amat, nmat, m * k);
+    DFMI_HIP(hipGetLastError());
+    combine32_device(p32, n, m, amat, k, d32, 1, s);
+    combine32_device(q32, n, m, nmat, k, r32, 1, s);
+    column_dot<float>(r32, r32, n, k, part, nbr, act, s);
+    reduce_partials(part, nbr, k, rr, s);
+    DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    sync(h);
+    ++step;
+    bool any_upd = false;
+    for (int c = 0; c < k; ++c) {
+      upd[c] = 0;
+      if (!active[c]) continue;
+      const double est = std::sqrt(rrh[c]);
+      if (est * sh[c] < tol * nb[c] || est < delta) upd[c] = 1, any_upd = true;
+    }
+    if (any_upd) {  // flush D into x, true FP64 residual, restart the column's scaled residual at norm 1
+      upload(act, upd);
+      add_scaled(x, d32, scale, n, k, act, s);
+      for (int c = 0; c < k; ++c) fh[c] = 0.0;
+      DFMI_HIP(hipMemcpyAsync(factor, fh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      scale_columns<float>(d32, n, k, factor, act, s);
+      fine_fp64(h, H, MODE_RESID, x, r, b, k, act, nullptr, nullptr);
+      column_norms(h, r, n, k, act, tn);
+      std::vector<int> cont(k, 0), done(k, 0);
+      for (int c = 0; c < k; ++c) {
+        if (!upd[c]) continue;
+        const double tr = tn[c] / nb[c];
+        rep[c].iterations = step;
+        rep[c].relative_residual = tr;
+        rep[c].converged = tr < tol;
+        if (tr < tol || !(tn[c] > 0)) {
+          active[c] = 0;
+          done[c] = 1;
+        } else {
+          cont[c] = 1;
+          sh[c] = tn[c];
+        }
+      }
+      if (count(cont)) {
+        upload(act, cont);
+        DFMI_HIP(hipMemcpyAsync(scale, sh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        to_work<float>(r, scale, r32, n, k, act, s);  // r / |r|
+      }
+      zero_columns(r32, done);
+      upload(act, active);
+    }
+    if (!count(active)) break;
+    // Z = M R; beta = -Q^T Z; V = Z + P beta
+    float* z = precondition_block();
+    gram32_device(q32, z, n, m, k, amat, work, BLOCK_GRAM_BLOCKS, s);
+    hipLaunchKernelGGL(k_negate_copy, dim3((m * k + 255) / 256), dim3(256), 0, s, amat, nmat, m * k);
+    DFMI_HIP(hipGetLastError());
+    combine32_device(p32, n, m, nmat, k, z, 1, s);
+    vb = z;
+  }
+  if (count(active)) {  // iteration limit: flush and report the true residual
+    upload(act, active);
+    add_scaled(x, d32, scale, n, k, act, s);
+    fine_fp64(h, H, MODE_RESID, x, r, b, k, act, nullptr, nullptr);
+    column_norms(h, r, n, k, act, tn);
+    for (int c = 0; c < k; ++c)
+      if (active[c]) {
+        rep[c].iterations = step;
+        rep[c].relative_residual = tn[c] / nb[c];
+        rep[c].converged = rep[c].relative_residual < tol;
+      }
+  }
+  grid_to_flat(h->pad, h->ux, h->uy, h->uz, x, x_flat, k, s);
+  sync(h);
+}
+
 }  // namespace dfmi
 
 // ---- C ABI ---------------------------------------------------------------------------------
@@ -1204,6 +1464,7 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->fine_smoother = 0;
   o->retries = 2;
   o->coarse_krylov = 0;
+  o->block_krylov = 0;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
@@ -1223,6 +1484,7 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
     if (st.fine_smoother != 0 && st.fine_smoother != 1) throw std::invalid_argument("fine_smoother must be 0 or 1");
     if (st.retries < 0) throw std::invalid_argument("retries must be >= 0");
     if (st.coarse_krylov < 0 || st.coarse_krylov > 2) throw std::invalid_argument("coarse_krylov must be 0, 1 or 2");
+    if (st.block_krylov != 0 && st.block_krylov != 1) throw std::invalid_argument("block_krylov must be 0 or 1");
     if (st.chebyshev_degree < 1 || st.max_iterations < 1 || !(st.tolerance > 0))
       throw std::invalid_argument("invalid multigrid settings");
     h.st = st;
@@ -1392,7 +1654,9 @@ int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t 
   DFMI_GUARD({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     if (k < 1 || !rhs || !x || !report) throw std::invalid_argument("solve needs k >= 1 and buffers");
-    if (mg->h.st.refine && mg->h.slots[slot].nk == 0)
+    if (mg->h.st.refine && mg->h.slots[slot].nk == 0 && mg->h.st.block_krylov && k <= BLOCK_MAX_K)
+      solve_block(&mg->h, slot, rhs, x, k, report);
+    else if (mg->h.st.refine && mg->h.slots[slot].nk == 0)
       solve_refine(&mg->h, slot, rhs, x, k, report);
     else
       solve(&mg->h, slot, rhs, x, k, report);

@@ -55,6 +55,9 @@ typedef struct dfmi_settings {
                                 preconditioned by the level-1 V-cycle, instead of one level-1 V-cycle (a
                                 Krylov "K-cycle"; needs three or more levels, and a flexible outer PCG, which
                                 both solve schemes are; hierarchies with a free rigid kernel keep the V-cycle) */
+  int32_t block_krylov;      /* refine = 1: 0 (default) solves the columns as separate PCG recurrences, 1 as one
+                                breakdown-free block PCG over up to 48 columns (more columns, or a free rigid
+                                kernel, keep the separate recurrences) */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */
