@@ -29,6 +29,7 @@ def test_default_settings_mirror_reference_multigrid_defaults():
     assert (s.chebyshev_ratio, s.growth, s.tolerance, s.dead_ratio) == (20.0, 2.0, 1e-8, 1e-12)
     assert s.retries == 2  # MULTIGRID["retries"], topology_multigrid.py:9
     assert s.fine_smoother == 0 and s.precision == 1
+    assert s.coarse_krylov == 0 and s.block_krylov == 0  # the reference's V-cycle and per-column PCG
 
 
 def test_invalid_arguments_raise_without_touching_a_gpu():
@@ -37,3 +38,28 @@ def test_invalid_arguments_raise_without_touching_a_gpu():
     with pytest.raises(RuntimeError, match="dfmi_create failed"):
         _native.call("dfmi_create", bad, (ctypes.c_double * 3)(1, 1, 1), (ctypes.c_double * 576)(),
                      None, ctypes.byref(ctypes.c_void_p()))
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of dfmi_settings and dfmi_report have the C compiler's field offsets and sizes."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dfmi.h"', "int main(void) {"]
+    for struct, mirror in (("dfmi_settings", _native.Settings), ("dfmi_report", _native.Report)):
+        lines.append(f'  printf("{struct} sizeof %zu\\n", sizeof({struct}));')
+        for name, _ in mirror._fields_:
+            lines.append(f'  printf("{struct} {name} %zu\\n", offsetof({struct}, {name}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", f"-I{HEADER.parent}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for struct, mirror in (("dfmi_settings", _native.Settings), ("dfmi_report", _native.Report)):
+        assert got[(struct, "sizeof")] == __import__("ctypes").sizeof(mirror)
+        for name, _ in mirror._fields_:
+            assert got[(struct, name)] == getattr(mirror, name).offset, (struct, name)
