@@ -549,34 +549,41 @@ __global__ void k_panel_copy(const double* __restrict__ a, int n, int k0, int b,
   r[idx] = a[(long long)(k0 + p) * n + j];
 }
 
-// in-place Gauss-Jordan inverse of the b x b SPD block R[:, k0:k0+b] -> dinv (one workgroup)
-__global__ __launch_bounds__(256) void k_block_inverse(const double* __restrict__ r, int n, int k0, int b,
-                                                       double* __restrict__ dinv) {
-  __shared__ double m[NB][NB + 1];
-  for (int v = threadIdx.x; v < b * b; v += blockDim.x) m[v / b][v % b] = r[(long long)(v / b) * n + k0 + v % b];
-  __syncthreads();
-  for (int k = 0; k < b; ++k) {
-    const double d = m[k][k];
-    __syncthreads();
-    // sweep pivot k inside the block (same rule as the scalar sweep)
-    double nv[(NB * NB + 255) / 256];
-    int cnt = 0;
-    for (int v = threadIdx.x; v < b * b; v += blockDim.x, ++cnt) {
-      const int i = v / b, j = v % b;
-      double x;
-      if (i == k && j == k) x = -1.0 / d;
-      else if (i == k) x = m[k][j] / d;
-      else if (j == k) x = m[i][k] / d;
-      else x = m[i][j] - m[i][k] * m[k][j] / d;
-      nv[cnt] = x;
+// Gauss-Jordan inverse of the b x b SPD block R[:, k0:k0+b] -> dinv, in one wavefront: lane j keeps column j
+// of the block in registers; at pivot k the pivot column is broadcast from lane k through the scalar unit
+// (readlane), so no barrier or LDS round trip separates the pivots.
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(64) void k_block_inverse(const double* __restrict__ r, int n, int k0, int b,
+                                                      double* __restrict__ dinv) {
+  const int j = threadIdx.x;
+  double col[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) col[i] = (i < b && j < b) ? r[(long long)i * n + k0 + j] : 0.0;
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    if (k < b) {  // uniform; the loop is fully unrolled so that col stays in registers
+      const double inv = 1.0 / lane_value(col[k], k);
+      const double f = col[k] * inv;  // m[k][j] / d
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        if (i == k) continue;
+        const double ck = lane_value(col[i], k);  // m[i][k]
+        col[i] = j == k ? ck * inv : col[i] - ck * f;
+      }
+      col[k] = j == k ? -inv : f;
     }
-    __syncthreads();
-    cnt = 0;
-    for (int v = threadIdx.x; v < b * b; v += blockDim.x, ++cnt) m[v / b][v % b] = nv[cnt];
-    __syncthreads();
   }
   // the block now holds -D^-1
-  for (int v = threadIdx.x; v < b * b; v += blockDim.x) dinv[v] = -m[v / b][v % b];
+  if (j < b) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i < b) dinv[i * b + j] = -col[i];
+  }
 }
 
 // W[p][j] = sum_q dinv[p][q] R[q][j]
@@ -647,7 +654,7 @@ void dense_sweep_inverse(double* a, int n, hipStream_t s) {
     const int b = std::min(NB, n - k0);
     hipLaunchKernelGGL(k_panel_copy, dim3((unsigned)(((long long)b * n + 255) / 256)), dim3(256), 0, s, a, n, k0, b,
                        r);
-    hipLaunchKernelGGL(k_block_inverse, dim3(1), dim3(256), 0, s, r, n, k0, b, dinv);
+    hipLaunchKernelGGL(k_block_inverse, dim3(1), dim3(64), 0, s, r, n, k0, b, dinv);
     hipLaunchKernelGGL(k_panel_w, dim3((unsigned)((n + 255) / 256), b), dim3(256), 0, s, dinv, r, n, b, w);
     hipLaunchKernelGGL(k_block_update, dim3(tiles, tiles), dim3(256), 0, s, a, n, k0, b, r, w, dinv);
   }
