@@ -205,45 +205,70 @@ template void stencil_apply<double>(const Geom&, const double*, const uint8_t*, 
                                     hipStream_t, int*);
 
 // ---- Galerkin product P^T A P in stencil form -------------------------------------------
-// Coarse node N, offset D: sum over fine f = 2N + e (weight w(e)) and fine neighbours
-// g = f + df inside the support of coarse node N + D (weight w(g - 2(N + D))).
-// One thread per (coarse node, coarse offset D).  The 27 offsets of one range of coarse nodes are
-// consecutive logical blocks on one XCD, so they share the fine-stencil reads in that XCD's L2.
-__global__ void k_rap(Geom gf, const double* __restrict__ fst, Geom gc, double* __restrict__ cst) {
-  const long long Nc = gc.nodes(), Nf = gf.nodes();
+// One axis of the separable Galerkin product: P = P_x (x) P_y (x) P_z, so P^T A P is the 1-D Galerkin
+// coarsening applied along x, then y, then z, each on both sides; the intermediate operators stay 27-point
+// 3x3-block stencils.  Output node N, offset D: along the coarsened axis the sum over input nodes
+// f = 2 N + e (weight w(e)) and input offsets df with f + df inside the support of N + D (weight
+// w(f + df - 2 (N + D))); the other axes pass through (same node coordinate, same offset).  One thread per
+// (output node, offset), nine entries each.
+__global__ void k_rap_axis(Geom gi, const double* __restrict__ sin, Geom go, double* __restrict__ sout, int axis) {
+  const long long No = go.nodes(), Ni = gi.nodes();
   const unsigned logical = xcd_block(blockIdx.x, gridDim.x);
   const int D = (int)(logical % 27);
   const long long n = (long long)(logical / 27) * blockDim.x + threadIdx.x;
-  if (n >= Nc) return;
-  const int K = (int)(n % gc.nz), J = (int)((n / gc.nz) % gc.ny), I = (int)(n / ((long long)gc.nz * gc.ny));
-  const int MI = I + D / 9 - 1, MJ = J + (D / 3) % 3 - 1, MK = K + D % 3 - 1;
+  if (n >= No) return;
+  int c[3] = {(int)(n / ((long long)go.nz * go.ny)), (int)((n / go.nz) % go.ny), (int)(n % go.nz)};
+  const int dd[3] = {D / 9 - 1, (D / 3) % 3 - 1, D % 3 - 1};
+  const int ni[3] = {gi.nx, gi.ny, gi.nz};
+  const int mo = c[axis] + dd[axis];  // the coarse neighbour along the axis
+  const int no_axis = axis == 0 ? go.nx : (axis == 1 ? go.ny : go.nz);
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (MI >= 0 && MJ >= 0 && MK >= 0 && MI < gc.nx && MJ < gc.ny && MK < gc.nz) {
-    for (int e = 0; e < 27; ++e) {
-      const int ex = e / 9 - 1, ey = (e / 3) % 3 - 1, ez = e % 3 - 1;
-      const int fi = 2 * I + ex, fj = 2 * J + ey, fk = 2 * K + ez;
-      if (fi < 0 || fj < 0 || fk < 0 || fi >= gf.nx || fj >= gf.ny || fk >= gf.nz) continue;
-      const double wf = weight1(ex) * weight1(ey) * weight1(ez);
-      const long long f = ((long long)fi * gf.ny + fj) * gf.nz + fk;
-      for (int df = 0; df < 27; ++df) {
-        const int gi = fi + df / 9 - 1, gj = fj + (df / 3) % 3 - 1, gk = fk + df % 3 - 1;
-        const int tx = gi - 2 * MI, ty = gj - 2 * MJ, tz = gk - 2 * MK;
-        if (tx < -1 || tx > 1 || ty < -1 || ty > 1 || tz < -1 || tz > 1) continue;
-        if (gi < 0 || gj < 0 || gk < 0 || gi >= gf.nx || gj >= gf.ny || gk >= gf.nz) continue;
-        const double w = wf * weight1(tx) * weight1(ty) * weight1(tz);
-        const double* s = fst + (long long)df * 9 * Nf + f;
+  if (mo >= 0 && mo < no_axis) {
+    for (int e = -1; e <= 1; ++e) {
+      const int f = 2 * c[axis] + e;
+      if (f < 0 || f >= ni[axis]) continue;
+      const double we = e == 0 ? 1.0 : 0.5;
+      for (int df = -1; df <= 1; ++df) {
+        const int t = f + df - 2 * mo;
+        if (t < -1 || t > 1) continue;
+        if (f + df < 0 || f + df >= ni[axis]) continue;
+        const double w = we * (t == 0 ? 1.0 : 0.5);
+        int fi[3] = {c[0], c[1], c[2]}, od[3] = {dd[0], dd[1], dd[2]};
+        fi[axis] = f;
+        od[axis] = df;
+        const long long fn = ((long long)fi[0] * gi.ny + fi[1]) * gi.nz + fi[2];
+        const int din = 9 * (od[0] + 1) + 3 * (od[1] + 1) + (od[2] + 1);
+        const double* src = sin + (long long)din * 9 * Ni + fn;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] += w * s[q * Nf];
+        for (int q = 0; q < 9; ++q) acc[q] = fma(w, src[q * Ni], acc[q]);
       }
     }
   }
-  for (int q = 0; q < 9; ++q) cst[((long long)D * 9 + q) * Nc + n] = acc[q];
+  for (int q = 0; q < 9; ++q) sout[((long long)D * 9 + q) * No + n] = acc[q];
 }
 
 void stencil_rap(const Geom& fine, const double* fst, const Geom& coarse, double* cst, hipStream_t s) {
-  hipLaunchKernelGGL(k_rap, dim3((unsigned)((coarse.nodes() + 255) / 256) * 27), dim3(256), 0, s, fine, fst,
-                     coarse, cst);
-  DFMI_HIP(hipGetLastError());
+  // fine (nx, ny, nz) -> (Nx, ny, nz) -> (Nx, Ny, nz) -> (Nx, Ny, Nz) = coarse
+  Geom g1 = fine, g2 = fine;
+  g1.nx = coarse.nx;
+  g1.ex = coarse.ex;
+  g2.nx = coarse.nx;
+  g2.ex = coarse.ex;
+  g2.ny = coarse.ny;
+  g2.ey = coarse.ey;
+  double *s1 = nullptr, *s2 = nullptr;
+  DFMI_HIP(hipMallocAsync((void**)&s1, sizeof(double) * 243 * (size_t)g1.nodes(), s));
+  DFMI_HIP(hipMallocAsync((void**)&s2, sizeof(double) * 243 * (size_t)g2.nodes(), s));
+  const Geom gin[3] = {fine, g1, g2}, gout[3] = {g1, g2, coarse};
+  const double* in[3] = {fst, s1, s2};
+  double* out[3] = {s1, s2, cst};
+  for (int a = 0; a < 3; ++a) {
+    hipLaunchKernelGGL(k_rap_axis, dim3((unsigned)((gout[a].nodes() + 255) / 256) * 27), dim3(256), 0, s, gin[a],
+                       in[a], gout[a], out[a], a);
+    DFMI_HIP(hipGetLastError());
+  }
+  DFMI_HIP(hipFreeAsync(s1, s));
+  DFMI_HIP(hipFreeAsync(s2, s));
 }
 
 __global__ void k_stencil_diag(Geom g, const double* __restrict__ st, double* __restrict__ diag) {
