@@ -19,7 +19,8 @@ class Settings(Structure):
                 ("power_iterations", c_int32), ("precision", c_int32), ("tolerance", c_double),
                 ("max_iterations", c_int32), ("dead_ratio", c_double), ("seed", c_int32), ("refine", c_int32),
                 ("inner_tolerance", c_double), ("fine_smoother", c_int32), ("retries", c_int32),
-                ("coarse_krylov", c_int32), ("block_krylov", c_int32)]
+                ("coarse_krylov", c_int32), ("block_krylov", c_int32), ("smoother", c_int32), ("sweeps", c_int32),
+                ("damping", c_double), ("cycle", c_int32), ("coarse", c_int32)]
 
 
 class Report(Structure):
@@ -56,6 +57,8 @@ SIGNATURES = {
     "dfmi_time_apply": ([c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(c_double)], c_int32),
     "dfmi_tune_fine": ([c_void_p, c_int32, c_int32, POINTER(c_int32), POINTER(c_double)], c_int32),
     "dfmi_diagnostics": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
+    "dfmi_history": ([c_void_p, POINTER(c_double), c_int32, POINTER(c_int32)], c_int32),
+    "dfmi_kernel_check": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
     "dfmi_gram": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_project": ([c_void_p, c_void_p, c_int64, c_double, POINTER(c_double), c_int32, c_int32, c_void_p, c_void_p,

@@ -85,6 +85,8 @@ struct Hierarchy {
   // vectors, the Gram work buffer and the nk x k projection coefficients
   int nk = 0;
   DevBuf kern, kwork, kcoef;
+  // |A_l B_l| / |A_l R_l| per level for slots with a free kernel (empty otherwise)
+  std::vector<double> kernel_check;
 };
 
 constexpr int KERNEL_GRAM_BLOCKS = 512;
@@ -118,6 +120,8 @@ struct Handle {
   NetImpl* net = nullptr;
   float* harvest = nullptr;
   long long harvest_cap = 0, harvest_count = 0;
+  // largest relative recursive residual over the active columns after each PCG iteration of the last solve
+  std::vector<double> history;
 };
 
 static void hex8_blocks(const double* ke_bits, KeBlocks& kb) {
@@ -356,6 +360,78 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
   if (cur != L.x.as<T>()) std::swap(L.x.p, L.xa.p), std::swap(L.x.bytes, L.xa.bytes);
 }
 
+// Damped Jacobi smoothing of A x = b (topology_multigrid.py:286-290): x_{j+1} = x_j + w Dinv (b - A x_j) with
+// w = damping / lambda_max, from x_0 = 0 (first sweep x_1 = w Dinv b) or from the level's x.  Each sweep is
+// the Chebyshev step form with c1 = 0 and c2 = w, so it runs on the same fused operator kernels (on the fine
+// level the first two sweeps from zero are one launch, as for Chebyshev).  On exit the iterate is in L.x.
+template <typename T>
+static void jacobi(VArgs& v, int idx, const T* b, bool from_zero, int count) {
+  Level& L = *v.H->levels[idx];
+  const double w = v.h->st.damping / L.lambda_max;
+  T* cur = L.x.as<T>();
+  T* other = L.xa.as<T>();
+  const T* dinv = level_dinv<T>(L);
+  int j0 = 0;
+  if (from_zero && idx == 0 && count >= 2) {
+    ApplyArgs a{};
+    a.b = b;
+    a.dinv = dinv;
+    a.c0 = w;
+    a.c1 = 0.0;
+    a.c2 = w;
+    a.y = cur;
+    level_apply<T>(v, idx, MODE_CHEB0, a);
+    j0 = 2;
+    if (count >= 3) {
+      ApplyArgs a2{};
+      a2.x = cur;
+      a2.y = other;
+      a2.b = b;
+      a2.dinv = dinv;
+      a2.c0 = w;
+      a2.c1 = 0.0;
+      a2.c2 = w;
+      level_apply<T>(v, idx, MODE_CHEB1, a2);
+      std::swap(cur, other);
+      j0 = 3;
+    }
+  } else if (from_zero) {
+    cheb_start<T>(b, dinv, w, cur, L.g.nodes(), v.k, v.active, level_layout(L, idx, v.k), v.h->stream);
+    j0 = 1;
+  }
+  for (int j = j0; j < count; ++j) {
+    ApplyArgs a{};
+    a.x = cur;
+    a.y = other;
+    a.b = b;
+    a.xp = nullptr;
+    a.dinv = dinv;
+    a.c1 = 0.0;
+    a.c2 = w;
+    final_dot(v, idx, !from_zero && j == count - 1, a);
+    level_apply<T>(v, idx, MODE_CHEB, a, v.final_blocks);
+    std::swap(cur, other);
+  }
+  if (cur != L.x.as<T>()) std::swap(L.x.p, L.xa.p), std::swap(L.x.bytes, L.xa.bytes);
+}
+
+// Smoother steps on level idx (topology_multigrid.py:284-285): round(degree * growth^idx) Chebyshev steps or
+// round(sweeps * growth^idx) Jacobi sweeps, at least one.
+static int smoother_count(const Handle* h, int idx) {
+  const double base = h->st.smoother == 1 ? (double)h->st.sweeps : (double)h->st.chebyshev_degree;
+  // ties to even, as Python's round() in the reference
+  return std::max(1, (int)std::nearbyint(base * std::pow(h->st.growth, (double)idx)));
+}
+
+template <typename T>
+static void smooth(VArgs& v, int idx, const T* b, bool from_zero) {
+  const int count = smoother_count(v.h, idx);
+  if (v.h->st.smoother == 1)
+    jacobi<T>(v, idx, b, from_zero, count);
+  else
+    chebyshev<T>(v, idx, b, from_zero, count);
+}
+
 // Learned coarse correction on the fine level, between the coarse-grid correction and the
 // post-smoother: g = R (b - A x) on level 1, delta = Dinv g + Dinv^(1/2) s N(Dinv^(1/2) g / s, f), scaled per column by the
 // energy-optimal step max(0, delta.g / delta.A_1 delta), then x += P delta.  With the Galerkin level-1
@@ -456,40 +532,42 @@ static void vcycle(VArgs& v, int idx, const T* b) {
     return;
   }
   Level& C = *H.levels[idx + 1];
-  const int count =
-      std::max(1, (int)std::lround(v.h->st.chebyshev_degree * std::pow(v.h->st.growth, (double)idx)));
-  chebyshev<T>(v, idx, b, true, count);
-  ApplyArgs a{};
-  a.x = L.x.as<T>();
-  a.y = L.res.as<T>();
-  a.b = b;
-  level_apply<T>(v, idx, MODE_RESID, a);
+  smooth<T>(v, idx, b, true);
   // Between the column-major fine level and the interleaved level 1 the transfers run column-major on
   // both sides (coalesced), with a tiled layout change of the level-1 vector through its free res buffer.
   const bool change = idx == 0 && v.k > 1;
-  if (change) {
-    restrict_full<T>(L.g, L.res.as<T>(), column_major(L.g), C.g, C.mask.as<uint8_t>(), C.res.as<T>(),
-                     column_major(C.g), v.k, v.active, s);
-    relayout<T>(C.res.as<T>(), C.rhs.as<T>(), C.g.nodes(), v.k, false, v.active, s);
-  } else {
-    restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
-                     level_layout(C, idx + 1, v.k), v.k, v.active, s);
-  }
-  // (slots with a free rigid kernel keep the V-cycle: their level-1 operator is singular)
-  if (idx == 0 && v.h->st.coarse_krylov > 0 && H.levels.size() > 2 && H.nk == 0)
-    krylov_coarse<T>(v, C.rhs.as<T>());
-  else
-    vcycle<T>(v, idx + 1, C.rhs.as<T>());
-  if (change) {
-    relayout<T>(C.x.as<T>(), C.res.as<T>(), C.g.nodes(), v.k, true, v.active, s);
-    prolong_add<T>(C.g, C.res.as<T>(), column_major(C.g), L.g, L.mask.as<uint8_t>(), L.x.as<T>(), column_major(L.g),
-                   v.k, v.active, s);
-  } else {
-    prolong_add<T>(C.g, C.x.as<T>(), level_layout(C, idx + 1, v.k), L.g, L.mask.as<uint8_t>(), L.x.as<T>(),
-                   level_layout(L, idx, v.k), v.k, v.active, s);
+  // W-cycle: a second coarse-grid correction, except above the coarsest level (topology_multigrid.py:312)
+  const int corrections = (v.h->st.cycle == 1 && idx + 2 < (int)H.levels.size()) ? 2 : 1;
+  for (int rep = 0; rep < corrections; ++rep) {
+    ApplyArgs a{};
+    a.x = L.x.as<T>();
+    a.y = L.res.as<T>();
+    a.b = b;
+    level_apply<T>(v, idx, MODE_RESID, a);
+    if (change) {
+      restrict_full<T>(L.g, L.res.as<T>(), column_major(L.g), C.g, C.mask.as<uint8_t>(), C.res.as<T>(),
+                       column_major(C.g), v.k, v.active, s);
+      relayout<T>(C.res.as<T>(), C.rhs.as<T>(), C.g.nodes(), v.k, false, v.active, s);
+    } else {
+      restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
+                       level_layout(C, idx + 1, v.k), v.k, v.active, s);
+    }
+    // (slots with a free rigid kernel keep the V-cycle: their level-1 operator is singular)
+    if (idx == 0 && v.h->st.coarse_krylov > 0 && H.levels.size() > 2 && H.nk == 0)
+      krylov_coarse<T>(v, C.rhs.as<T>());
+    else
+      vcycle<T>(v, idx + 1, C.rhs.as<T>());
+    if (change) {
+      relayout<T>(C.x.as<T>(), C.res.as<T>(), C.g.nodes(), v.k, true, v.active, s);
+      prolong_add<T>(C.g, C.res.as<T>(), column_major(C.g), L.g, L.mask.as<uint8_t>(), L.x.as<T>(), column_major(L.g),
+                     v.k, v.active, s);
+    } else {
+      prolong_add<T>(C.g, C.x.as<T>(), level_layout(C, idx + 1, v.k), L.g, L.mask.as<uint8_t>(), L.x.as<T>(),
+                     level_layout(L, idx, v.k), v.k, v.active, s);
+    }
   }
   if (idx == 0 && (v.h->net || v.h->harvest)) learned_correction<T>(v, b);
-  chebyshev<T>(v, idx, b, false, count);
+  smooth<T>(v, idx, b, false);
 }
 
 // ---- hierarchy ------------------------------------------------------------------------------
@@ -593,6 +671,45 @@ __global__ void k_inverse_diag(const double* d, const uint8_t* mask, double* inv
   inv32[i] = (float)v;
 }
 
+// Rediscretised coarse level (topology_multigrid.py:160-166): element field of the coarse grid g as the mean
+// of its 2x2x2 children on the finer grid f (avg_pool3d), times `scale`.
+__global__ void k_pool_moduli(Geom f, Geom g, const double* __restrict__ fine, double scale, double* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.elems()) return;
+  const int l = (int)(e % g.ez), j = (int)((e / g.ez) % g.ey), i = (int)(e / ((long long)g.ez * g.ey));
+  double acc = 0.0;
+  for (int q = 0; q < 8; ++q)
+    acc += fine[((long long)(2 * i + (q & 1)) * f.ey + 2 * j + ((q >> 1) & 1)) * f.ez + 2 * l + ((q >> 2) & 1)];
+  out[e] = scale * 0.125 * acc;
+}
+
+// mask of coarse node (i, j, l) = mask of finer node (2i, 2j, 2l) (the injection of the finer level's mask)
+__global__ void k_inject_mask(Geom f, Geom g, const uint8_t* __restrict__ fine, uint8_t* __restrict__ out) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.nodes()) return;
+  const int l = (int)(n % g.nz), j = (int)((n / g.nz) % g.ny), i = (int)(n / ((long long)g.nz * g.ny));
+  out[n] = fine[((long long)(2 * i) * f.ny + 2 * j) * f.nz + 2 * l];
+}
+
+// v = u (one uniform [0, 1) value per DOF from a counter hash) on live DOFs, 0 elsewhere; k columns
+__global__ void k_uniform_masked(const uint8_t* __restrict__ mask, double* __restrict__ v, long long nodes, int k,
+                                 unsigned seed) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nodes * 3 * k) return;
+  const long long dof = i % (nodes * 3);
+  unsigned long long z = (unsigned long long)i * 0x9E3779B97F4A7C15ULL + seed;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  v[i] = ((mask[dof / 3] >> (dof % 3)) & 1) ? u : 0.0;
+}
+
+__global__ void k_to_double(const float* a, double* b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
 static void require_workspace(Handle* h) {
   if (!h->partial.p || !h->scal.p) throw std::logic_error("reduction workspace used before allocation");
 }
@@ -658,6 +775,76 @@ static double spectrum(Handle* h, Hierarchy& H, int idx) {
   return std::sqrt(out);
 }
 
+// Euclidean norm of n contiguous FP64 values (set-up diagnostics; own partial buffers)
+static double vector_norm(Handle* h, const double* v, long long n) {
+  const int nb = reduce_blocks(n);
+  DevBuf part, out;
+  part.ensure(sizeof(double) * nb);
+  out.ensure(sizeof(double));
+  column_dot<double>(v, v, n, 1, part.as<double>(), nb, nullptr, h->stream);
+  reduce_partials(part.as<double>(), nb, 1, out.as<double>(), h->stream);
+  double sq = 0.0;
+  DFMI_HIP(hipMemcpyAsync(&sq, out.p, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  sync(h);
+  return std::sqrt(sq);
+}
+
+// hierarchy["kernel_check"] (topology_multigrid.py:325-335): per level |A_l B_l| / |A_l R_l| with B_l the nk
+// orthonormal masked rigid modes of the level (about the design-grid centre, level spacing) and R_l uniform
+// [0, 1) values on the level's live DOFs, both products in the V-cycle precision.
+static void kernel_check(Handle* h, Hierarchy& H, const double* coef, int nk, const double center[3]) {
+  H.kernel_check.clear();
+  hipStream_t s = h->stream;
+  const bool fp32 = h->st.precision == 1;
+  for (int l = 0; l < (int)H.levels.size(); ++l) {
+    Level& L = *H.levels[l];
+    const long long n = col(L.g), kn = (long long)nk * n;
+    const double hl[3] = {h->spacing[0] * (1 << l), h->spacing[1] * (1 << l), h->spacing[2] * (1 << l)};
+    DevBuf basis, tmp, work, small, xs, ys, out;
+    basis.ensure(sizeof(double) * kn);
+    tmp.ensure(sizeof(double) * kn);
+    work.ensure(sizeof(double) * (size_t)KERNEL_GRAM_BLOCKS * nk * 64);
+    small.ensure(sizeof(double) * (size_t)nk * 64);
+    out.ensure(sizeof(double) * kn);
+    fine_kernel(L.g, hl, center, coef, nk, L.mask.as<uint8_t>(), basis.as<double>(), tmp.as<double>(),
+                work.as<double>(), KERNEL_GRAM_BLOCKS, small.as<double>(), s);
+    hipLaunchKernelGGL(k_uniform_masked, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, s, L.mask.as<uint8_t>(),
+                       tmp.as<double>(), L.g.nodes(), nk, (unsigned)(h->st.seed + 104729u * (l + 1)));
+    DFMI_HIP(hipGetLastError());
+    double norms[2];
+    for (int which = 0; which < 2; ++which) {
+      const double* x = which == 0 ? basis.as<double>() : tmp.as<double>();
+      ApplyArgs a{};
+      a.k = nk;
+      if (fp32) {
+        xs.ensure(sizeof(float) * kn);
+        ys.ensure(sizeof(float) * kn);
+        hipLaunchKernelGGL(k_to_float, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, s, x, xs.as<float>(), kn);
+        a.x = xs.as<float>();
+        a.y = ys.as<float>();
+        if (l == 0)
+          fine_apply<float>(L.g, h->m32.as<float>(), L.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, s, nullptr);
+        else
+          stencil_apply<float>(L.g, L.st32.as<float>(), L.mask.as<uint8_t>(), MODE_APPLY, a, column_major(L.g), s,
+                               nullptr);
+        hipLaunchKernelGGL(k_to_double, dim3((unsigned)((kn + 255) / 256)), dim3(256), 0, s, ys.as<float>(),
+                           out.as<double>(), kn);
+      } else {
+        a.x = x;
+        a.y = out.as<double>();
+        if (l == 0)
+          fine_apply<double>(L.g, h->m64.as<double>(), L.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, s, nullptr);
+        else
+          stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, column_major(L.g), s,
+                                nullptr);
+      }
+      DFMI_HIP(hipGetLastError());
+      norms[which] = vector_norm(h, out.as<double>(), kn);
+    }
+    H.kernel_check.push_back(norms[0] / std::max(norms[1], 1e-300));
+  }
+}
+
 static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* coef = nullptr, int nk = 0) {
   if (!h->have_moduli) throw std::runtime_error("update(moduli) must precede hierarchy construction");
   auto t0 = std::chrono::steady_clock::now();
@@ -696,17 +883,40 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
     fine_kernel(F.g, h->spacing, center, coef, nk, F.mask.as<uint8_t>(), H.kern.as<double>(), tmp.as<double>(),
                 H.kwork.as<double>(), KERNEL_GRAM_BLOCKS, H.kcoef.as<double>(), s);
   }
-  // Galerkin levels
-  DevBuf fst;
-  fst.ensure(sizeof(double) * 243 * (size_t)F.g.nodes());
-  fine_stencil(F.g, h->m64.as<double>(), F.mask.as<uint8_t>(), h->ke_dev.as<double>(), fst.as<double>(), s);
+  // Galerkin levels (P^T A P), or rediscretised ones (coarse = 1): the operator assembled on the coarse grid
+  // from 2x2x2-averaged moduli with Ke scaled by the spacing ratio 2^l (3-D elasticity: Ke(2h) = 2 Ke(h)) and
+  // the finer level's mask injected (topology_multigrid.py:160-166)
+  const bool rediscretize = h->st.coarse == 1;
+  DevBuf fst, pooled[2];
+  if (!rediscretize) {
+    fst.ensure(sizeof(double) * 243 * (size_t)F.g.nodes());
+    fine_stencil(F.g, h->m64.as<double>(), F.mask.as<uint8_t>(), h->ke_dev.as<double>(), fst.as<double>(), s);
+  }
   const double* prev = fst.as<double>();
+  const double* prev_moduli = h->m64.as<double>();
   for (int l = 1; l < h->nlev; ++l) {
     Level& P = *H.levels[l - 1];
     Level& L = *H.levels[l];
     const long long nn = L.g.nodes();
     L.st64.ensure(sizeof(double) * 243 * (size_t)nn);
-    stencil_rap(P.g, prev, L.g, L.st64.as<double>(), s);
+    if (rediscretize) {
+      DevBuf& mean = pooled[l & 1];
+      DevBuf scaled;
+      mean.ensure(sizeof(double) * (size_t)L.g.elems());
+      scaled.ensure(sizeof(double) * (size_t)L.g.elems());
+      const unsigned ge = (unsigned)((L.g.elems() + 255) / 256);
+      hipLaunchKernelGGL(k_pool_moduli, dim3(ge), dim3(256), 0, s, P.g, L.g, prev_moduli, 1.0, mean.as<double>());
+      hipLaunchKernelGGL(k_pool_moduli, dim3(ge), dim3(256), 0, s, P.g, L.g, prev_moduli, (double)(1 << l),
+                         scaled.as<double>());
+      hipLaunchKernelGGL(k_inject_mask, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, P.g, L.g,
+                         P.mask.as<uint8_t>(), L.mask.as<uint8_t>());
+      DFMI_HIP(hipGetLastError());
+      fine_stencil(L.g, scaled.as<double>(), L.mask.as<uint8_t>(), h->ke_dev.as<double>(), L.st64.as<double>(), s);
+      sync(h);  // `scaled` and the older pooled field are released below
+      prev_moduli = mean.as<double>();
+    } else {
+      stencil_rap(P.g, prev, L.g, L.st64.as<double>(), s);
+    }
     DevBuf d;
     d.ensure(sizeof(double) * nn * 3);
     stencil_diagonal(L.g, L.st64.as<double>(), d.as<double>(), s);
@@ -769,6 +979,7 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
     }
     sync(h);
   }
+  if (nk > 0) kernel_check(h, H, coef, nk, center);
   H.built = true;
   H.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -917,11 +1128,14 @@ static void solve(Handle* h, int slot, const double* rhs_flat, double* x_flat, i
     ++step;
     std::vector<int> check(k, 0);
     bool need = false;
+    double worst = 0.0;
     for (int c = 0; c < k; ++c) {
       if (!active[c]) continue;
       rel[c] = std::sqrt(rrh[c]) / nb[c];
+      worst = std::max(worst, rel[c]);
       if (rel[c] < tol) check[c] = 1, need = true;
     }
+    h->history.push_back(worst);
     if (need) {
       // true residual confirmation (topology_multigrid.py:431-437)
       DFMI_HIP(hipMemcpyAsync(act, check.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
@@ -1094,12 +1308,15 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
     sync(h);
     ++step;
     bool any_upd = false;
+    double worst = 0.0;
     for (int c = 0; c < k; ++c) {
       upd[c] = 0;
       if (!active[c]) continue;
       const double est = std::sqrt(rrh[c]);  // |r| / s since the last update (r / s had norm 1 there)
+      worst = std::max(worst, est * sh[c] / nb[c]);
       if (est * sh[c] < tol * nb[c] || est < delta) upd[c] = 1, any_upd = true;
     }
+    h->history.push_back(worst);
     if (any_upd) {
       upload(act, upd);
       add_scaled(x, d32, scale, n, k, act, s);
@@ -1354,12 +1571,15 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
     sync(h);
     ++step;
     bool any_upd = false;
+    double worst = 0.0;
     for (int c = 0; c < k; ++c) {
       upd[c] = 0;
       if (!active[c]) continue;
       const double est = std::sqrt(rrh[c]);
+      worst = std::max(worst, est * sh[c] / nb[c]);
       if (est * sh[c] < tol * nb[c] || est < delta) upd[c] = 1, any_upd = true;
     }
+    h->history.push_back(worst);
     if (any_upd) {  // flush D into x, true FP64 residual, restart the column's scaled residual at norm 1
       upload(act, upd);
       add_scaled(x, d32, scale, n, k, act, s);
@@ -1465,6 +1685,11 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->retries = 2;
   o->coarse_krylov = 0;
   o->block_krylov = 0;
+  o->smoother = 0;
+  o->sweeps = 2;
+  o->damping = 1.0;
+  o->cycle = 0;
+  o->coarse = 0;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
@@ -1487,6 +1712,11 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
     if (st.block_krylov != 0 && st.block_krylov != 1) throw std::invalid_argument("block_krylov must be 0 or 1");
     if (st.chebyshev_degree < 1 || st.max_iterations < 1 || !(st.tolerance > 0))
       throw std::invalid_argument("invalid multigrid settings");
+    if (st.smoother != 0 && st.smoother != 1) throw std::invalid_argument("smoother must be 0 (chebyshev) or 1 (jacobi)");
+    if (st.smoother == 1 && (st.sweeps < 1 || !(st.damping > 0)))
+      throw std::invalid_argument("jacobi needs sweeps >= 1 and damping > 0");
+    if (st.cycle != 0 && st.cycle != 1) throw std::invalid_argument("cycle must be 0 (V) or 1 (W)");
+    if (st.coarse != 0 && st.coarse != 1) throw std::invalid_argument("coarse must be 0 (galerkin) or 1 (rediscretize)");
     h.st = st;
     h.ux = shape[0];
     h.uy = shape[1];
@@ -1654,6 +1884,7 @@ int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t 
   DFMI_GUARD({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     if (k < 1 || !rhs || !x || !report) throw std::invalid_argument("solve needs k >= 1 and buffers");
+    mg->h.history.clear();
     if (mg->h.st.refine && mg->h.slots[slot].nk == 0 && mg->h.st.block_krylov && k <= BLOCK_MAX_K)
       solve_block(&mg->h, slot, rhs, x, k, report);
     else if (mg->h.st.refine && mg->h.slots[slot].nk == 0)
@@ -1973,10 +2204,30 @@ int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double
 
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs) {
   DFMI_GUARD({
+    if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     Hierarchy& H = mg->h.slots[slot];
     if (!H.built) throw std::invalid_argument("slot not built");
     for (size_t l = 0; l + 1 < H.levels.size(); ++l) lambda_max[l] = H.levels[l]->lambda_max;
     *coarsest_dofs = H.coarsest_n;
+  })
+}
+
+int dfmi_history(dfmi_mg* mg, double* out, int32_t capacity, int32_t* count) {
+  DFMI_GUARD({
+    const auto& hist = mg->h.history;
+    if (count) *count = (int32_t)hist.size();
+    if (out)
+      for (int i = 0; i < std::min<int>(capacity, (int)hist.size()); ++i) out[i] = hist[i];
+  })
+}
+
+int dfmi_kernel_check(dfmi_mg* mg, int32_t slot, double* out, int32_t* count) {
+  DFMI_GUARD({
+    if (slot < 0 || slot >= 8 || !mg->h.slots[slot].built) throw std::invalid_argument("slot not built");
+    const auto& kc = mg->h.slots[slot].kernel_check;
+    if (count) *count = (int32_t)kc.size();
+    if (out)
+      for (size_t i = 0; i < kc.size(); ++i) out[i] = kc[i];
   })
 }
 

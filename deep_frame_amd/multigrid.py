@@ -24,6 +24,10 @@ MULTIGRID = {"coarse": "galerkin", "coarsest_dofs": 80000, "max_levels": 8, "smo
              "eigen_tolerance": 1e-8, "eigen_iterations": 300, "eigen_guard": 2, "eigen_refresh": 10,
              "eigen_drop": 1e-12, "eigen_seed": 0}
 PRECISIONS = {"float64": 0, "float32": 1}
+# MULTIGRID["smoother"], ["cycle"] and ["coarse"] as the library's codes (topology_multigrid.py:51)
+SMOOTHERS = {"chebyshev": 0, "jacobi": 1}
+CYCLES = {"V": 0, "W": 1}
+COARSE = {"galerkin": 0, "rediscretize": 1}
 # operator precisions of dfmi_apply: "bf16x3" is the V-cycle's fine-level matrix-core kernel (FP32 vectors,
 # three split-bf16 products per element block)
 APPLY_PRECISIONS = {**PRECISIONS, "bf16x3": 2}
@@ -56,8 +60,8 @@ class GeometricMultigrid:
         torch = self.torch = _torch()
         self.settings = {**MULTIGRID, **(settings or {})}
         s = self.settings
-        if (s["precision"] not in PRECISIONS or s["coarse"] != "galerkin" or s["smoother"] != "chebyshev"
-                or s["cycle"] != "V" or s["fine_smoother"] not in FINE_SMOOTHERS):
+        if (s["precision"] not in PRECISIONS or s["coarse"] not in COARSE or s["smoother"] not in SMOOTHERS
+                or s["cycle"] not in CYCLES or s["fine_smoother"] not in FINE_SMOOTHERS):
             raise ValueError("Invalid multigrid settings")
         if not torch.cuda.is_available():
             raise RuntimeError("GeometricMultigrid needs a ROCm GPU (libdfmi has no CPU path)")
@@ -97,6 +101,8 @@ class GeometricMultigrid:
         n.retries = int(s["retries"])
         n.coarse_krylov = int(s["coarse_krylov"])
         n.block_krylov = int(bool(s["block_krylov"]))
+        n.smoother, n.sweeps, n.damping = SMOOTHERS[s["smoother"]], int(s["sweeps"]), float(s["damping"])
+        n.cycle, n.coarse = CYCLES[s["cycle"]], COARSE[s["coarse"]]
         return n
 
     def _stream(self):
@@ -149,8 +155,11 @@ class GeometricMultigrid:
         lam = (c_double * max(1, self.level_count - 1))()
         coarsest = c_int32()
         _native.call("dfmi_diagnostics", self.handle, slot, lam, byref(coarsest))
+        check, checked = (c_double * self.level_count)(), c_int32()
+        _native.call("dfmi_kernel_check", self.handle, slot, check, byref(checked))
         entry = {"slot": slot, "setup_s": seconds.value, "lambda_max": list(lam)[:self.level_count - 1],
-                 "coarsest_dofs": coarsest.value, "fixed": fixed, "coefficients": coefficients, "kernel": nk}
+                 "coarsest_dofs": coarsest.value, "fixed": fixed, "coefficients": coefficients, "kernel": nk,
+                 "kernel_check": list(check)[:checked.value]}
         self.hierarchies[key] = entry
         return entry
 
@@ -204,6 +213,15 @@ class GeometricMultigrid:
         _native.call("dfmi_solve", self.handle, entry["slot"], _native.ptr(rhs), _native.ptr(x), k, reports)
         return x, [(r.iterations, r.relative_residual, bool(r.converged), r.retried) for r in reports]
 
+    def last_history(self):
+        """Largest relative recursive residual over the active columns after each PCG iteration of the last
+        device solve, FP64 retries appended (the per-batch ``history`` of topology_multigrid.py:421-426)."""
+        count = c_int32()
+        _native.call("dfmi_history", self.handle, None, 0, byref(count))
+        values = (c_double * max(1, count.value))()
+        _native.call("dfmi_history", self.handle, values, count.value, byref(count))
+        return list(values)[:count.value]
+
     def solve(self, key, fixed, forces, support=None):
         """Reference signature (topology_multigrid.py:336-375); forces (ndof, k) host FP64.
 
@@ -227,7 +245,7 @@ class GeometricMultigrid:
         flat = np.zeros_like(forces)
         iterations, relative = np.zeros(forces.shape[1], dtype=int), np.zeros(forces.shape[1])
         batches = retried = 0
-        consistency = []
+        consistency, history = [], []
         for start in range(0, len(live), self.settings["batch"]):
             cols = live[start:start + self.settings["batch"]]
             rhs = self._tensor(forces[:, cols].T)
@@ -238,6 +256,7 @@ class GeometricMultigrid:
                              None, component)
                 consistency += list(component)
             x, reps = self.solve_device(key, fixed, rhs)
+            history.append(self.last_history())
             flat[:, cols] = x.cpu().numpy().T
             iterations[cols] = [r[0] for r in reps]
             relative[cols] = [r[1] for r in reps]
@@ -248,11 +267,11 @@ class GeometricMultigrid:
             shift = np.linalg.lstsq(rigid[support], flat[support], rcond=None)[0]
             flat = flat - rigid @ shift
         report = {"iterations": iterations.tolist(), "relative_residual": relative.tolist(), "converged": converged,
-                  "batches": batches, "retried_columns": int(retried), "zero_rhs_columns": int(forces.shape[1] - len(live)),
+                  "history": history, "batches": batches, "retried_columns": int(retried), "zero_rhs_columns": int(forces.shape[1] - len(live)),
                   "seconds": perf_counter() - started, "setup_s": entry.pop("setup_s", 0.0), "floating": floating,
                   "levels": self.level_count, "coarsest_dofs": entry["coarsest_dofs"],
                   "kernel": int(entry["kernel"]) if floating else 0, "rhs_kernel_component": consistency or None,
-                  "lambda_max": entry["lambda_max"]}
+                  "kernel_check": entry["kernel_check"], "lambda_max": entry["lambda_max"]}
         self.statistics.append(report)
         return flat, report
 

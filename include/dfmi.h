@@ -58,6 +58,15 @@ typedef struct dfmi_settings {
   int32_t block_krylov;      /* refine = 1: 0 (default) solves the columns as separate PCG recurrences, 1 as one
                                 breakdown-free block PCG over up to 48 columns (more columns, or a free rigid
                                 kernel, keep the separate recurrences) */
+  int32_t smoother;          /* MULTIGRID["smoother"] (topology_multigrid.py:284-305): 0 = "chebyshev" (default),
+                                1 = "jacobi" (damped, step damping / lambda_max, round(sweeps * growth^l) sweeps) */
+  int32_t sweeps;            /* 2: Jacobi sweeps on the fine level */
+  double damping;            /* 1.0: Jacobi damping */
+  int32_t cycle;             /* MULTIGRID["cycle"] (topology_multigrid.py:312): 0 = "V" (default), 1 = "W" (two
+                                coarse-grid corrections on every level whose coarser level is not the coarsest) */
+  int32_t coarse;            /* MULTIGRID["coarse"] (topology_multigrid.py:160-166): 0 = "galerkin" (default),
+                                1 = "rediscretize" (coarse moduli by 2x2x2 averaging, Ke scaled by the spacing
+                                ratio, mask injected from the finer level and cut by the dead-DOF ratio) */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */
@@ -178,6 +187,17 @@ int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double
 
 /* Diagnostics of a built slot: lambda_max per smoothed level and the coarsest live DOF count. */
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs);
+
+/* report["history"] of GeometricMultigrid.solve (topology_multigrid.py:421-426, :400, :367): the largest
+ * relative recursive residual over the active columns after every PCG iteration of the handle's last
+ * dfmi_solve, FP64 retries appended.  Copies min(count, capacity) values into out (host, may be NULL). */
+int dfmi_history(dfmi_mg* mg, double* out, int32_t capacity, int32_t* count);
+
+/* hierarchy["kernel_check"] (topology_multigrid.py:325-335, :206): for a slot with a free rigid kernel, per
+ * level |A_l B_l| / |A_l R_l| with B_l the orthonormal masked kernel modes of the level and R_l uniform [0, 1)
+ * values on its live DOFs, in the V-cycle precision; count = levels (0 for a clamped slot).  out: host array
+ * of at least the level count, may be NULL. */
+int dfmi_kernel_check(dfmi_mg* mg, int32_t slot, double* out, int32_t* count);
 
 /* InterfaceCovariance.matrix (deep_frame/topology_problem.py:639-646) and the Gram matrices of
  * GeometricMultigrid.eigenpairs (topology_multigrid.py:462-500): out_dev[i * kb + j] =

@@ -8,8 +8,13 @@ Restates ``deep_frame/topology_multigrid.py``:
     (:69, :314-316); restriction = prolongation^T,
   * Galerkin coarse operator A_c = P^T (M A M) P (:174-181),
   * Chebyshev smoother with Jacobi scaling, degree 3 * 2^level, bounds
-    [1.1 lmax / 20, 1.1 lmax] (:284-305),
-  * V-cycle (:306-317) with an exact coarsest solve,
+    [1.1 lmax / 20, 1.1 lmax] (:284-305), or damped Jacobi with step damping / lmax and
+    round(sweeps * growth^level) sweeps (:284-290),
+  * V-cycle, or W-cycle (two coarse-grid corrections on every level whose coarser level is not the
+    coarsest) (:306-317), with an exact coarsest solve,
+  * rediscretised coarse levels (``coarse = "rediscretize"``, :160-166): moduli averaged over 2x2x2
+    children, Ke scaled by the spacing ratio, mask injected from the finer level and cut by the dead-DOF
+    ratio,
   * (no counterpart in the reference) the K-cycle variant of the fine level's V-cycle: the level-1
     system of the coarse-grid correction is solved by a few flexible-CG steps preconditioned by the
     level-1 V-cycle (``fcg``, ``kcycle``),
@@ -70,7 +75,21 @@ def fine_mask(shape_padded, moduli_padded, fixed_padded):
     return mask
 
 
-def hierarchy(shape, ke, moduli, fixed, levels, dead_ratio=1e-12):
+def pool(moduli):
+    """Mean of the 2x2x2 children of every coarse element (avg_pool3d, topology_multigrid.py:161)."""
+    m = np.asarray(moduli, float)
+    nx, ny, nz = (n // 2 for n in m.shape)
+    return m.reshape(nx, 2, ny, 2, nz, 2).mean(axis=(1, 3, 5))
+
+
+def inject(mask, shape):
+    """DOF mask of the coarse node grid (shape elements) taken at the even nodes of the finer one."""
+    fine_nodes = tuple(2 * n + 1 for n in shape)
+    grid3 = np.asarray(mask).reshape(fine_nodes + (3,))
+    return grid3[::2, ::2, ::2].reshape(-1).copy()
+
+
+def hierarchy(shape, ke, moduli, fixed, levels, dead_ratio=1e-12, coarse="galerkin"):
     """List of dicts with 'A' (masked), 'mask', 'dinv', 'P' (to the next finer level)."""
     _, padded = levels
     count = levels[0]
@@ -79,16 +98,27 @@ def hierarchy(shape, ke, moduli, fixed, levels, dead_ratio=1e-12):
     a = fine_operator(padded, ke, mpad)
     m = fine_mask(padded, mpad, fixed)
     a = diags(m) @ a @ diags(m)
-    out = [{"A": a.tocsr(), "mask": m, "shape": padded}]
+    out = [{"A": a.tocsr(), "mask": m, "shape": padded, "moduli": mpad}]
     for lvl in range(1, count):
         cshape = tuple(n // 2 ** lvl for n in padded)
         p = prolongation(cshape)
-        ac = (p.T @ out[-1]["A"] @ p).tocsr()
-        ac = ((ac + ac.T) * 0.5).tocsr()
-        dg = ac.diagonal()
-        cm = (dg > dead_ratio * dg.max()).astype(float)
+        if coarse == "rediscretize":
+            # coarse operator of the averaged moduli with Ke(2^l h) = 2^l Ke(h) (3-D elasticity), assembled
+            # with the injected mask on both sides; the dead-DOF ratio then cuts the injected mask
+            moduli_c = pool(out[-1]["moduli"])
+            injected = inject(out[-1]["mask"], cshape)
+            ac = fine_operator(cshape, ke * 2.0 ** lvl, moduli_c)
+            ac = (diags(injected) @ ac @ diags(injected)).tocsr()
+            dg = ac.diagonal()
+            cm = injected * (dg > dead_ratio * dg.max())
+        else:
+            moduli_c = None
+            ac = (p.T @ out[-1]["A"] @ p).tocsr()
+            ac = ((ac + ac.T) * 0.5).tocsr()
+            dg = ac.diagonal()
+            cm = (dg > dead_ratio * dg.max()).astype(float)
         ac = (diags(cm) @ ac @ diags(cm)).tocsr()
-        out.append({"A": ac, "mask": cm, "shape": cshape, "P": p})
+        out.append({"A": ac, "mask": cm, "shape": cshape, "P": p, "moduli": moduli_c})
     for lv in out:
         dg = lv["A"].diagonal()
         lv["dinv"] = np.where(lv["mask"] > 0, 1.0 / np.where(dg > 0, dg, 1.0), 0.0)
@@ -125,7 +155,16 @@ def chebyshev(level, rhs, x, count, lmax, ratio=20.0):
     return x
 
 
-def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0):
+def jacobi(level, rhs, x, count, lmax, damping=1.0):
+    """Damped Jacobi sweeps x += (damping / lmax) Dinv (rhs - A x), the first from x = 0 when x is None."""
+    step = damping / lmax
+    for _ in range(count):
+        x = step * level["dinv"] * rhs if x is None else x + step * level["dinv"] * (rhs - level["A"] @ x)
+    return x
+
+
+def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0, smoother="chebyshev", sweeps=2, damping=1.0,
+           cycle="V"):
     lv = levels[index]
     if index == len(levels) - 1:
         live = np.flatnonzero(lv["mask"] > 0)
@@ -133,13 +172,21 @@ def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0):
         sub = lv["A"][live][:, live].tocsc()
         out[live] = splu(sub).solve(rhs[live])
         return out
-    count = max(1, int(round(degree * growth ** index)))
-    x = chebyshev(lv, rhs, None, count, lmax[index])
-    res = (rhs - lv["A"] @ x) * lv["mask"]
+    count = max(1, int(round((sweeps if smoother == "jacobi" else degree) * growth ** index)))
+
+    def smooth(x):
+        if smoother == "jacobi":
+            return jacobi(lv, rhs, x, count, lmax[index], damping)
+        return chebyshev(lv, rhs, x, count, lmax[index])
+
+    x = smooth(None)
     p = levels[index + 1]["P"]
-    corr = vcycle(levels, lmax, (p.T @ res) * levels[index + 1]["mask"], index + 1, degree, growth)
-    x = x + (p @ corr) * lv["mask"]
-    return chebyshev(lv, rhs, x, count, lmax[index])
+    for _ in range(2 if cycle == "W" and index + 2 < len(levels) else 1):
+        res = (rhs - lv["A"] @ x) * lv["mask"]
+        corr = vcycle(levels, lmax, (p.T @ res) * levels[index + 1]["mask"], index + 1, degree, growth, smoother,
+                      sweeps, damping, cycle)
+        x = x + (p @ corr) * lv["mask"]
+    return smooth(x)
 
 
 def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0):

@@ -30,6 +30,8 @@ def test_default_settings_mirror_reference_multigrid_defaults():
     assert s.retries == 2  # MULTIGRID["retries"], topology_multigrid.py:9
     assert s.fine_smoother == 0 and s.precision == 1
     assert s.coarse_krylov == 0 and s.block_krylov == 0  # the reference's V-cycle and per-column PCG
+    # MULTIGRID smoother "chebyshev", sweeps 2, damping 1.0, cycle "V", coarse "galerkin" (topology_multigrid.py:8)
+    assert (s.smoother, s.sweeps, s.damping, s.cycle, s.coarse) == (0, 2, 1.0, 0, 0)
 
 
 def test_invalid_arguments_raise_without_touching_a_gpu():

@@ -42,6 +42,11 @@ def test_relief_support_matches_clamped_solve(torch_gpu, shape, projection):
     assert report["floating"] == projection and report["kernel"] == (6 if projection else 0)
     if projection:
         assert max(report["rhs_kernel_component"]) < 1e-12
+        # hierarchy["kernel_check"] (topology_multigrid.py:325-335): every level's operator annihilates its
+        # masked rigid modes up to FP32 rounding
+        assert len(report["kernel_check"]) == report["levels"] and max(report["kernel_check"]) < 1e-4
+    else:
+        assert report["kernel_check"] == []
     assert np.linalg.norm(u - expected) < 1e-7 * np.linalg.norm(expected)
     mg.close()
 

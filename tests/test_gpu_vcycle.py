@@ -79,3 +79,61 @@ def test_kcycle_matches_oracle(shape, spacing, precision, steps, torch_gpu):
         for c in range(cols):
             assert np.linalg.norm(z[:, c] - ref[:, c]) < tol * np.linalg.norm(ref[:, c]), (cols, c)
     mg.close()
+
+
+@pytest.mark.parametrize("settings", [{"smoother": "jacobi"}, {"smoother": "jacobi", "sweeps": 3, "damping": 0.8},
+                                      {"cycle": "W"}, {"smoother": "jacobi", "cycle": "W"},
+                                      {"coarse": "rediscretize"}, {"coarse": "rediscretize", "cycle": "W"}])
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_reference_cycle_settings_match_oracle(settings, precision, torch_gpu):
+    """The reference's other MULTIGRID choices (topology_multigrid.py:51, :160-166, :284-290, :312): damped Jacobi
+    smoothing, the W-cycle and rediscretised coarse levels, against the oracle's cycle on the same hierarchy
+    (FP64 to 1e-9, FP32 to 1e-4) for 1, 2 and 7 columns."""
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    print(f"cycle settings={settings} precision={precision}", flush=True)
+    shape, spacing = (16, 8, 8), (1.0, 1.0, 1.0)
+    ke = element.stiffness(spacing, element.isotropic(0.3))
+    rho = np.clip(np.random.default_rng(1).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
+    mod = 1e-6 + (1 - 1e-6) * rho ** 3
+    pts = element.node_points(shape, spacing)
+    fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    mg = GeometricMultigrid(shape, spacing, ke, {"coarsest_dofs": 200, "precision": precision, **settings})
+    mg.update(mod)
+    entry = mg.hierarchy("v", fixed)
+    levels = gmg.hierarchy(shape, ke, mod, fixed, gmg.level_count(shape, 200), coarse=settings.get("coarse", "galerkin"))
+    assert len(levels) == mg.level_count >= 3
+    for cols in (1, 2, 7):
+        r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
+        ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c], smoother=settings.get("smoother", "chebyshev"),
+                                          sweeps=settings.get("sweeps", 2), damping=settings.get("damping", 1.0),
+                                          cycle=settings.get("cycle", "V")) for c in range(cols)])
+        z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
+        tol = 1e-9 if precision == "float64" else 1e-4
+        for c in range(cols):
+            assert np.linalg.norm(z[:, c] - ref[:, c]) < tol * np.linalg.norm(ref[:, c]), (cols, c)
+    mg.close()
+
+
+@pytest.mark.parametrize("settings", [{"smoother": "jacobi"}, {"cycle": "W"}, {"coarse": "rediscretize"}])
+def test_reference_cycle_settings_solve_with_history(settings, torch_gpu):
+    """A solve with each setting meets 1e-8 against the assembled FP64 operator, and the report carries the
+    reference's per-batch ``history`` (one entry per PCG iteration, ending below the tolerance) and an empty
+    ``kernel_check`` for a clamped fixture."""
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    from oracle import hex8
+    print(f"solve settings={settings}", flush=True)
+    prob = problems.cantilever(seed=0, shape=(16, 8, 8))
+    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
+    mod, _ = problems.moduli(prob["density"])
+    mg = GeometricMultigrid(prob["shape"], (1, 1, 1), ke, {"coarsest_dofs": 500, **settings})
+    mg.update(mod)
+    x, report = mg.solve(b"s", prob["fixed"], prob["forces"])
+    pts, conn, dofs = hex8.grid(prob["shape"])
+    k = hex8.assemble(dofs, ke, mod, 3 * len(pts))
+    free = np.setdiff1d(np.arange(k.shape[0]), prob["fixed"])
+    res = prob["forces"][free] - (k @ x)[free]
+    assert np.linalg.norm(res) < 1e-8 * np.linalg.norm(prob["forces"][free])
+    assert report["converged"] and len(report["history"]) == 1
+    assert len(report["history"][0]) == max(report["iterations"])
+    assert report["history"][0][-1] < 1e-8 and report["kernel_check"] == []
+    mg.close()
