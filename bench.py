@@ -81,27 +81,77 @@ def parse():
     ap.add_argument("--fp64-only", action="store_true", help="only time the plain FP64 solve (for kernel traces)")
     ap.add_argument("--apply-only", action="store_true",
                     help="build the hierarchy once and only time the fine operator (for counter profiles)")
+    ap.add_argument("--learned", default=None,
+                    help="mixed solver: trained coarse-correction weights (.npz under the repository) to attach")
     return ap.parse_args()
 
 
-def cpu_baseline(seconds_budget=30.0):
-    """SuperLU (oracle) 42-RHS solve on a bounded sample of the same workload family."""
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _cpu_run(name, threads, wall_cap_s, memory_cap_gb):
+    """One child process of oracle.cpu_baseline under a wall-clock and an address-space cap; DNF if it does not
+    finish within them."""
+    import resource
+    import subprocess
+    env = {**os.environ, "OMP_NUM_THREADS": str(threads), "OPENBLAS_NUM_THREADS": str(threads),
+           "MKL_NUM_THREADS": str(threads)}
+    limit = int(memory_cap_gb * 2 ** 30)
+
+    def cap():
+        resource.setrlimit(resource.RLIMIT_AS, (limit, limit))
+
+    caps = {"wall_cap_s": wall_cap_s, "address_space_cap_gb": memory_cap_gb}
+    try:
+        done = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", name, str(threads)], cwd=ROOT, env=env,
+                              capture_output=True, text=True, timeout=wall_cap_s, preexec_fn=cap)
+    except subprocess.TimeoutExpired:
+        return {"workload": name, "threads": threads, "status": "DNF (wall-clock cap)", **caps}
+    if done.returncode != 0:
+        tail = done.stderr.strip().splitlines()[-1:] if done.stderr else []
+        return {"workload": name, "threads": threads, "status": f"DNF (exit {done.returncode})", "error": tail, **caps}
+    out = json.loads(done.stdout.strip().splitlines()[-1])
+    print(f"# cpu {name} threads {threads}: {out['seconds']:.1f} s", file=sys.stderr, flush=True)
+    return {**out, "status": "ok", **caps}
+
+
+def cpu_baseline():
+    """The reference's CPU path (SuperLU with MMD_AT_PLUS_A, assembly to sensitivity; oracle.cpu_baseline) on the
+    host cores: the 48 x 24 x 24 cantilever on one thread (the value) and on the host's share of cores, two
+    small grids of the frame family with 42 RHS for the time scaling, the 96 x 96 x 16 frame under a cap, and the
+    256 x 256 x 48 frame extrapolated from the fitted scaling (SuperLU fill grows about N^2: not attempted)."""
     import numpy as np
-    from deep_frame_amd import element, problems
-    from oracle import hex8, system
-    from threadpoolctl import threadpool_limits
-    shape = (48, 48, 8)  # 10-40 s of single-core SuperLU work depending on the host; fill-in grows ~N^2
-    prob = problems.frame(shape, seed=0, rhs=42)
-    ke = element.stiffness((1, 1, 1), element.isotropic(0.3))
-    mod, _ = problems.moduli(prob["density"])
-    pts, conn, dofs = hex8.grid(shape)
-    with threadpool_limits(limits=1):  # one core: SuperLU itself is serial; keep its BLAS calls serial too
-        t0 = time.perf_counter()
-        u, rel = system.solve(dofs, conn, ke, mod, prob["fixed"], prob["forces"], len(pts))
-        dt = time.perf_counter() - t0
-    return {"value": dt, "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"SuperLU (MMD_AT_PLUS_A) assemble+factor+42-RHS solve, frame {shape[0]}x{shape[1]}x{shape[2]} "
-                      f"({3 * len(pts)} DOF), max rel-res {float(np.max(rel)):.1e}, BLAS limited to one thread"}
+    affinity = len(os.sched_getaffinity(0))
+    cores = max(1, min(affinity, 16))
+    runs = [_cpu_run("cantilever", 1, 300, 48), _cpu_run("cantilever", cores, 300, 48),
+            _cpu_run("frame:32,32,6", 1, 120, 48), _cpu_run("frame:48,48,8", 1, 180, 48),
+            _cpu_run("frame:96,96,16", 1, 90, 48)]
+    small, large = runs[2], runs[3]
+    extrapolation = None
+    if small.get("status") == "ok" and large.get("status") == "ok":
+        alpha = np.log(large["seconds"] / small["seconds"]) / np.log(large["dof"] / small["dof"])
+        beta = np.log(large["peak_rss_gb"] / small["peak_rss_gb"]) / np.log(large["dof"] / small["dof"])
+        extrapolation = {"fit": "t = t(48x48x8) (N / N(48x48x8))^alpha over the frame family, 42 RHS, 1 thread",
+                         "alpha": float(alpha), "rss_exponent": float(beta)}
+        for label, dof in (("frame 96x96x16", 479859), ("frame 256x256x48", 9709203)):
+            extrapolation[label] = {"dof": dof, "seconds": float(large["seconds"] * (dof / large["dof"]) ** alpha),
+                                    "peak_rss_gb": float(large["peak_rss_gb"] * (dof / large["dof"]) ** beta)}
+    runs.append({"workload": "frame:256,256,48", "threads": 1, "status": "DNF (not attempted: the extrapolated time "
+                 "and memory exceed any cap of a benchmark run)"})
+    first = runs[0]
+    return {"value": first.get("seconds"), "unit": "s", "cores": 1, "kind": "port",
+            "sample": "48x24x24 cantilever (91,875 DOF, 1 RHS): assembly, SuperLU factor, solve, compliance and "
+                      "sensitivity on one thread (the reference's documented thread setting); the other runs are "
+                      "listed in runs",
+            "host": {"cpu": _cpu_model(), "nproc": os.cpu_count(), "affinity": affinity, "threads_all_cores": cores},
+            "runs": runs, "extrapolation": extrapolation}
 
 
 def main():
@@ -131,8 +181,10 @@ def main():
     nel = moduli.numel()
     print(f"# config {args.config} shape {cfg['shape']} ndof {ndof} rhs {rhs.shape[0]}", file=sys.stderr, flush=True)
 
-    def run(settings, steps, warmup, label):
+    def run(settings, steps, warmup, label, net=None):
         mg = GeometricMultigrid(cfg["shape"], (1, 1, 1), ke, {**settings, "levels": cfg["levels"]})
+        if net is not None:
+            net.attach(mg)
         times, reports = [], None
         for it in range(warmup + steps):
             torch.cuda.synchronize()
@@ -189,7 +241,11 @@ def main():
              "block_krylov": args.block}
     if args.inner_tolerance:
         mixed["inner_tolerance"] = args.inner_tolerance
-    mg, times, reports, x = run(mixed, args.steps, args.warmup, f"mixed-{args.scheme}")
+    learned = None
+    if args.learned:
+        from deep_frame_amd.learned import CoarseCorrection
+        learned = CoarseCorrection.load(ROOT / args.learned)
+    mg, times, reports, x = run(mixed, args.steps, args.warmup, f"mixed-{args.scheme}", learned)
     t_mixed = float(np.mean(times))
     # dominant K.u kernel of the mixed solve: the fine-level smoother / residual operator of the V-cycle
     # (matrix cores unless --fine-smoother vector), on all RHS columns, HIP events on the library stream;
@@ -208,10 +264,20 @@ def main():
     main_precision = 2 if tile else 1
     achieved = alg_bytes / (timings[main_precision] * 1e-3) / 1e9
     mg.close()
-    t_fp64 = None
+    t_fp64 = t_fp64_same = None
+    it64 = it64_same = None
     if not args.no_fp64:
-        _, t64, _, _ = run({"precision": "float64"}, 1, 0, "fp64")
-        t_fp64 = float(np.mean(t64))
+        # the reference's scheme in FP64, timed like the mixed leg (same warm-up and timed steps)
+        mg64, t64, rep64, _ = run({"precision": "float64"}, args.steps, args.warmup, "fp64")
+        mg64.close()
+        t_fp64, it64 = float(np.mean(t64)), max(r[0] for r in rep64)
+        # FP64 with the mixed leg's algorithm (level-1 Krylov solve, degree growth, flexible FP64 PCG): the part
+        # of the speed-up that precision alone buys
+        same = {"precision": "float64", "coarse_krylov": args.coarse_krylov, "growth": args.growth,
+                "chebyshev_degree": args.degree}
+        mgs, ts, reps, _ = run(same, min(args.steps, 3), 1, "fp64-mixed-algorithm")
+        mgs.close()
+        t_fp64_same, it64_same = float(np.mean(ts)), max(r[0] for r in reps)
     baseline = None
     if not args.no_cpu_baseline:
         try:
@@ -226,7 +292,11 @@ def main():
         "vs_baseline": None, "dtype": "fp32 V-cycle / fp64 outer", "data": "synthetic",
         "config": {"workload": f"{args.config} {cfg['shape'][0]}x{cfg['shape'][1]}x{cfg['shape'][2]} hex8, "
                                f"{ndof} DOF, {k} RHS, {cfg['levels']}-level GMG", "scheme": args.scheme,
-                   "mixed_s": t_mixed, "fp64_s": t_fp64, "iterations": max(r[0] for r in reports),
+                   "learned": learned is not None, "learned_weights": args.learned,
+                   "mixed_s": t_mixed, "fp64_s": t_fp64, "fp64_iterations": it64,
+                   "fp64_mixed_algorithm_s": t_fp64_same, "fp64_mixed_algorithm_iterations": it64_same,
+                   "precision_speedup": (t_fp64_same / t_mixed) if t_fp64_same else None,
+                   "iterations": max(r[0] for r in reports),
                    "max_rel_res": max(r[1] for r in reports), "fine_smoother": args.fine_smoother,
                    "coarse_krylov": args.coarse_krylov, "growth": args.growth,
                    "chebyshev_degree": args.degree, "inner_tolerance": args.inner_tolerance,

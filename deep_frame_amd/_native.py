@@ -70,10 +70,11 @@ SIGNATURES = {
     "dfmi_filter_set_stream": ([c_void_p, c_void_p], c_int32),
     "dfmi_filter_forward": ([c_void_p, c_void_p, c_void_p], c_int32),
     "dfmi_filter_pullback": ([c_void_p, c_void_p, c_void_p, c_void_p], c_int32),
+    "dfmi_filter_set_corridor": ([c_void_p, c_double, POINTER(c_double)], c_int32),
     "dfmi_filter_volume": ([c_void_p, c_void_p, POINTER(c_double)], c_int32),
     "dfmi_oc_update": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, c_void_p], c_int32),
-    "dfmi_mma_step": ([c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
-                       c_double, c_void_p, c_double, c_double, c_void_p], c_int32),
+    "dfmi_mma_step": ([c_int64, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
+                       c_void_p, POINTER(c_double), c_void_p, c_double, c_double, c_void_p], c_int32),
     "dfmi_net_create": ([POINTER(c_void_p)], c_int32),
     "dfmi_net_destroy": ([c_void_p], c_int32),
     "dfmi_net_set_stream": ([c_void_p, c_void_p], c_int32),

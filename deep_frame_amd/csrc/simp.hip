@@ -4,6 +4,7 @@
 // (Svanberg's method as called by deep_frame/topology_optimizers.py:55-67).
 // All element-wise work is HBM-bound; reductions use per-block partials and a
 // fixed-order second pass.
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <stdexcept>
@@ -140,7 +141,7 @@ __global__ void k_cone(FilterGeom g, const int* offs, const double* w, const uin
   if (e >= n) return;
   const int l = (int)(e % g.nz), j = (int)((e / g.nz) % g.ny), i = (int)(e / ((long long)g.nz * g.ny));
   double acc = 0.0;
-  if (allowed[e]) {
+  if (allowed[e] && !(keep && !keep[e])) {
     for (int t = 0; t < g.taps; ++t) {
       const int a = i + offs[3 * t], b = j + offs[3 * t + 1], c = l + offs[3 * t + 2];
       if (a < 0 || b < 0 || c < 0 || a >= g.nx || b >= g.ny || c >= g.nz) continue;
@@ -152,6 +153,26 @@ __global__ void k_cone(FilterGeom g, const int* offs, const double* w, const uin
   if (post) acc *= post[e];
   if (keep && !keep[e]) acc = 0.0;
   out[e] = acc;
+}
+
+// corridor blend of the cone filter (topology_optimization.py:1125-1129, :1142, :1176):
+// a = (1 - w) a + w b; lo = (1 - w) s d, hi = w s d; out += a
+__global__ void k_blend(double* a, const double* b, const double* w, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = (1.0 - w[i]) * a[i] + w[i] * b[i];
+}
+
+__global__ void k_mul_split(const double* s, const double* d, const double* w, double* lo, double* hi, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = s[i] * d[i];
+  lo[i] = (1.0 - w[i]) * v;
+  hi[i] = w[i] * v;
+}
+
+__global__ void k_add(double* out, const double* a, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += a[i];
 }
 
 __global__ void k_inverse(const double* a, double* b, long long n) {
@@ -178,16 +199,40 @@ __global__ void k_sum(const double* a, long long n, double* partial) {
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
+// cone taps max(r - |o h|, 0) > 0 over |o_a| <= floor(r / h_a) (topology_optimization.py:1082-1092)
+static void cone_taps(const double spacing[3], double radius, std::vector<int>& offs, std::vector<double>& w) {
+  int pad[3];
+  for (int a = 0; a < 3; ++a) pad[a] = (int)std::floor(radius / spacing[a]);
+  for (int i = -pad[0]; i <= pad[0]; ++i)
+    for (int j = -pad[1]; j <= pad[1]; ++j)
+      for (int l = -pad[2]; l <= pad[2]; ++l) {
+        const double dd =
+            std::sqrt(std::pow(i * spacing[0], 2) + std::pow(j * spacing[1], 2) + std::pow(l * spacing[2], 2));
+        const double wt = std::max(radius - dd, 0.0);
+        if (wt > 0) {
+          offs.insert(offs.end(), {i, j, l});
+          w.push_back(wt);
+        }
+      }
+}
+
 struct Filter {
   FilterGeom g{};
+  double spacing[3]{};
   hipStream_t stream = nullptr;
   long long n = 0;
   void *offs = nullptr, *w = nullptr, *allowed = nullptr, *free_ = nullptr, *sums = nullptr, *inv = nullptr,
        *tmp = nullptr, *tmp2 = nullptr, *partial = nullptr, *lo = nullptr, *hi = nullptr, *ratio = nullptr,
        *prop = nullptr;
+  // corridor blend: the wide cone (g2 taps), blend weights w in [0, 1] on allowed cells, their support and
+  // the wide cone's inverse sums (1 off the support)
+  bool corridor = false;
+  FilterGeom g2{};
+  void *offs2 = nullptr, *w2 = nullptr, *blend = nullptr, *support = nullptr, *inv2 = nullptr, *tmp3 = nullptr;
   int nb = 1;
   ~Filter() {
-    for (void* p : {offs, w, allowed, free_, sums, inv, tmp, tmp2, partial, lo, hi, ratio, prop})
+    for (void* p : {offs, w, allowed, free_, sums, inv, tmp, tmp2, partial, lo, hi, ratio, prop, offs2, w2, blend,
+                    support, inv2, tmp3})
       if (p) (void)hipFree(p);
   }
   double sum(const double* a) {
@@ -199,9 +244,37 @@ struct Filter {
     for (double v : h) acc += v;
     return acc;
   }
+  // filtered field x~ = (W (x a)) a / sums, blended with the wide cone on the corridor: (1 - w) x~ + w x~_wide
   void forward(const double* x, double* out) {
-    hipLaunchKernelGGL(k_cone, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, g, (const int*)offs,
-                       (const double*)w, (const uint8_t*)allowed, x, nullptr, (const double*)inv, nullptr, out);
+    const unsigned ge = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_cone, dim3(ge), dim3(256), 0, stream, g, (const int*)offs, (const double*)w,
+                       (const uint8_t*)allowed, x, nullptr, (const double*)inv, nullptr, out);
+    if (corridor) {
+      hipLaunchKernelGGL(k_cone, dim3(ge), dim3(256), 0, stream, g2, (const int*)offs2, (const double*)w2,
+                         (const uint8_t*)allowed, x, nullptr, (const double*)inv2, (const uint8_t*)support,
+                         (double*)tmp3);
+      hipLaunchKernelGGL(k_blend, dim3(ge), dim3(256), 0, stream, out, (const double*)tmp3, (const double*)blend, n);
+    }
+    DFMI_HIP(hipGetLastError());
+  }
+  // adjoint: a (W (a (1 - w) s d / sums) + W_wide (a w s d / sums_wide)), zero outside the free cells
+  void pullback(const double* sens, const double* dproj, double* out) {
+    const unsigned ge = (unsigned)((n + 255) / 256);
+    if (!corridor) {
+      hipLaunchKernelGGL(k_mul, dim3(ge), dim3(256), 0, stream, sens, dproj, (double*)tmp, n);
+    } else {
+      hipLaunchKernelGGL(k_mul_split, dim3(ge), dim3(256), 0, stream, sens, dproj, (const double*)blend, (double*)tmp,
+                         (double*)tmp2, n);
+    }
+    hipLaunchKernelGGL(k_cone, dim3(ge), dim3(256), 0, stream, g, (const int*)offs, (const double*)w,
+                       (const uint8_t*)allowed, (const double*)tmp, (const double*)inv, nullptr,
+                       (const uint8_t*)free_, out);
+    if (corridor) {
+      hipLaunchKernelGGL(k_cone, dim3(ge), dim3(256), 0, stream, g2, (const int*)offs2, (const double*)w2,
+                         (const uint8_t*)allowed, (const double*)tmp2, (const double*)inv2, nullptr,
+                         (const uint8_t*)free_, (double*)tmp3);
+      hipLaunchKernelGGL(k_add, dim3(ge), dim3(256), 0, stream, out, (const double*)tmp3, n);
+    }
     DFMI_HIP(hipGetLastError());
   }
 };
@@ -257,17 +330,25 @@ __global__ void k_ratio_max(const double* ratio, const uint8_t* free_, long long
   if (threadIdx.x == 0) partial[blockIdx.x] = mx;
 }
 
-// ---- MMA (m = 1, a = 0) ------------------------------------------------------------------------
-struct MmaScalars {
-  double y, z, lam, mu, zet, s;
+// ---- MMA (m constraints, a = 0) -------------------------------------------------------------------
+// Svanberg's MMA as mmapy.mmasub runs it for deep_frame/topology_optimizers.py:55-67 (a0 = 1, a = 0, c = 1e4,
+// d = 1, bounds [0, 1]): asymptote update, move-limited bounds, the convex separable approximation and the
+// primal-dual interior-point subproblem.  The n-sized passes run here; the m x m Newton system of the dual
+// variables is solved on the host (m <= MMA_MAX_M).
+constexpr int MMA_MAX_M = 16;
+constexpr int MMA_PAIRS = MMA_MAX_M * (MMA_MAX_M + 1) / 2;
+
+struct MmaLam {
+  double v[MMA_MAX_M];
 };
 
-__global__ void k_mma_prepare(long long n, int it, const double* x, const double* xold1, const double* xold2,
+__global__ void k_mma_prepare(long long n, int m, int it, const double* x, const double* xold1, const double* xold2,
                               double* low, double* upp, const double* df0, const double* dg, double move,
                               double scale, double* alfa, double* beta, double* p0, double* q0, double* pp,
-                              double* qq, double* xs, double* xsi, double* eta, double* partial) {
+                              double* qq, double* xs, double* xsi, double* eta, double* partial, int nb) {
   const double asyinit = 0.5, asydecr = 0.7, asyincr = 1.2, raa0 = 1e-5, albefa = 0.1;
-  double acc = 0.0;
+  double acc[MMA_MAX_M];
+  for (int j = 0; j < m; ++j) acc[j] = 0.0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double xv = x[i], span = 1.0;  // bounds [0, 1]
     double lo, up;
@@ -290,64 +371,94 @@ __global__ void k_mma_prepare(long long n, int it, const double* x, const double
     beta[i] = be;
     const double ux2 = (up - xv) * (up - xv), xl2 = (xv - lo) * (xv - lo);
     const double inv = 1.0 / fmax(span, 1e-5);
-    const double d0 = scale * df0[i], d1 = scale * dg[i];
-    double a0p = fmax(d0, 0.0), a0q = fmax(-d0, 0.0);
+    const double d0 = scale * df0[i];
+    const double a0p = fmax(d0, 0.0), a0q = fmax(-d0, 0.0);
     const double pq0 = 0.001 * (a0p + a0q) + raa0 * inv;
     p0[i] = (a0p + pq0) * ux2;
     q0[i] = (a0q + pq0) * xl2;
-    double a1p = fmax(d1, 0.0), a1q = fmax(-d1, 0.0);
-    const double pq = 0.001 * (a1p + a1q) + raa0 * inv;
-    pp[i] = (a1p + pq) * ux2;
-    qq[i] = (a1q + pq) * xl2;
-    acc += pp[i] / (up - xv) + qq[i] / (xv - lo);
+    for (int j = 0; j < m; ++j) {
+      const double d1 = scale * dg[(long long)j * n + i];
+      const double a1p = fmax(d1, 0.0), a1q = fmax(-d1, 0.0);
+      const double pq = 0.001 * (a1p + a1q) + raa0 * inv;
+      const double pv = (a1p + pq) * ux2, qv = (a1q + pq) * xl2;
+      pp[(long long)j * n + i] = pv;
+      qq[(long long)j * n + i] = qv;
+      acc[j] += pv / (up - xv) + qv / (xv - lo);
+    }
     const double x0 = 0.5 * (al + be);
     xs[i] = x0;
     xsi[i] = fmax(1.0 / (x0 - al), 1.0);
     eta[i] = fmax(1.0 / (be - x0), 1.0);
   }
-  acc = blk_sum(acc);
-  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  for (int j = 0; j < m; ++j) {
+    const double t = blk_sum(acc[j]);
+    if (threadIdx.x == 0) partial[(long long)j * nb + blockIdx.x] = t;
+  }
 }
 
 struct MmaVecs {
   const double *low, *upp, *alfa, *beta, *p0, *q0, *pp, *qq;
 };
 
-// Newton system pieces: sums of gvec, GG*delx/diagx and GG^2/diagx
-__global__ void k_mma_newton(long long n, MmaVecs v, const double* x, const double* xsi, const double* eta,
-                             double lam, double epsi, double* partial, int nb) {
-  double sg = 0, sgd = 0, sgg = 0;
+// Newton system pieces: per constraint j the sums of gvec_j and of GG_j delx / diagx, and per pair (j, l),
+// j <= l, the sum of GG_j GG_l / diagx (rows of partial: m, m, m (m + 1) / 2)
+__global__ void k_mma_newton(long long n, int m, MmaVecs v, const double* x, const double* xsi, const double* eta,
+                             MmaLam lam, double epsi, double* partial, int nb) {
+  double sg[MMA_MAX_M], sgd[MMA_MAX_M], sgg[MMA_PAIRS];
+  for (int j = 0; j < m; ++j) sg[j] = sgd[j] = 0.0;
+  for (int q = 0; q < m * (m + 1) / 2; ++q) sgg[q] = 0.0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double ux = v.upp[i] - x[i], xl = x[i] - v.low[i];
-    const double plam = v.p0[i] + lam * v.pp[i], qlam = v.q0[i] + lam * v.qq[i];
-    const double gg = v.pp[i] / (ux * ux) - v.qq[i] / (xl * xl);
+    double plam = v.p0[i], qlam = v.q0[i];
+    for (int j = 0; j < m; ++j) {
+      plam += lam.v[j] * v.pp[(long long)j * n + i];
+      qlam += lam.v[j] * v.qq[(long long)j * n + i];
+    }
     const double delx = plam / (ux * ux) - qlam / (xl * xl) - epsi / (x[i] - v.alfa[i]) + epsi / (v.beta[i] - x[i]);
     const double diagx = 2.0 * (plam / (ux * ux * ux) + qlam / (xl * xl * xl)) + xsi[i] / (x[i] - v.alfa[i]) +
                          eta[i] / (v.beta[i] - x[i]);
-    sg += v.pp[i] / ux + v.qq[i] / xl;
-    sgd += gg * delx / diagx;
-    sgg += gg * gg / diagx;
+    double gg[MMA_MAX_M];
+    for (int j = 0; j < m; ++j) {
+      const double pj = v.pp[(long long)j * n + i], qj = v.qq[(long long)j * n + i];
+      gg[j] = pj / (ux * ux) - qj / (xl * xl);
+      sg[j] += pj / ux + qj / xl;
+      sgd[j] += gg[j] * delx / diagx;
+    }
+    int q = 0;
+    for (int j = 0; j < m; ++j)
+      for (int l = j; l < m; ++l) sgg[q++] += gg[j] * gg[l] / diagx;
   }
-  const double a = blk_sum(sg), b = blk_sum(sgd), c = blk_sum(sgg);
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = a;
-    partial[nb + blockIdx.x] = b;
-    partial[2 * nb + blockIdx.x] = c;
+  for (int j = 0; j < m; ++j) {
+    const double a = blk_sum(sg[j]);
+    const double b = blk_sum(sgd[j]);
+    if (threadIdx.x == 0) {
+      partial[(long long)j * nb + blockIdx.x] = a;
+      partial[(long long)(m + j) * nb + blockIdx.x] = b;
+    }
+  }
+  for (int q = 0; q < m * (m + 1) / 2; ++q) {
+    const double c = blk_sum(sgg[q]);
+    if (threadIdx.x == 0) partial[(long long)(2 * m + q) * nb + blockIdx.x] = c;
   }
 }
 
-__global__ void k_mma_direction(long long n, MmaVecs v, const double* x, const double* xsi, const double* eta,
-                                double lam, double dlam, double epsi, double* dx, double* dxsi, double* deta,
+__global__ void k_mma_direction(long long n, int m, MmaVecs v, const double* x, const double* xsi, const double* eta,
+                                MmaLam lam, MmaLam dlam, double epsi, double* dx, double* dxsi, double* deta,
                                 double* partial) {
   double mx = -1e300;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double ux = v.upp[i] - x[i], xl = x[i] - v.low[i];
-    const double plam = v.p0[i] + lam * v.pp[i], qlam = v.q0[i] + lam * v.qq[i];
-    const double gg = v.pp[i] / (ux * ux) - v.qq[i] / (xl * xl);
+    double plam = v.p0[i], qlam = v.q0[i], ggd = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double pj = v.pp[(long long)j * n + i], qj = v.qq[(long long)j * n + i];
+      plam += lam.v[j] * pj;
+      qlam += lam.v[j] * qj;
+      ggd += (pj / (ux * ux) - qj / (xl * xl)) * dlam.v[j];
+    }
     const double xa = x[i] - v.alfa[i], bx = v.beta[i] - x[i];
     const double delx = plam / (ux * ux) - qlam / (xl * xl) - epsi / xa + epsi / bx;
     const double diagx = 2.0 * (plam / (ux * ux * ux) + qlam / (xl * xl * xl)) + xsi[i] / xa + eta[i] / bx;
-    const double d = -delx / diagx - gg * dlam / diagx;
+    const double d = -delx / diagx - ggd / diagx;
     const double ds = -xsi[i] + epsi / xa - xsi[i] * d / xa;
     const double de = -eta[i] + epsi / bx + eta[i] * d / bx;
     dx[i] = d;
@@ -359,11 +470,14 @@ __global__ void k_mma_direction(long long n, MmaVecs v, const double* x, const d
   if (threadIdx.x == 0) partial[blockIdx.x] = mx;
 }
 
-// trial point at step t and its residual pieces: sum gvec, sum of squares, max |.| of the n-sized residuals
-__global__ void k_mma_trial(long long n, MmaVecs v, const double* x, const double* xsi, const double* eta,
-                            const double* dx, const double* dxsi, const double* deta, double t, double lam,
+// trial point at step t and its residual pieces: per constraint sum gvec_j (rows 0..m-1), then the sum of
+// squares and the max |.| of the n-sized residuals (rows m, m + 1)
+__global__ void k_mma_trial(long long n, int m, MmaVecs v, const double* x, const double* xsi, const double* eta,
+                            const double* dx, const double* dxsi, const double* deta, double t, MmaLam lam,
                             double epsi, double* xt, double* xsit, double* etat, double* partial, int nb) {
-  double sg = 0, ss = 0, mx = 0;
+  double sg[MMA_MAX_M];
+  for (int j = 0; j < m; ++j) sg[j] = 0.0;
+  double ss = 0, mx = 0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const double xv = x[i] + t * (dx ? dx[i] : 0.0);
     const double sv = xsi[i] + t * (dxsi ? dxsi[i] : 0.0);
@@ -374,19 +488,27 @@ __global__ void k_mma_trial(long long n, MmaVecs v, const double* x, const doubl
       etat[i] = ev;
     }
     const double ux = v.upp[i] - xv, xl = xv - v.low[i];
-    const double plam = v.p0[i] + lam * v.pp[i], qlam = v.q0[i] + lam * v.qq[i];
+    double plam = v.p0[i], qlam = v.q0[i];
+    for (int j = 0; j < m; ++j) {
+      const double pj = v.pp[(long long)j * n + i], qj = v.qq[(long long)j * n + i];
+      plam += lam.v[j] * pj;
+      qlam += lam.v[j] * qj;
+      sg[j] += pj / ux + qj / xl;
+    }
     const double rex = plam / (ux * ux) - qlam / (xl * xl) - sv + ev;
     const double rexsi = sv * (xv - v.alfa[i]) - epsi;
     const double reeta = ev * (v.beta[i] - xv) - epsi;
-    sg += v.pp[i] / ux + v.qq[i] / xl;
     ss += rex * rex + rexsi * rexsi + reeta * reeta;
     mx = fmax(mx, fmax(fabs(rex), fmax(fabs(rexsi), fabs(reeta))));
   }
-  const double a = blk_sum(sg), b = blk_sum(ss), c = blk_max(mx);
+  for (int j = 0; j < m; ++j) {
+    const double a = blk_sum(sg[j]);
+    if (threadIdx.x == 0) partial[(long long)j * nb + blockIdx.x] = a;
+  }
+  const double b = blk_sum(ss), c = blk_max(mx);
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = a;
-    partial[nb + blockIdx.x] = b;
-    partial[2 * nb + blockIdx.x] = c;
+    partial[(long long)m * nb + blockIdx.x] = b;
+    partial[(long long)(m + 1) * nb + blockIdx.x] = c;
   }
 }
 
@@ -400,98 +522,163 @@ __global__ void k_clip01(double* x, long long n) {
   if (i < n) x[i] = fmin(fmax(x[i], 0.0), 1.0);
 }
 
-static void mma_step(long long n, int it, double* x, double* xold1, double* xold2, double* low, double* upp, double f0,
-                     const double* df0, double g, const double* dg, double move, double scale, hipStream_t s) {
-  (void)f0;
+// dual variables of the subproblem (y, lam, mu, s are m-vectors; z, zet scalars)
+struct MmaDual {
+  std::vector<double> y, lam, mu, s;
+  double z = 1.0, zet = 1.0;
+};
+
+// x = a^-1 b for a small symmetric positive definite m x m matrix (Cholesky, FP64, host)
+static std::vector<double> spd_solve(std::vector<double> a, std::vector<double> b, int m) {
+  for (int j = 0; j < m; ++j) {
+    double d = a[j * m + j];
+    for (int k = 0; k < j; ++k) d -= a[j * m + k] * a[j * m + k];
+    if (!(d > 0)) throw std::runtime_error("MMA Newton system is not positive definite");
+    d = std::sqrt(d);
+    a[j * m + j] = d;
+    for (int i = j + 1; i < m; ++i) {
+      double v = a[i * m + j];
+      for (int k = 0; k < j; ++k) v -= a[i * m + k] * a[j * m + k];
+      a[i * m + j] = v / d;
+    }
+  }
+  for (int i = 0; i < m; ++i) {
+    for (int k = 0; k < i; ++k) b[i] -= a[i * m + k] * b[k];
+    b[i] /= a[i * m + i];
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    for (int k = i + 1; k < m; ++k) b[i] -= a[k * m + i] * b[k];
+    b[i] /= a[i * m + i];
+  }
+  return b;
+}
+
+static void mma_step(long long n, int m, int it, double* x, double* xold1, double* xold2, double* low, double* upp,
+                     const double* df0, const double* g, const double* dg, double move, double scale, hipStream_t s) {
   const int nb = std::max(1, std::min(256, (int)((n + 1023) / 1024)));
   const unsigned ge = (unsigned)((n + 255) / 256);
-  std::vector<double*> buf(13, nullptr);
+  const int rows = 2 * m + m * (m + 1) / 2;
+  std::vector<double*> buf(15, nullptr);
   for (auto& p : buf) DFMI_HIP(hipMallocAsync((void**)&p, sizeof(double) * n, s));
-  double* partial = nullptr;
-  DFMI_HIP(hipMallocAsync((void**)&partial, sizeof(double) * 3 * nb, s));
-  double *alfa = buf[0], *beta = buf[1], *p0 = buf[2], *q0 = buf[3], *pp = buf[4], *qq = buf[5], *xs = buf[6],
-         *xsi = buf[7], *eta = buf[8], *dx = buf[9], *dxsi = buf[10], *deta = buf[11], *tx = buf[12];
-  double *tsi = nullptr, *te = nullptr;
-  DFMI_HIP(hipMallocAsync((void**)&tsi, sizeof(double) * n, s));
-  DFMI_HIP(hipMallocAsync((void**)&te, sizeof(double) * n, s));
-  std::vector<double> h(3 * nb);
+  double *pp = nullptr, *qq = nullptr, *partial = nullptr;
+  DFMI_HIP(hipMallocAsync((void**)&pp, sizeof(double) * m * n, s));
+  DFMI_HIP(hipMallocAsync((void**)&qq, sizeof(double) * m * n, s));
+  DFMI_HIP(hipMallocAsync((void**)&partial, sizeof(double) * rows * nb, s));
+  double *alfa = buf[0], *beta = buf[1], *p0 = buf[2], *q0 = buf[3], *xs = buf[4], *xsi = buf[5], *eta = buf[6],
+         *dx = buf[7], *dxsi = buf[8], *deta = buf[9], *tx = buf[10], *tsi = buf[11], *te = buf[12];
+  std::vector<double> h((size_t)rows * nb);
   auto fetch = [&](int parts) {
     DFMI_HIP(hipMemcpyAsync(h.data(), partial, sizeof(double) * parts * nb, hipMemcpyDeviceToHost, s));
     DFMI_HIP(hipStreamSynchronize(s));
   };
-  auto psum = [&](int part) {
+  auto psum = [&](int row) {
     double a = 0;
-    for (int i = 0; i < nb; ++i) a += h[part * nb + i];
+    for (int i = 0; i < nb; ++i) a += h[(size_t)row * nb + i];
     return a;
   };
-  auto pmax = [&](int part) {
+  auto pmax = [&](int row) {
     double a = -1e300;
-    for (int i = 0; i < nb; ++i) a = std::max(a, h[part * nb + i]);
+    for (int i = 0; i < nb; ++i) a = std::max(a, h[(size_t)row * nb + i]);
     return a;
   };
-  hipLaunchKernelGGL(k_mma_prepare, dim3(nb), dim3(256), 0, s, n, it, x, xold1, xold2, low, upp, df0, dg, move, scale,
-                     alfa, beta, p0, q0, pp, qq, xs, xsi, eta, partial);
-  fetch(1);
-  const double gs = scale * g;
-  const double b = psum(0) - gs;
+  auto pack = [&](const std::vector<double>& v) {
+    MmaLam o{};
+    for (int j = 0; j < m; ++j) o.v[j] = v[j];
+    return o;
+  };
+  hipLaunchKernelGGL(k_mma_prepare, dim3(nb), dim3(256), 0, s, n, m, it, x, xold1, xold2, low, upp, df0, dg, move,
+                     scale, alfa, beta, p0, q0, pp, qq, xs, xsi, eta, partial, nb);
+  fetch(m);
+  std::vector<double> b(m);
+  for (int j = 0; j < m; ++j) b[j] = psum(j) - scale * g[j];
   const double a0 = 1.0, c = 1e4, d = 1.0;
   MmaVecs v{low, upp, alfa, beta, p0, q0, pp, qq};
-  MmaScalars sc{1.0, 1.0, 1.0, std::max(1.0, 0.5 * c), 1.0, 1.0};
+  MmaDual sc;
+  sc.y.assign(m, 1.0);
+  sc.lam.assign(m, 1.0);
+  sc.mu.assign(m, std::max(1.0, 0.5 * c));
+  sc.s.assign(m, 1.0);
   double epsi = 1.0;
-  auto residual = [&](const MmaScalars& q, double gvec, double ss_n, double mx_n, double& norm, double& mx) {
-    const double rey = c + d * q.y - q.mu - q.lam;
-    const double rez = a0 - q.zet;
-    const double relam = gvec - q.y + q.s - b;
-    const double remu = q.mu * q.y - epsi, rezet = q.zet * q.z - epsi, res = q.lam * q.s - epsi;
-    norm = std::sqrt(ss_n + rey * rey + rez * rez + relam * relam + remu * remu + rezet * rezet + res * res);
-    mx = std::max({mx_n, std::fabs(rey), std::fabs(rez), std::fabs(relam), std::fabs(remu), std::fabs(rezet),
-                   std::fabs(res)});
+  // norm and max |.| of the full KKT residual from the n-sized pieces and the dual variables
+  auto residual = [&](const MmaDual& q, const std::vector<double>& gvec, double ss_n, double mx_n, double& norm,
+                      double& mx) {
+    double ss = ss_n;
+    mx = mx_n;
+    auto add = [&](double r) {
+      ss += r * r;
+      mx = std::max(mx, std::fabs(r));
+    };
+    for (int j = 0; j < m; ++j) add(c + d * q.y[j] - q.mu[j] - q.lam[j]);
+    add(a0 - q.zet);
+    for (int j = 0; j < m; ++j) add(gvec[j] - q.y[j] + q.s[j] - b[j]);
+    for (int j = 0; j < m; ++j) add(q.mu[j] * q.y[j] - epsi);
+    add(q.zet * q.z - epsi);
+    for (int j = 0; j < m; ++j) add(q.lam[j] * q.s[j] - epsi);
+    norm = std::sqrt(ss);
   };
+  std::vector<double> gvec(m);
   while (epsi > 1e-7) {
-    hipLaunchKernelGGL(k_mma_trial, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, (const double*)nullptr,
-                       (const double*)nullptr, (const double*)nullptr, 0.0, sc.lam, epsi, (double*)nullptr,
+    hipLaunchKernelGGL(k_mma_trial, dim3(nb), dim3(256), 0, s, n, m, v, xs, xsi, eta, (const double*)nullptr,
+                       (const double*)nullptr, (const double*)nullptr, 0.0, pack(sc.lam), epsi, (double*)nullptr,
                        (double*)nullptr, (double*)nullptr, partial, nb);
-    fetch(3);
+    fetch(m + 2);
+    for (int j = 0; j < m; ++j) gvec[j] = psum(j);
     double norm, rmax;
-    residual(sc, psum(0), psum(1), pmax(2), norm, rmax);
+    residual(sc, gvec, psum(m), pmax(m + 1), norm, rmax);
     int inner = 0;
     while (rmax > 0.9 * epsi && inner < 200) {
       ++inner;
-      hipLaunchKernelGGL(k_mma_newton, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, sc.lam, epsi, partial, nb);
-      fetch(3);
-      const double gvec = psum(0), sgd = psum(1), sgg = psum(2);
-      const double dely = c + d * sc.y - sc.lam - epsi / sc.y;
+      hipLaunchKernelGGL(k_mma_newton, dim3(nb), dim3(256), 0, s, n, m, v, xs, xsi, eta, pack(sc.lam), epsi, partial,
+                         nb);
+      fetch(rows);
+      std::vector<double> dely(m), diagy(m), blam(m), alam((size_t)m * m);
+      for (int j = 0; j < m; ++j) {
+        gvec[j] = psum(j);
+        dely[j] = c + d * sc.y[j] - sc.lam[j] - epsi / sc.y[j];
+        const double dellam = gvec[j] - sc.y[j] - b[j] + epsi / sc.lam[j];
+        diagy[j] = d + sc.mu[j] / sc.y[j];
+        blam[j] = dellam + dely[j] / diagy[j] - psum(m + j);
+      }
+      for (int j = 0, q = 0; j < m; ++j)
+        for (int l = j; l < m; ++l, ++q) alam[(size_t)j * m + l] = alam[(size_t)l * m + j] = psum(2 * m + q);
+      for (int j = 0; j < m; ++j) alam[(size_t)j * m + j] += sc.s[j] / sc.lam[j] + 1.0 / diagy[j];
+      const std::vector<double> dlam = spd_solve(alam, blam, m);
       const double delz = a0 - epsi / sc.z;
-      const double dellam = gvec - sc.y - b + epsi / sc.lam;
-      const double diagy = d + sc.mu / sc.y;
-      const double diaglamyi = sc.s / sc.lam + 1.0 / diagy;
-      const double blam = dellam + dely / diagy - sgd;
-      const double alam = diaglamyi + sgg;
-      const double dlam = blam / alam;
       const double dz = -delz * sc.z / sc.zet;  // second block row with a = 0
-      hipLaunchKernelGGL(k_mma_direction, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, sc.lam, dlam, epsi, dx, dxsi,
-                         deta, partial);
+      hipLaunchKernelGGL(k_mma_direction, dim3(nb), dim3(256), 0, s, n, m, v, xs, xsi, eta, pack(sc.lam), pack(dlam),
+                         epsi, dx, dxsi, deta, partial);
       fetch(1);
-      const double dy = -dely / diagy + dlam / diagy;
-      const double dmu = -sc.mu + epsi / sc.y - sc.mu * dy / sc.y;
-      const double dzet = -sc.zet + epsi / sc.z - sc.zet * dz / sc.z;
-      const double ds = -sc.s + epsi / sc.lam - sc.s * dlam / sc.lam;
+      std::vector<double> dy(m), dmu(m), ds(m);
       double stm = pmax(0);
-      for (double r : {-1.01 * dy / sc.y, -1.01 * dz / sc.z, -1.01 * dlam / sc.lam, -1.01 * dmu / sc.mu,
-                       -1.01 * dzet / sc.zet, -1.01 * ds / sc.s})
-        stm = std::max(stm, r);
+      for (int j = 0; j < m; ++j) {
+        dy[j] = -dely[j] / diagy[j] + dlam[j] / diagy[j];
+        dmu[j] = -sc.mu[j] + epsi / sc.y[j] - sc.mu[j] * dy[j] / sc.y[j];
+        ds[j] = -sc.s[j] + epsi / sc.lam[j] - sc.s[j] * dlam[j] / sc.lam[j];
+        stm = std::max({stm, -1.01 * dy[j] / sc.y[j], -1.01 * dlam[j] / sc.lam[j], -1.01 * dmu[j] / sc.mu[j],
+                        -1.01 * ds[j] / sc.s[j]});
+      }
+      const double dzet = -sc.zet + epsi / sc.z - sc.zet * dz / sc.z;
+      stm = std::max({stm, -1.01 * dz / sc.z, -1.01 * dzet / sc.zet});
       double step = 1.0 / std::max(stm, 1.0);
-      const MmaScalars old = sc;
+      const MmaDual old = sc;
       double newnorm = 2.0 * norm;
       int tries = 0;
       while (newnorm > norm && tries < 50) {
         ++tries;
-        MmaScalars t{old.y + step * dy, old.z + step * dz, old.lam + step * dlam, old.mu + step * dmu,
-                     old.zet + step * dzet, old.s + step * ds};
-        hipLaunchKernelGGL(k_mma_trial, dim3(nb), dim3(256), 0, s, n, v, xs, xsi, eta, dx, dxsi, deta, step, t.lam,
-                           epsi, tx, tsi, te, partial, nb);
-        fetch(3);
-        residual(t, psum(0), psum(1), pmax(2), newnorm, rmax);
+        MmaDual t = old;
+        for (int j = 0; j < m; ++j) {
+          t.y[j] += step * dy[j];
+          t.lam[j] += step * dlam[j];
+          t.mu[j] += step * dmu[j];
+          t.s[j] += step * ds[j];
+        }
+        t.z += step * dz;
+        t.zet += step * dzet;
+        hipLaunchKernelGGL(k_mma_trial, dim3(nb), dim3(256), 0, s, n, m, v, xs, xsi, eta, dx, dxsi, deta, step,
+                           pack(t.lam), epsi, tx, tsi, te, partial, nb);
+        fetch(m + 2);
+        for (int j = 0; j < m; ++j) gvec[j] = psum(j);
+        residual(t, gvec, psum(m), pmax(m + 1), newnorm, rmax);
         sc = t;
         step *= 0.5;
       }
@@ -508,8 +695,8 @@ static void mma_step(long long n, int it, double* x, double* xold1, double* xold
   hipLaunchKernelGGL(k_copy, dim3(ge), dim3(256), 0, s, xs, x, n);
   hipLaunchKernelGGL(k_clip01, dim3(ge), dim3(256), 0, s, x, n);
   DFMI_HIP(hipGetLastError());
-  for (auto p : {alfa, beta, p0, q0, pp, qq, xs, xsi, eta, dx, dxsi, deta, tx, tsi, te, partial})
-    DFMI_HIP(hipFreeAsync(p, s));
+  for (auto p : buf) DFMI_HIP(hipFreeAsync(p, s));
+  for (auto p : {pp, qq, partial}) DFMI_HIP(hipFreeAsync(p, s));
   DFMI_HIP(hipStreamSynchronize(s));
 }
 
@@ -574,21 +761,10 @@ int dfmi_filter_create(const int32_t shape[3], const double spacing[3], double r
     F.g.ny = shape[1];
     F.g.nz = shape[2];
     F.n = (long long)shape[0] * shape[1] * shape[2];
+    for (int a = 0; a < 3; ++a) F.spacing[a] = spacing[a];
     std::vector<int> offs;
     std::vector<double> w;
-    int pad[3];
-    for (int a = 0; a < 3; ++a) pad[a] = (int)std::floor(radius / spacing[a]);
-    for (int i = -pad[0]; i <= pad[0]; ++i)
-      for (int j = -pad[1]; j <= pad[1]; ++j)
-        for (int l = -pad[2]; l <= pad[2]; ++l) {
-          const double dd = std::sqrt(std::pow(i * spacing[0], 2) + std::pow(j * spacing[1], 2) +
-                                      std::pow(l * spacing[2], 2));
-          const double wt = std::max(radius - dd, 0.0);
-          if (wt > 0) {
-            offs.insert(offs.end(), {i, j, l});
-            w.push_back(wt);
-          }
-        }
+    cone_taps(spacing, radius, offs, w);
     F.g.taps = (int)w.size();
     F.nb = std::max(1, std::min(256, (int)((F.n + 1023) / 1024)));
     DFMI_HIP(hipMalloc(&F.offs, sizeof(int) * offs.size() + 4));
@@ -638,14 +814,58 @@ int dfmi_filter_forward(dfmi_filter* f, const double* x, double* out) {
 int dfmi_filter_pullback(dfmi_filter* f, const double* sens, const double* dproj, double* out) {
   DFMI_GUARD2({
     Filter& F = f->f;
-    // out = a (W (a s dproj / sums)), zero outside the free cells
-    hipLaunchKernelGGL(k_mul, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, F.stream, sens, dproj,
-                       (double*)F.tmp, F.n);
-    hipLaunchKernelGGL(k_cone, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, F.stream, F.g, (const int*)F.offs,
-                       (const double*)F.w, (const uint8_t*)F.allowed, (const double*)F.tmp, (const double*)F.inv,
-                       nullptr, (const uint8_t*)F.free_, out);
-    DFMI_HIP(hipGetLastError());
+    F.pullback(sens, dproj, out);
     DFMI_HIP(hipStreamSynchronize(F.stream));
+  })
+}
+
+int dfmi_filter_set_corridor(dfmi_filter* f, double radius, const double* weights) {
+  DFMI_GUARD2({
+    Filter& F = f->f;
+    if (!weights) throw std::invalid_argument("corridor weights missing");
+    if (!(radius > 0)) throw std::invalid_argument("corridor radius must be positive");
+    std::vector<uint8_t> allowed(F.n);
+    DFMI_HIP(hipMemcpy(allowed.data(), F.allowed, F.n, hipMemcpyDeviceToHost));
+    std::vector<double> blend(F.n);
+    std::vector<uint8_t> support(F.n);
+    bool any = false;
+    for (long long i = 0; i < F.n; ++i) {
+      blend[i] = allowed[i] ? std::min(std::max(weights[i], 0.0), 1.0) : 0.0;
+      support[i] = blend[i] > 0;
+      any = any || support[i];
+    }
+    F.corridor = false;
+    if (!any) return 0;  // no allowed cell carries a corridor weight: the plain filter stays
+    std::vector<int> offs;
+    std::vector<double> w;
+    cone_taps(F.spacing, radius, offs, w);
+    F.g2 = F.g;
+    F.g2.taps = (int)w.size();
+    for (void** p : {&F.offs2, &F.w2, &F.blend, &F.support, &F.inv2, &F.tmp3})
+      if (*p) (void)hipFree(*p), *p = nullptr;
+    DFMI_HIP(hipMalloc(&F.offs2, sizeof(int) * offs.size() + 4));
+    DFMI_HIP(hipMalloc(&F.w2, sizeof(double) * w.size() + 8));
+    DFMI_HIP(hipMalloc(&F.support, F.n));
+    for (void** p : {&F.blend, &F.inv2, &F.tmp3}) DFMI_HIP(hipMalloc(p, sizeof(double) * F.n));
+    DFMI_HIP(hipMemcpy(F.offs2, offs.data(), sizeof(int) * offs.size(), hipMemcpyHostToDevice));
+    DFMI_HIP(hipMemcpy(F.w2, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    DFMI_HIP(hipMemcpy(F.blend, blend.data(), sizeof(double) * F.n, hipMemcpyHostToDevice));
+    DFMI_HIP(hipMemcpy(F.support, support.data(), F.n, hipMemcpyHostToDevice));
+    // wide sums (W_wide a) a on the corridor support, 1 elsewhere
+    std::vector<double> ones(F.n, 1.0), sums(F.n);
+    DFMI_HIP(hipMemcpy(F.tmp3, ones.data(), sizeof(double) * F.n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_cone, dim3((unsigned)((F.n + 255) / 256)), dim3(256), 0, nullptr, F.g2, (const int*)F.offs2,
+                       (const double*)F.w2, (const uint8_t*)F.allowed, (const double*)F.tmp3, nullptr, nullptr,
+                       (const uint8_t*)F.support, (double*)F.inv2);
+    DFMI_HIP(hipGetLastError());
+    DFMI_HIP(hipMemcpy(sums.data(), F.inv2, sizeof(double) * F.n, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < F.n; ++i) {
+      if (!support[i]) sums[i] = 1.0;
+      if (!(sums[i] > 0)) throw std::runtime_error("corridor filter has an empty allowed-cell neighbourhood");
+      sums[i] = 1.0 / sums[i];
+    }
+    DFMI_HIP(hipMemcpy(F.inv2, sums.data(), sizeof(double) * F.n, hipMemcpyHostToDevice));
+    F.corridor = true;
   })
 }
 
@@ -714,12 +934,15 @@ int dfmi_oc_update(dfmi_filter* f, const double* x, const double* dc, const doub
   })
 }
 
-int dfmi_mma_step(int64_t n, int32_t iteration, double* x, double* xold1, double* xold2, double* low, double* upp,
-                  double f0, const double* df0, double g, const double* dg, double move, double scale,
-                  void* stream) {
+int dfmi_mma_step(int64_t n, int32_t m, int32_t iteration, double* x, double* xold1, double* xold2, double* low,
+                  double* upp, double f0, const double* df0, const double* g, const double* dg, double move,
+                  double scale, void* stream) {
   DFMI_GUARD2({
+    (void)f0;  // the subproblem does not use the objective value (mmasub only passes it through)
     if (n < 1) throw std::invalid_argument("MMA needs n >= 1");
-    mma_step((long long)n, iteration, x, xold1, xold2, low, upp, f0, df0, g, dg, move, scale, (hipStream_t)stream);
+    if (m < 1 || m > MMA_MAX_M) throw std::invalid_argument("MMA takes 1 to 16 constraints");
+    if (!x || !xold1 || !xold2 || !low || !upp || !df0 || !g || !dg) throw std::invalid_argument("null MMA argument");
+    mma_step((long long)n, m, iteration, x, xold1, xold2, low, upp, df0, g, dg, move, scale, (hipStream_t)stream);
   })
 }
 

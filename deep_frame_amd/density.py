@@ -7,8 +7,9 @@ eta - delta, :1114-1116), ``volume`` (:1148-1151; the dilated field when robust)
 (:1152-1170), ``pullback`` (:1171-1178) and ``initial`` (:1179-1195, volume bisection); and the row
 pullbacks of ``TopologyProblem.evaluate`` (deep_frame/topology_problem.py:366-379).  The filter is
 libdfmi's cone kernel (``dfmi_filter_*``), the projection ``dfmi_project`` (all thresholds in one pass),
-the weighted volume one ``dfmi_gram`` dot.  Volume weights of soft propeller discs (``volume_weights``,
-:827-837) are computed on the host once.  The corridor blend (``settings["corridor"]``) is not built.
+the weighted volume one ``dfmi_gram`` dot; the corridor blend (``settings["corridor"]``, :1125-1129, :1142,
+:1176) is the filter's second, wider cone (``dfmi_filter_set_corridor``).  Volume weights of soft propeller
+discs (``volume_weights``, :827-837) are computed on the host once.
 """
 from ctypes import POINTER, c_double
 
@@ -23,17 +24,23 @@ DENSITY = {"filter_radius_mm": 6.0, "projection_beta": 1.0, "projection_eta": 0.
 
 
 def volume_weights(points, discs):
-    """Cell volume weights: 1, plus the soft propeller-disc share (topology_optimization.py:827-837)."""
-    weights = np.ones(len(points))
+    """Volume weight of each cell centre (n, 3): 1 outside the propeller discs; with soft discs
+    (``discs["mode"] == "soft"``) 1 + weight * s * exp(-|z - plane| / length), where s is the largest disc
+    profile over the motors: 1 inside a disc of ``radius_mm`` about a motor axis (x, y), or with
+    ``radial_power`` the ramp ((r - hub) / (radius - hub))^power clipped to [0, 1], and 0 outside every disc
+    (the soft-disc weighting of topology_optimization.py:827-837)."""
+    p = np.asarray(points, dtype=float).reshape(-1, 3)
     if not discs or discs.get("mode") != "soft":
-        return weights
-    points = np.asarray(points, dtype=float)
-    share, hub, power = np.zeros(len(points)), discs.get("hub_radius_mm", 0.0), discs.get("radial_power")
-    for x, y in discs["motors_mm"]:
-        radial = np.hypot(points[:, 0] - x, points[:, 1] - y)
-        profile = np.ones(len(points)) if power is None else np.clip((radial - hub) / (discs["radius_mm"] - hub), 0.0, 1.0) ** power
-        share = np.maximum(share, np.where(radial <= discs["radius_mm"], profile, 0.0))
-    return weights + discs["weight"] * share * np.exp(-np.abs(points[:, 2] - discs["plane_mm"]) / discs["length_mm"])
+        return np.ones(len(p))
+    motors = np.asarray(discs["motors_mm"], dtype=float).reshape(-1, 2)
+    radius, hub = float(discs["radius_mm"]), float(discs.get("hub_radius_mm", 0.0))
+    power = discs.get("radial_power")
+    # distance of every cell centre to every motor axis: (motors, cells)
+    r = np.hypot(p[None, :, 0] - motors[:, :1], p[None, :, 1] - motors[:, 1:])
+    profile = np.ones_like(r) if power is None else np.clip((r - hub) / (radius - hub), 0.0, 1.0) ** power
+    share = np.max(np.where(r <= radius, profile, 0.0), axis=0, initial=0.0)
+    decay = np.exp(-np.abs(p[:, 2] - discs["plane_mm"]) / discs["length_mm"])
+    return 1.0 + discs["weight"] * share * decay
 
 
 class DensityMap:
@@ -46,8 +53,6 @@ class DensityMap:
         self.settings = s = {**DENSITY, **(settings or {})}
         if s["projection"] not in ("single", "robust"):
             raise ValueError("Invalid topology projection")
-        if s["corridor"] is not None:
-            raise NotImplementedError("the corridor blend of the density filter is not part of this build")
         self.shape = tuple(int(n) for n in shape)
         self.n = int(np.prod(self.shape))
         masks = [np.asarray(m, dtype=bool).reshape(-1) for m in (allowed, preserve, forbidden)]
@@ -67,6 +72,9 @@ class DensityMap:
             else {"intermediate": eta}
         self.beta = s["beta_schedule"][0] if s["beta_schedule"] else s["projection_beta"]
         self.filter = DensityFilter(self.shape, spacing, s["filter_radius_mm"], self.allowed, self.preserve)
+        corridor = s["corridor"]
+        if corridor is not None:  # blended with a wider cone where the corridor weights are positive
+            self.filter.set_corridor(corridor["radius_mm"], corridor["weights"])
         flags = (self.preserve.astype(np.uint8) | (self.forbidden.astype(np.uint8) << 1)
                  | (self.free.astype(np.uint8) << 2))
         self.flags = torch.as_tensor(flags, device="cuda")
@@ -146,23 +154,29 @@ class DensityMap:
         return self.filter.pullback(sensitivity, projection_derivative)
 
     def initial(self, target):
-        """Uniform free density meeting the volume target by bisection (topology_optimization.py:1179-1195)."""
-        design = np.zeros(self.n)
-        design[self.preserve] = 1
-        lower = self.settings["minimum_design_density"]
-        design[self.free] = lower
-        if self.volume(design) > target + 1e-8:
-            raise ValueError("Volume budget cannot contain preserves and their filtered transition")
-        upper = 1.0
-        for _ in range(70):
-            value = (lower + upper) / 2
-            design[self.free] = value
-            if self.volume(design) > target:
-                upper = value
+        """Uniform start design: preserved cells 1, every free cell the largest common density (at least
+        ``minimum_design_density``) whose volume stays within ``target``, found by bisection down to adjacent
+        doubles (the start of topology_optimization.py:1179-1195)."""
+        base = np.where(self.preserve, 1.0, 0.0)
+
+        def design(value):
+            out = base.copy()
+            out[self.free] = value
+            return out
+
+        low, high = float(self.settings["minimum_design_density"]), 1.0
+        if self.volume(design(low)) > target + 1e-8:
+            raise ValueError(f"volume target {target} lies below the volume of the preserved cells with every "
+                             f"free cell at the minimum density {low}")
+        while True:
+            middle = 0.5 * (low + high)
+            if not low < middle < high:
+                break
+            if self.volume(design(middle)) > target:
+                high = middle
             else:
-                lower = value
-        design[self.free] = lower
-        return design
+                low = middle
+        return design(low)
 
     def close(self):
         self.filter.close()

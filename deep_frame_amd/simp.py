@@ -60,6 +60,15 @@ class DensityFilter:
         _native.call("dfmi_filter_pullback", self.handle, _native.ptr(s), _native.ptr(d), _native.ptr(out))
         return out
 
+    def set_corridor(self, radius, weights):
+        """Blend a cone of radius ``radius`` into the filter with per-cell weights clip(weights, 0, 1) on
+        allowed cells (DensityMap's corridor, topology_optimization.py:1125-1129, :1142, :1176)."""
+        w = np.ascontiguousarray(np.asarray(weights, dtype=float).ravel())
+        if w.size != self.n:
+            raise ValueError("corridor weights need one value per cell")
+        _native.call("dfmi_filter_set_corridor", self.handle, float(radius),
+                     w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
     def volume(self, x):
         x = self._dev(x)
         v = c_double()
@@ -109,14 +118,20 @@ class MMAState:
         self.low, self.upp = torch.zeros_like(x), torch.ones_like(x)
 
     def step(self, x, f0, df0, g, dg, move=MMA["move"], scale=MMA["scale"]):
-        """One MMA update with one constraint g <= 0; returns the new design (bounds [0, 1])."""
+        """One MMA update for the constraints g_j <= 0 (g a scalar or m values, dg (n,) or (m, n) device
+        tensors), as MMAOptimizer.step passes every constraint row (topology_optimizers.py:55-67, m = len(g));
+        returns the new design (bounds [0, 1])."""
         torch = _torch()
         x = x.to(device="cuda", dtype=torch.float64).contiguous().clone()
         df0 = df0.to(device="cuda", dtype=torch.float64).contiguous()
-        dg = dg.to(device="cuda", dtype=torch.float64).contiguous()
-        _native.call("dfmi_mma_step", x.numel(), self.iteration, _native.ptr(x), _native.ptr(self.old1),
+        g = np.atleast_1d(np.asarray(g, dtype=float)).ravel()
+        dg = dg.to(device="cuda", dtype=torch.float64).reshape(len(g), -1).contiguous()
+        if dg.shape[1] != x.numel():
+            raise ValueError("dg needs one row of n values per constraint")
+        _native.call("dfmi_mma_step", x.numel(), len(g), self.iteration, _native.ptr(x), _native.ptr(self.old1),
                      _native.ptr(self.old2), _native.ptr(self.low), _native.ptr(self.upp), float(f0), _native.ptr(df0),
-                     float(g), _native.ptr(dg), float(move), float(scale), _native.stream_handle(torch))
+                     np.ascontiguousarray(g).ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _native.ptr(dg),
+                     float(move), float(scale), _native.stream_handle(torch))
         self.iteration += 1
         return x
 

@@ -234,6 +234,11 @@ int dfmi_filter_destroy(dfmi_filter* f);
 int dfmi_filter_set_stream(dfmi_filter* f, void* hip_stream);
 int dfmi_filter_forward(dfmi_filter* f, const double* x_dev, double* out_dev);
 int dfmi_filter_pullback(dfmi_filter* f, const double* sens_dev, const double* dproj_dev, double* out_dev);
+/* Corridor blend of DensityMap (topology_optimization.py:1125-1129, :1142, :1176): a second cone of radius
+ * `radius` whose normalised field is blended in with per-cell weights w = clip(weights, 0, 1) on allowed cells
+ * (host array of Ne): forward (1 - w) x~ + w x~_wide, pullback through both cones.  Without a positive weight
+ * on an allowed cell the filter stays plain. */
+int dfmi_filter_set_corridor(dfmi_filter* f, double radius, const double* weights);
 /* Sum of the filtered field over elements with weights (NULL = 1): the volume of DensityMap.volume
  * with the identity projection (topology_optimization.py:1148-1151). */
 int dfmi_filter_volume(dfmi_filter* f, const double* x_dev, double* volume);
@@ -242,10 +247,13 @@ int dfmi_filter_volume(dfmi_filter* f, const double* x_dev, double* volume);
 int dfmi_oc_update(dfmi_filter* f, const double* x_dev, const double* dc_dev, const double* dv_dev, double target,
                    double xmin, double move, double* out_dev);
 
-/* One MMA step (Svanberg's method as called by deep_frame/topology_optimizers.py:55-67), m = 1:
- * state arrays low/upp/xold1/xold2 (n, device) are updated in place; x_dev receives the new design. */
-int dfmi_mma_step(int64_t n, int32_t iteration, double* x_dev, double* xold1_dev, double* xold2_dev,
-                  double* low_dev, double* upp_dev, double f0, const double* df0_dev, double g,
+/* One MMA step (Svanberg's method as MMAOptimizer.step calls mmapy.mmasub, deep_frame/topology_optimizers.py:55-67,
+ * with a0 = 1, a = 0, c = 1e4, d = 1, bounds [0, 1]) for m (1..16) constraints g_j <= 0 (m = len(g), :58):
+ * g host array of m, dg_dev the m x n constraint gradients row-major (device), f0/df0 and g/dg multiplied by
+ * `scale` inside; state arrays low/upp/xold1/xold2 (n, device) are updated in place; x_dev receives the new
+ * design. */
+int dfmi_mma_step(int64_t n, int32_t m, int32_t iteration, double* x_dev, double* xold1_dev, double* xold2_dev,
+                  double* low_dev, double* upp_dev, double f0, const double* df0_dev, const double* g,
                   const double* dg_dev, double move, double scale, void* hip_stream);
 
 /* ---- Learned coarse correction (no counterpart in the reference; the nearest pattern is the

@@ -33,7 +33,11 @@ def _neighbour_sum(field, offs, w):
 
 
 class Cone:
-    def __init__(self, allowed, spacing, radius, preserve=None):
+    """With ``corridor`` = (radius, weights) the filtered field is blended with a wider cone,
+    (1 - b) x~ + b x~_wide with b = clip(weights, 0, 1) on allowed cells and the wide sums 1 where b = 0
+    (DensityMap corridor, topology_optimization.py:1125-1129, :1142, :1176)."""
+
+    def __init__(self, allowed, spacing, radius, preserve=None, corridor=None):
         self.allowed = np.asarray(allowed, bool)
         self.shape = self.allowed.shape
         self.preserve = np.zeros(self.shape, bool) if preserve is None else np.asarray(preserve, bool)
@@ -42,19 +46,34 @@ class Cone:
         a = self.allowed.astype(float)
         self.sums = _neighbour_sum(a, self.offs, self.w) * a
         self.sums[~self.allowed] = 1.0
+        self.blend = None
+        if corridor is not None:
+            blend = np.clip(np.asarray(corridor[1], float).reshape(self.shape), 0, 1) * a
+            if np.any(blend > 0):
+                self.blend = blend
+                self.offs2, self.w2 = cone_weights(spacing, corridor[0])
+                self.sums2 = _neighbour_sum(a, self.offs2, self.w2) * a
+                self.sums2[blend <= 0] = 1.0
 
-    def apply(self, x):
+    def apply(self, x, wide=False):
         a = self.allowed.astype(float)
-        return _neighbour_sum(np.asarray(x, float).reshape(self.shape) * a, self.offs, self.w) * a
+        offs, w = (self.offs2, self.w2) if wide else (self.offs, self.w)
+        return _neighbour_sum(np.asarray(x, float).reshape(self.shape) * a, offs, w) * a
 
     def filtered(self, design):
-        return (self.apply(design) / self.sums).ravel()
+        out = self.apply(design) / self.sums
+        if self.blend is not None:
+            out = (1 - self.blend) * out + self.blend * self.apply(design, True) / self.sums2
+        return out.ravel()
 
     def pullback(self, sensitivity, dproj=None):
         s = np.asarray(sensitivity, float).reshape(self.shape)
         if dproj is not None:
             s = s * np.asarray(dproj, float).reshape(self.shape)
-        out = self.apply(s / self.sums)
+        if self.blend is None:
+            out = self.apply(s / self.sums)
+        else:
+            out = self.apply((1 - self.blend) * s / self.sums) + self.apply(self.blend * s / self.sums2, True)
         out[~self.free] = 0.0
         return out.ravel()
 

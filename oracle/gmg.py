@@ -19,6 +19,7 @@ Restates ``deep_frame/topology_multigrid.py``:
     system of the coarse-grid correction is solved by a few flexible-CG steps preconditioned by the
     level-1 V-cycle (``fcg``, ``kcycle``),
   * flexible PCG with Polak-Ribiere beta and true-residual confirmation (:407-454).
+The smoothers and cycles also take an (ndof, k) block of independent columns.
 Matrices are SciPy sparse over padded node grids with DOF = 3*node + comp.
 Test infrastructure only.
 """
@@ -136,8 +137,13 @@ def spectrum(level, iterations=60, seed=0):
     return est
 
 
+def _cols(v, like):
+    """A per-DOF vector broadcast against a vector or an (ndof, k) block of columns."""
+    return v if np.ndim(like) == 1 else v[:, None]
+
+
 def chebyshev(level, rhs, x, count, lmax, ratio=20.0):
-    a, dinv = level["A"], level["dinv"]
+    a, dinv = level["A"], _cols(level["dinv"], rhs)
     upper = 1.1 * lmax
     lower = upper / ratio
     theta, delta = (upper + lower) / 2, (upper - lower) / 2
@@ -158,8 +164,9 @@ def chebyshev(level, rhs, x, count, lmax, ratio=20.0):
 def jacobi(level, rhs, x, count, lmax, damping=1.0):
     """Damped Jacobi sweeps x += (damping / lmax) Dinv (rhs - A x), the first from x = 0 when x is None."""
     step = damping / lmax
+    dinv = _cols(level["dinv"], rhs)
     for _ in range(count):
-        x = step * level["dinv"] * rhs if x is None else x + step * level["dinv"] * (rhs - level["A"] @ x)
+        x = step * dinv * rhs if x is None else x + step * dinv * (rhs - level["A"] @ x)
     return x
 
 
@@ -182,10 +189,10 @@ def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0, smoother="chebyshev
     x = smooth(None)
     p = levels[index + 1]["P"]
     for _ in range(2 if cycle == "W" and index + 2 < len(levels) else 1):
-        res = (rhs - lv["A"] @ x) * lv["mask"]
-        corr = vcycle(levels, lmax, (p.T @ res) * levels[index + 1]["mask"], index + 1, degree, growth, smoother,
-                      sweeps, damping, cycle)
-        x = x + (p @ corr) * lv["mask"]
+        res = (rhs - lv["A"] @ x) * _cols(lv["mask"], rhs)
+        corr = vcycle(levels, lmax, (p.T @ res) * _cols(levels[index + 1]["mask"], rhs), index + 1, degree, growth,
+                      smoother, sweeps, damping, cycle)
+        x = x + (p @ corr) * _cols(lv["mask"], rhs)
     return smooth(x)
 
 

@@ -151,3 +151,27 @@ def test_solve_on_exported_075mm_half_frame_density(torch_gpu, support):
     rel = (torch.linalg.vector_norm(res, dim=1) / torch.linalg.vector_norm(f, dim=1)).cpu().numpy()
     assert np.all(rel < 1.5e-8)
     mg.close()
+
+
+@pytest.mark.parametrize("radii", [(1.5, 3.0), (2.5, 2.0)])
+def test_corridor_blend_matches_oracle(torch_gpu, radii):
+    """DensityMap's corridor (topology_optimization.py:1125-1129, :1142, :1176): a wider (or narrower) cone
+    blended in with per-cell weights; forward, pullback and the OC volume go through the blended filter."""
+    from deep_frame_amd.density import DensityMap
+    torch = torch_gpu
+    shape, spacing = (10, 8, 6), (1.0, 0.8, 1.2)
+    allowed, preserve, forbidden = _domain(shape, seed=3)
+    rng = np.random.default_rng(5)
+    weights = np.where(rng.random(shape) < 0.4, rng.uniform(-0.2, 1.3, shape), 0.0)
+    corridor = {"radius_mm": radii[1], "weights": weights.ravel()}
+    dm = DensityMap(shape, spacing, allowed, preserve, forbidden,
+                    {"filter_radius_mm": radii[0], "projection_beta": 0.0, "corridor": corridor})
+    ref = filters.Cone(allowed, spacing, radii[0], preserve, corridor=(radii[1], weights))
+    x = rng.uniform(0, 1, allowed.size)
+    got = dm.filtered(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    assert np.max(np.abs(got - ref.filtered(x))) < 1e-13
+    s, d = rng.normal(size=allowed.size), rng.uniform(0.5, 1.5, allowed.size)
+    back = dm.pullback(torch.as_tensor(s, device="cuda"), torch.as_tensor(d, device="cuda")).cpu().numpy()
+    assert np.max(np.abs(back - ref.pullback(s, d))) < 1e-12 * np.abs(ref.pullback(s, d)).max()
+    assert dm.filter.volume(torch.as_tensor(x, device="cuda")) == pytest.approx(ref.filtered(x).sum(), rel=1e-13)
+    dm.close()

@@ -66,26 +66,32 @@ def test_oc_update_matches_oracle(torch_gpu):
 
 
 @pytest.mark.parametrize("iteration", [1, 4])
-def test_mma_step_matches_oracle(iteration, torch_gpu):
+@pytest.mark.parametrize("m", [1, 3])
+def test_mma_step_matches_oracle(iteration, m, torch_gpu):
+    """One device MMA step with m constraints (MMAOptimizer.step passes m = len(g) rows,
+    topology_optimizers.py:55-67) against the oracle, whose steps are pinned by hand-derived cases."""
     from deep_frame_amd.simp import MMAState
-    rng = np.random.default_rng(6 + iteration)
+    print(f"mma iteration={iteration} m={m}", flush=True)
+    rng = np.random.default_rng(6 + iteration + 10 * m)
     n = 500
     x = rng.uniform(0.2, 0.8, n)
     old1 = np.clip(x + rng.normal(0, 0.05, n), 0, 1)
     old2 = np.clip(old1 + rng.normal(0, 0.05, n), 0, 1)
     low, upp = x - 0.3, x + 0.3
     df0 = rng.uniform(-1.0, 0.2, n)
-    dg = rng.uniform(0.5, 1.5, n) / n
-    g = float(x @ dg - 0.4)
-    exp, elow, eupp, _, _ = mma.mma_step(iteration, x, old1, old2, low, upp, 0.0, 1.0, 1.0, df0, g, dg[None, :],
+    dg = rng.uniform(0.5, 1.5, (m, n)) / n
+    dg[1:] *= rng.choice([-1.0, 1.0], size=(m - 1, 1))
+    g = dg @ x - rng.uniform(0.3, 0.5, m)
+    exp, elow, eupp, _, _ = mma.mma_step(iteration, x, old1, old2, low, upp, 0.0, 1.0, 1.0, df0, g, dg,
                                          move=0.1, scale=100.0)
     t = torch_gpu
-    dev = lambda a: t.as_tensor(a, device="cuda")
+    dev = lambda a: t.as_tensor(np.ascontiguousarray(a), device="cuda")
     st = MMAState(dev(x))
     st.iteration = iteration
     st.old1, st.old2 = dev(old1), dev(old2)
     st.low, st.upp = dev(low), dev(upp)
-    out = st.step(dev(x), 1.0, dev(df0), g, dev(dg), move=0.1, scale=100.0).cpu().numpy()
+    out = st.step(dev(x), 1.0, dev(df0), g if m > 1 else float(g[0]), dev(dg if m > 1 else dg[0]), move=0.1,
+                  scale=100.0).cpu().numpy()
     assert np.max(np.abs(out - np.clip(exp, 0, 1))) < 1e-6
     assert np.allclose(st.low.cpu().numpy(), elow, atol=1e-14) and np.allclose(st.upp.cpu().numpy(), eupp, atol=1e-14)
     assert np.allclose(st.old1.cpu().numpy(), x) and np.allclose(st.old2.cpu().numpy(), old1)

@@ -1,8 +1,8 @@
 """Host logic of the LOBPCG eigensolver and the modal constraint (deep_frame_amd/modal.py) on the CPU:
 the device passes (K V, M V, V-cycle, mask, Gram matrices, block combinations, modal sensitivity) are
-replaced by assembled sparse matrices of the oracle, so that the block algebra, the SVQB/Rayleigh-Ritz
-bookkeeping, the refresh and convergence logic and the aggregate are checked against shift-invert eigsh
-without a GPU.  The device passes themselves are checked in test_gpu_modal.py."""
+replaced by assembled sparse matrices of the oracle, so that the block algebra, the M-orthonormalisation and
+Rayleigh-Ritz bookkeeping, the refresh and convergence logic and the aggregate are checked against
+shift-invert eigsh without a GPU.  The device passes themselves are checked in test_gpu_modal.py."""
 import types
 
 import numpy as np
@@ -144,3 +144,31 @@ def test_modal_constraint_host_logic_matches_oracle(fake):
                                      1.2, target, 40.0, 0.1)
     assert violation == pytest.approx(ev, rel=1e-9) and info["f1_hz"] == pytest.approx(f1, rel=1e-9)
     assert np.linalg.norm(gradient - eg) < 1e-7 * np.linalg.norm(eg)
+
+
+def test_modal_constraint_lumped_slope_and_augmented_call(fake):
+    """``lumped_slope`` subtracts sum_i w_i lambda_i slope(u_i) / target from the sensitivity of KS, and the
+    call form returns the augmented-Lagrangian penalty of the violation with the multiplier it used."""
+    mg, ke, me, pts, conn, dofs, fixed = fake((8, 2, 3), (1.0, 1.0, 1.0))
+    density = np.random.default_rng(5).uniform(0.05, 1.0, mg.nelem)
+    settings = {"f1_min_hz": 2000.0, "modes": 3, "tracked": 2, "ks": 40.0, "mass_cutoff": 0.1, "penalty": 10.0,
+                "multiplier_interval": 1}
+    plain = modal.MultigridModal(mg, [{"key": "m", "fixed": fixed}], 2000.0, 1.2, settings=settings)
+    violation, gradient, info = plain.measure(density)
+    field = np.random.default_rng(6).uniform(0.0, 1.0, mg.nelem)
+    hooked = modal.MultigridModal(mg, [{"key": "m", "fixed": fixed}], 2000.0, 1.2, settings=settings)
+    seen = []
+    hooked.lumped_slope = lambda mode: (seen.append(mode.shape), field)[1]
+    _, hooked_gradient, _ = hooked.measure(density)
+    values = np.array([f for _, f in info["frequencies_hz"]])
+    values = (2 * np.pi * values) ** 2
+    target = (2 * np.pi * 2000.0) ** 2
+    ratios = values / target
+    weights = np.exp(-40.0 * (ratios - ratios.min()))
+    weights /= weights.sum()
+    assert seen == [(mg.ndof,)] * 2
+    assert np.allclose(hooked_gradient - gradient, np.sum(weights * values) * field / target, rtol=1e-7, atol=0)
+    penalty, augmented, details = plain(density)
+    shifted = max(violation, 0.0)
+    assert penalty == pytest.approx(5.0 * shifted ** 2, rel=1e-9) and details["multiplier"] == 0.0
+    assert np.allclose(augmented, 10.0 * shifted * gradient, rtol=1e-7, atol=1e-300)

@@ -186,3 +186,118 @@ def test_rediscretized_mask_is_the_injected_fine_mask():
     # Galerkin keeps the coarse face DOFs (their prolongation reaches the free x = 1 fine nodes)
     galerkin = gmg.hierarchy(shape, ke, np.ones(128), fixed, (3, shape))
     assert np.all(galerkin[1]["mask"].reshape(-1, 3)[face] == 1)
+
+
+# ---- MMA (Svanberg 1987 / KTH note 2007; mmasub as called by topology_optimizers.py:55-67) ----------------
+# Hand-derived numbers: bounds [0, 1] (span 1), asyinit 0.5, asydecr 0.7, asyincr 1.2, asymptote clamps
+# x -+ 10 span and x -+ 0.01 span, albefa 0.1, raa0 1e-5, the 0.001 (p + q) term.
+
+def test_mma_asymptotes_over_four_iterations():
+    from oracle import mma
+    x = np.array([0.3, 0.6])
+    for it in (1, 2):  # low = x - 0.5, upp = x + 0.5
+        low, upp = mma.asymptotes(it, x, x, x, None, None, 0.0, 1.0)
+        assert np.allclose(low, [-0.2, 0.1], atol=1e-15) and np.allclose(upp, [0.8, 1.1], atol=1e-15)
+    # iteration 3: variable 0 keeps its direction (factor 1.2), variable 1 stalled (factor 1)
+    low, upp = mma.asymptotes(3, np.array([0.5, 0.5]), np.array([0.4, 0.6]), np.array([0.3, 0.6]),
+                              np.array([0.0, 0.2]), np.array([0.9, 1.0]), 0.0, 1.0)
+    assert np.allclose(low, [0.02, 0.1], atol=1e-15) and np.allclose(upp, [1.1, 0.9], atol=1e-15)
+    # iteration 4: variable 0 oscillates (factor 0.7, upp clamped up to x + 0.01), variable 1 keeps its
+    # direction (factor 1.2, low clamped up to x - 10)
+    low, upp = mma.asymptotes(4, np.array([0.5, 0.45]), np.array([0.6, 0.5]), np.array([0.4, 0.52]),
+                              np.array([0.3, -30.0]), np.array([0.605, 0.9]), 0.0, 1.0)
+    assert np.allclose(low, [0.29, -9.55], atol=1e-14) and np.allclose(upp, [0.51, 0.93], atol=1e-14)
+
+
+def test_mma_approximation_terms():
+    from oracle import mma
+    # x = 0.5 between asymptotes 0 and 1, df0 = 2, dg = -1 (already scaled), g = -0.1, move 0.1
+    alfa, beta, p0, q0, p, q, b = mma.approximation(np.array([0.5]), np.array([0.0]), np.array([1.0]), 0.0, 1.0,
+                                                    np.array([2.0]), np.array([-0.1]), np.array([[-1.0]]), 0.1)
+    assert alfa[0] == pytest.approx(0.4) and beta[0] == pytest.approx(0.6)  # move limits inside the albefa bounds
+    assert p0[0] == pytest.approx((2 + 0.002 + 1e-5) * 0.25, rel=1e-14)  # 0.5005025
+    assert q0[0] == pytest.approx((0.002 + 1e-5) * 0.25, rel=1e-14)      # 0.0005025
+    assert p[0, 0] == pytest.approx((0.001 + 1e-5) * 0.25, rel=1e-14)    # 0.0002525
+    assert q[0, 0] == pytest.approx((1.001 + 1e-5) * 0.25, rel=1e-14)    # 0.2502525
+    assert b[0] == pytest.approx(0.0002525 / 0.5 + 0.2502525 / 0.5 + 0.1, rel=1e-14)
+
+
+def _root(a2, a1, a0):
+    """Larger root of a2 x^2 + a1 x + a0."""
+    return (-a1 + np.sqrt(a1 * a1 - 4 * a2 * a0)) / (2 * a2)
+
+
+@pytest.mark.parametrize("case", ["one_active", "second_inactive", "both_active"])
+def test_mma_subproblem_closed_form_solutions(case):
+    """Two variables at x = 0.5 (iteration 1: asymptotes 0 and 1, move 0.1 -> [0.4, 0.6]), objective
+    slope -1 and constraint slopes +1 (scaled), so each term is p / (1 - x) + q / x with p = 0.2502525,
+    q = 0.0002525 (constraints) and p0 = 0.0002525, q0 = 0.2502525 (objective).  An active constraint j
+    fixes its variables by sum_i [p / (1 - x_i) + q / x_i] = b_j, a quadratic in x."""
+    from oracle import mma
+    x = np.array([0.5, 0.5])
+    df0 = np.array([-1.0, -1.0])
+    if case == "one_active":  # x1 + x2 <= 1.1: symmetric, 2 [p / (1 - x) + q / x] = 1.10202
+        g, dg = np.array([-0.1]), np.array([[1.0, 1.0]])
+        expected = np.full(2, _root(0.55101, 0.2502525 - 0.0002525 - 0.55101, 0.0002525))
+    elif case == "second_inactive":  # and x1 <= 0.9: its bound 0.90101 is not reached at the same point
+        g, dg = np.array([-0.1, -0.4]), np.array([[1.0, 1.0], [1.0, 0.0]])
+        expected = np.full(2, _root(0.55101, 0.2502525 - 0.0002525 - 0.55101, 0.0002525))
+    else:  # and x1 <= 0.52: x1 from p / (1 - x1) + q / x1 = 0.52101, x2 from the rest 1.10202 - 0.52101
+        g, dg = np.array([-0.1, -0.02]), np.array([[1.0, 1.0], [1.0, 0.0]])
+        expected = np.array([_root(0.52101, 0.2502525 - 0.0002525 - 0.52101, 0.0002525),
+                             _root(0.58101, 0.2502525 - 0.0002525 - 0.58101, 0.0002525)])
+    assert np.all((expected > 0.4) & (expected < 0.6))
+    xn, low, upp, lam, _ = mma.mma_step(1, x, x, x, None, None, 0.0, 1.0, 0.0, df0, g, dg, move=0.1, scale=1.0)
+    assert np.allclose(low, 0.0) and np.allclose(upp, 1.0)
+    assert np.max(np.abs(xn - expected)) < 1e-6, (xn, expected)
+    if case == "second_inactive":
+        assert lam[1] < 1e-5 and lam[0] > 0.1
+    if case == "both_active":
+        assert lam[0] > 0.1 and lam[1] > 0.1
+
+
+def _dense_cone(allowed, spacing, radius):
+    """Row-normalised cone matrix from its definition: H_ij = a_i a_j max(r - |x_i - x_j|, 0) / sum_j (...),
+    rows of non-allowed cells zero."""
+    cells = np.argwhere(np.ones(allowed.shape, bool)) * np.asarray(spacing, float)
+    a = allowed.ravel().astype(float)
+    dist = np.linalg.norm(cells[:, None, :] - cells[None, :, :], axis=2)
+    w = np.maximum(radius - dist, 0.0) * a[:, None] * a[None, :]
+    sums = w.sum(axis=1)
+    sums[sums == 0] = 1.0
+    return w / sums[:, None]
+
+
+def test_corridor_filter_is_the_blended_dense_matrix_and_its_adjoint():
+    """oracle.filters.Cone with a corridor equals (1 - b) H_r + b H_R built from the definition, and its
+    pullback is the transposed matrix on the free cells (DensityMap corridor,
+    topology_optimization.py:1125-1129, :1142, :1176)."""
+    from oracle import filters
+    rng = np.random.default_rng(7)
+    shape, spacing = (5, 4, 3), (1.0, 0.8, 1.2)
+    allowed = rng.random(shape) < 0.8
+    preserve = allowed & (rng.random(shape) < 0.1)
+    weights = np.where(rng.random(shape) < 0.5, rng.uniform(-0.3, 1.4, shape), 0.0)
+    cone = filters.Cone(allowed, spacing, 1.5, preserve, corridor=(2.6, weights))
+    b = (np.clip(weights, 0, 1) * allowed).ravel()
+    h = (1 - b)[:, None] * _dense_cone(allowed, spacing, 1.5) + b[:, None] * _dense_cone(allowed, spacing, 2.6)
+    x = rng.uniform(0, 1, allowed.size)
+    assert np.max(np.abs(cone.filtered(x) - h @ x)) < 1e-14
+    s, d = rng.normal(size=allowed.size), rng.uniform(0.5, 1.5, allowed.size)
+    free = (allowed & ~preserve).ravel()
+    assert np.max(np.abs(cone.pullback(s, d) - np.where(free, h.T @ (s * d), 0.0))) < 1e-13
+
+
+def test_volume_weights_of_soft_discs():
+    from deep_frame_amd.density import volume_weights
+    discs = {"mode": "soft", "motors_mm": [[0.0, 0.0], [10.0, 0.0]], "radius_mm": 4.0, "weight": 2.0,
+             "plane_mm": 1.0, "length_mm": 3.0}
+    points = np.array([[0.0, 0.0, 1.0],    # on a motor axis in the disc plane: 1 + weight
+                       [10.0, 3.0, 4.0],   # inside the second disc, 3 mm off the plane: 1 + 2 e^-1
+                       [5.0, 0.0, 1.0],    # between the discs (r = 5 > 4): 1
+                       [0.0, 4.0, 1.0]])   # on the disc rim (r = radius counts as inside): 1 + weight
+    assert np.allclose(volume_weights(points, discs), [3.0, 1 + 2 * np.exp(-1.0), 1.0, 3.0])
+    ramp = {**discs, "hub_radius_mm": 1.0, "radial_power": 2.0}  # ((r - 1) / 3)^2 inside the disc
+    assert np.allclose(volume_weights(points, ramp), [1.0, 1 + 2 * (2 / 3) ** 2 * np.exp(-1.0), 1.0, 3.0])
+    assert np.all(volume_weights(points, {**discs, "mode": "hard"}) == 1.0)
+    assert np.all(volume_weights(points, None) == 1.0)
