@@ -3,6 +3,7 @@
 // and the combined displacement fields U M (load cases, eigen-directions) whose element energies
 // give the gradients.  FP64 throughout; HBM-bound streams over the k columns (the Gram on the FP64 matrix
 // cores for up to 48 x 48 outputs).
+#include <mutex>
 #include <stdexcept>
 
 #include "dfmi_internal.h"
@@ -364,6 +365,32 @@ void combine_device(const double* u, long long n, int k, const double* m, int mc
   combine_mfma<double>(u, n, k, m, mcols, y, accumulate, s);
 }
 
+// ---- allocation-free Gram -------------------------------------------------------------------------------
+// The per-workgroup partial sums of dfmi_gram_static live in this file-scope device array (8 MiB: 256
+// workgroups x 64 x 64 doubles), one copy per GPU for the life of the process.  Host code never passes the
+// array's name to a kernel: its address on the current device comes from hipGetSymbolAddress, looked up once
+// per device and cached.  Every workgroup writes all of its partials before the fixed-order block sum reads
+// them, so the array needs no clearing; calls that overlap on one GPU share it and race.
+constexpr int STATIC_GRAM_BLOCKS = 256;
+constexpr int STATIC_GRAM_DEVICES = 64;
+static __device__ __align__(256) double scratch[(size_t)STATIC_GRAM_BLOCKS * GRAM_MAXK * GRAM_MAXK];
+
+static double* scratch_address() {
+  static std::mutex guard;
+  static double* addresses[STATIC_GRAM_DEVICES] = {};
+  int device = 0;
+  DFMI_HIP(hipGetDevice(&device));
+  if (device < 0 || device >= STATIC_GRAM_DEVICES) throw std::runtime_error("device index beyond the scratch table");
+  std::lock_guard<std::mutex> hold(guard);
+  if (!addresses[device]) {
+    void* address = nullptr;
+    DFMI_HIP(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)));
+    if (!address) throw std::runtime_error("hipGetSymbolAddress returned no address for the Gram scratch");
+    addresses[device] = static_cast<double*>(address);
+  }
+  return addresses[device];
+}
+
 }  // namespace dfmi
 
 using namespace dfmi;
@@ -386,6 +413,18 @@ int dfmi_gram(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, i
               double* work_dev, int32_t work_blocks, void* hip_stream) {
   DFMI_GUARD3({ gram_device(a_dev, b_dev, (long long)n, ka, kb, out_dev, work_dev, work_blocks, 1.0,
                             (hipStream_t)hip_stream); })
+}
+
+int dfmi_gram_static(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, int32_t kb, double* out_dev,
+                     void* hip_stream) {
+  DFMI_GUARD3({
+    if (!a_dev || !b_dev || !out_dev) throw std::invalid_argument("gram_static: null operand");
+    if (n < 1 || ka < 1 || kb < 1 || ka > GRAM_MAXK || kb > GRAM_MAXK)
+      throw std::invalid_argument("gram_static: need n >= 1 and 1 <= ka, kb <= 64 (the scratch holds 256 x 64 x 64 "
+                                  "partial sums)");
+    gram_device(a_dev, b_dev, (long long)n, ka, kb, out_dev, scratch_address(), STATIC_GRAM_BLOCKS, 1.0,
+                (hipStream_t)hip_stream);
+  })
 }
 
 int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev, int32_t mcols, double* y_dev,

@@ -206,6 +206,14 @@ int dfmi_kernel_check(dfmi_mg* mg, int32_t slot, double* out, int32_t* count);
 int dfmi_gram(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, int32_t kb, double* out_dev,
               double* work_dev, int32_t work_blocks, void* hip_stream);
 
+/* dfmi_gram without the work buffer, for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); no counterpart in the reference.  The
+ * partial sums live in a file-scope 8 MiB device array of the library (256 workgroups x 64 x 64 doubles, one
+ * per GPU, held until the process ends): ka, kb <= 64.  Calls that overlap on one GPU share that array and
+ * race; calls in one stream are ordered. */
+int dfmi_gram_static(const double* a_dev, const double* b_dev, int64_t n, int32_t ka, int32_t kb, double* out_dev,
+                     void* hip_stream);
+
 /* Combined displacement fields of InterfaceCovariance.measure / energy (topology_problem.py:647-652, 675-684)
  * and the block combinations of GeometricMultigrid.eigenpairs (topology_multigrid.py:476-480):
  * y_dev[m * n + i] (+ y_dev[m * n + i] if accumulate) = sum_r u_dev[r * n + i] m_dev[r * mcols + m] for k
