@@ -7,9 +7,11 @@ and the 42 unit-wrench right-hand sides already resident in HBM.
 
 The mixed solver is FP32 PCG with reliable FP64 updates around an FP32 V-cycle
 whose level-1 correction is two flexible-CG steps preconditioned by the level-1
-V-cycle (coarse_krylov 2), with Chebyshev degree 3 on every level (growth 1.0)
-and reliable updates after a 1e-4 drop of the FP32 residual (inner_tolerance);
-the FP64 solve is the reference's scheme (FP64 V-cycle, degree 3 * 2^level).
+V-cycle, whose own level-2 correction is two flexible-CG steps again (coarse_krylov 2,
+krylov_levels 2), with Chebyshev degree 3 on every level (growth 1.0) and reliable
+updates after a 1e-4 drop of the FP32 residual (inner_tolerance); the FP64 solve is
+the reference's scheme (FP64 V-cycle, degree 3 * 2^level), timed with the same
+warm-up and steps; a third leg runs FP64 with the mixed leg's algorithm.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 [--config frame|frame_small|cantilever] [--no-cpu-baseline] [--no-fp64]
@@ -72,6 +74,8 @@ def parse():
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--coarse-krylov", type=int, default=2, choices=[0, 1, 2],
                     help="mixed solver: flexible-CG steps of the level-1 solve inside the fine V-cycle (0: V-cycle)")
+    ap.add_argument("--krylov-levels", type=int, default=2,
+                    help="mixed solver: levels whose coarse-grid correction is the Krylov solve (recursive K-cycle)")
     ap.add_argument("--degree", type=int, default=3, help="mixed solver: Chebyshev degree on the fine level")
     ap.add_argument("--block", action="store_true", help="mixed solver: one block PCG recurrence over the columns")
     ap.add_argument("--growth", type=float, default=1.0,
@@ -132,7 +136,7 @@ def cpu_baseline():
     cores = max(1, min(affinity, 16))
     runs = [_cpu_run("cantilever", 1, 300, 48), _cpu_run("cantilever", cores, 300, 48),
             _cpu_run("frame:32,32,6", 1, 120, 48), _cpu_run("frame:48,48,8", 1, 180, 48),
-            _cpu_run("frame:96,96,16", 1, 90, 48)]
+            _cpu_run("frame:96,96,16", 1, 60, 48)]
     small, large = runs[2], runs[3]
     extrapolation = None
     if small.get("status") == "ok" and large.get("status") == "ok":
@@ -144,14 +148,17 @@ def cpu_baseline():
             extrapolation[label] = {"dof": dof, "seconds": float(large["seconds"] * (dof / large["dof"]) ** alpha),
                                     "peak_rss_gb": float(large["peak_rss_gb"] * (dof / large["dof"]) ** beta)}
     runs.append({"workload": "frame:256,256,48", "threads": 1, "status": "DNF (not attempted: the extrapolated time "
-                 "and memory exceed any cap of a benchmark run)"})
+                 "exceeds any cap of a benchmark run)"})
+    full = ROOT / "profiles" / "cpu_superlu_frame_96x96x16_42rhs.json"
+    recorded = json.loads(full.read_text()) if full.exists() else None
     first = runs[0]
     return {"value": first.get("seconds"), "unit": "s", "cores": 1, "kind": "port",
             "sample": "48x24x24 cantilever (91,875 DOF, 1 RHS): assembly, SuperLU factor, solve, compliance and "
                       "sensitivity on one thread (the reference's documented thread setting); the other runs are "
                       "listed in runs",
             "host": {"cpu": _cpu_model(), "nproc": os.cpu_count(), "affinity": affinity, "threads_all_cores": cores},
-            "runs": runs, "extrapolation": extrapolation}
+            "runs": runs, "extrapolation": extrapolation,
+            "recorded_uncapped": {"source": str(full.relative_to(ROOT)), "runs": recorded} if recorded else None}
 
 
 def main():
@@ -237,8 +244,8 @@ def main():
         print(json.dumps({"fp64_s": float(np.mean(t64)), "iterations": max(r[0] for r in rep64)}))
         return
     mixed = {"precision": "float32", "refine": args.scheme == "refine", "fine_smoother": args.fine_smoother,
-             "coarse_krylov": args.coarse_krylov, "growth": args.growth, "chebyshev_degree": args.degree,
-             "block_krylov": args.block}
+             "coarse_krylov": args.coarse_krylov, "krylov_levels": args.krylov_levels, "growth": args.growth,
+             "chebyshev_degree": args.degree, "block_krylov": args.block}
     if args.inner_tolerance:
         mixed["inner_tolerance"] = args.inner_tolerance
     learned = None
@@ -273,8 +280,8 @@ def main():
         t_fp64, it64 = float(np.mean(t64)), max(r[0] for r in rep64)
         # FP64 with the mixed leg's algorithm (level-1 Krylov solve, degree growth, flexible FP64 PCG): the part
         # of the speed-up that precision alone buys
-        same = {"precision": "float64", "coarse_krylov": args.coarse_krylov, "growth": args.growth,
-                "chebyshev_degree": args.degree}
+        same = {"precision": "float64", "coarse_krylov": args.coarse_krylov, "krylov_levels": args.krylov_levels,
+                "growth": args.growth, "chebyshev_degree": args.degree}
         mgs, ts, reps, _ = run(same, min(args.steps, 3), 1, "fp64-mixed-algorithm")
         mgs.close()
         t_fp64_same, it64_same = float(np.mean(ts)), max(r[0] for r in reps)
@@ -298,7 +305,7 @@ def main():
                    "precision_speedup": (t_fp64_same / t_mixed) if t_fp64_same else None,
                    "iterations": max(r[0] for r in reports),
                    "max_rel_res": max(r[1] for r in reports), "fine_smoother": args.fine_smoother,
-                   "coarse_krylov": args.coarse_krylov, "growth": args.growth,
+                   "coarse_krylov": args.coarse_krylov, "krylov_levels": args.krylov_levels, "growth": args.growth,
                    "chebyshev_degree": args.degree, "inner_tolerance": args.inner_tolerance,
                    "block_krylov": args.block, "fine_tile": list(tile[:2]) if tile else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

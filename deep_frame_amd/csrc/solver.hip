@@ -477,39 +477,39 @@ static void learned_correction(VArgs& v, const T* b) {
 template <typename T>
 static void vcycle(VArgs& v, int idx, const T* b);
 
-// Level-1 solve of the fine level's coarse-grid correction by m = coarse_krylov flexible-CG steps, each
-// preconditioned by the level-1 V-cycle (a K-cycle on the first coarse level): z0 = V(g), q = A z0,
-// alpha0 = g.z0 / z0.q; for m = 2 then r1 = g - alpha0 q, z1 = V(r1), beta = (r1.z1 - r1.z0) / g.z0 and
-// p1 = z1 + beta z0, whose product A p1 = A z1 + beta q needs one more operator apply, and
-// x = alpha0 z0 + alpha1 p1 with alpha1 = r1.z1 / p1.A p1.  g (level 1's rhs buffer) is overwritten by r1;
-// the result is left in level 1's x, as the V-cycle leaves it.
+// Solve of a coarse-grid correction on level lv >= 1 by m = coarse_krylov flexible-CG steps, each preconditioned
+// by the V-cycle from level lv (a K-cycle; the V-cycle there uses the same Krylov solve one level further down
+// while lv < krylov_levels): z0 = V(g), q = A z0, alpha0 = g.z0 / z0.q; for m = 2 then r1 = g - alpha0 q,
+// z1 = V(r1), beta = (r1.z1 - r1.z0) / g.z0 and p1 = z1 + beta z0, whose product A p1 = A z1 + beta q needs
+// one more operator apply, and x = alpha0 z0 + alpha1 p1 with alpha1 = r1.z1 / p1.A p1.  g (level lv's rhs
+// buffer) is overwritten by r1; the result is left in level lv's x, as the V-cycle leaves it.
 template <typename T>
-static void krylov_coarse(VArgs& v, T* g) {
-  Level& C = *v.H->levels[1];
+static void krylov_coarse(VArgs& v, int lv, T* g) {
+  Level& C = *v.H->levels[lv];
   hipStream_t s = v.h->stream;
   const int k = v.k, m = v.h->st.coarse_krylov;
   const long long nodes = C.g.nodes();
   double* ks = C.kscal.as<double>();
   double* part = C.kpart.as<double>();
   const int nb = level_dot_blocks(nodes);
-  vcycle<T>(v, 1, g);
+  vcycle<T>(v, lv, g);
   std::swap(C.x.p, C.kp.p), std::swap(C.x.bytes, C.kp.bytes);  // kp = z0
   const T* z0 = C.kp.as<T>();
   ApplyArgs a{};
   a.x = z0;
   a.y = C.kq.as<T>();
-  level_apply<T>(v, 1, MODE_APPLY, a);
+  level_apply<T>(v, lv, MODE_APPLY, a);
   LevelPairs<T> d1{{g, z0, nullptr, nullptr}, {z0, C.kq.as<T>(), nullptr, nullptr}};
   level_dots<T>(d1, 2, nodes, k, v.active, part, nb, s);
   reduce_partials(part, nb, 2 * k, ks, s);
   krylov_scalars(ks, k, v.active, 1, s);
   if (m >= 2) {
     level_axpy<T>(g, C.kq.as<T>(), ks + 6 * (size_t)k, nodes, k, v.active, s);  // r1
-    vcycle<T>(v, 1, g);                                                         // z1 in C.x
+    vcycle<T>(v, lv, g);                                                        // z1 in C.x
     ApplyArgs b{};
     b.x = C.x.as<T>();
     b.y = C.kaz.as<T>();
-    level_apply<T>(v, 1, MODE_APPLY, b);
+    level_apply<T>(v, lv, MODE_APPLY, b);
     const T* z1 = C.x.as<T>();
     LevelPairs<T> d2{{g, g, z1, z1}, {z1, z0, C.kaz.as<T>(), C.kq.as<T>()}};
     level_dots<T>(d2, 4, nodes, k, v.active, part, nb, s);
@@ -517,6 +517,14 @@ static void krylov_coarse(VArgs& v, T* g) {
     krylov_scalars(ks, k, v.active, 2, s);
   }
   level_combine<T>(C.x.as<T>(), z0, ks + 7 * (size_t)k, ks + 8 * (size_t)k, nodes, k, v.active, s);
+}
+
+// Whether the coarse-grid correction of level idx is the Krylov solve on level idx + 1: coarse_krylov steps
+// on the first krylov_levels levels whose coarser level is not the coarsest (slots with a free rigid kernel keep
+// the V-cycle: their coarse operators are singular).
+static bool krylov_at(const VArgs& v, int idx) {
+  return v.h->st.coarse_krylov > 0 && idx < v.h->st.krylov_levels && idx + 2 < (int)v.H->levels.size() &&
+         v.H->nk == 0;
 }
 
 // V-cycle on level idx for right-hand side b (device, T); result in levels[idx].x
@@ -552,9 +560,8 @@ static void vcycle(VArgs& v, int idx, const T* b) {
       restrict_full<T>(L.g, L.res.as<T>(), level_layout(L, idx, v.k), C.g, C.mask.as<uint8_t>(), C.rhs.as<T>(),
                        level_layout(C, idx + 1, v.k), v.k, v.active, s);
     }
-    // (slots with a free rigid kernel keep the V-cycle: their level-1 operator is singular)
-    if (idx == 0 && v.h->st.coarse_krylov > 0 && H.levels.size() > 2 && H.nk == 0)
-      krylov_coarse<T>(v, C.rhs.as<T>());
+    if (krylov_at(v, idx))
+      krylov_coarse<T>(v, idx + 1, C.rhs.as<T>());
     else
       vcycle<T>(v, idx + 1, C.rhs.as<T>());
     if (change) {
@@ -583,8 +590,8 @@ static void ensure_level_vectors(Handle* h, Hierarchy& H, int k) {
     L.xa.ensure(n);
     L.res.ensure(n);
   }
-  if (h->st.coarse_krylov > 0 && H.levels.size() > 2) {
-    Level& C = *H.levels[1];
+  for (int l = 1; h->st.coarse_krylov > 0 && l <= h->st.krylov_levels && l + 1 < (int)H.levels.size(); ++l) {
+    Level& C = *H.levels[l];
     const size_t n = (size_t)k * col(C.g) * el;
     C.kp.ensure(n);
     C.kq.ensure(n);
@@ -1690,6 +1697,7 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->damping = 1.0;
   o->cycle = 0;
   o->coarse = 0;
+  o->krylov_levels = 1;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
@@ -1709,6 +1717,7 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
     if (st.fine_smoother != 0 && st.fine_smoother != 1) throw std::invalid_argument("fine_smoother must be 0 or 1");
     if (st.retries < 0) throw std::invalid_argument("retries must be >= 0");
     if (st.coarse_krylov < 0 || st.coarse_krylov > 2) throw std::invalid_argument("coarse_krylov must be 0, 1 or 2");
+    if (st.krylov_levels < 1) throw std::invalid_argument("krylov_levels must be >= 1");
     if (st.block_krylov != 0 && st.block_krylov != 1) throw std::invalid_argument("block_krylov must be 0 or 1");
     if (st.chebyshev_degree < 1 || st.max_iterations < 1 || !(st.tolerance > 0))
       throw std::invalid_argument("invalid multigrid settings");

@@ -67,6 +67,10 @@ typedef struct dfmi_settings {
   int32_t coarse;            /* MULTIGRID["coarse"] (topology_multigrid.py:160-166): 0 = "galerkin" (default),
                                 1 = "rediscretize" (coarse moduli by 2x2x2 averaging, Ke scaled by the spacing
                                 ratio, mask injected from the finer level and cut by the dead-DOF ratio) */
+  int32_t krylov_levels;     /* coarse_krylov > 0: the coarse-grid corrections of levels 0 .. krylov_levels - 1
+                                (1, default: the fine level only) are Krylov solves on the next coarser level, so
+                                the V-cycles inside a Krylov solve use one again (a recursive K-cycle); levels
+                                whose coarser level is the coarsest keep the direct solve */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */

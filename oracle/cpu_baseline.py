@@ -12,11 +12,20 @@ memory cap.  Workloads: ``cantilever`` (48 x 24 x 24, 1 RHS), ``frame:NX,NY,NZ``
 """
 import json
 import os
-import resource
 import sys
 import time
 
 import numpy as np
+
+
+def peak_rss_gb():
+    """High-water resident set of this process (VmHWM: reset by exec, unlike ru_maxrss which a forked child
+    inherits from its parent)."""
+    with open("/proc/self/status") as status:
+        for line in status:
+            if line.startswith("VmHWM:"):
+                return int(line.split()[1]) / 2 ** 20
+    return None
 
 
 def workload(name):
@@ -61,7 +70,7 @@ def run(name, threads):
     rel = np.linalg.norm(kff @ u[free] - forces[free], axis=0) / np.linalg.norm(forces[free], axis=0)
     return {"workload": name, "shape": list(shape), "dof": ndof, "rhs": forces.shape[1], "threads": threads,
             "seconds": sum(times.values()), **times, "max_rel_res": float(rel.max()),
-            "peak_rss_gb": resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 2 ** 20}
+            "peak_rss_gb": peak_rss_gb()}
 
 
 if __name__ == "__main__":

@@ -196,13 +196,31 @@ def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0, smoother="chebyshev
     return smooth(x)
 
 
-def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0):
+def kvcycle(levels, lmax, rhs, index, steps, krylov_levels, degree=3, growth=2.0):
+    """V-cycle from level ``index`` whose coarse-grid correction on the first ``krylov_levels`` levels (counted
+    from the fine level 0) is ``steps`` flexible-CG steps on the next level (``fcg``), as long as that level is
+    not the coarsest; below, the plain V-cycle."""
+    lv = levels[index]
+    if index == len(levels) - 1 or not (index < krylov_levels and index + 2 < len(levels)):
+        return vcycle(levels, lmax, rhs, index, degree, growth)
+    count = max(1, int(round(degree * growth ** index)))
+    x = chebyshev(lv, rhs, None, count, lmax[index])
+    res = (rhs - lv["A"] @ x) * _cols(lv["mask"], rhs)
+    p = levels[index + 1]["P"]
+    corr = fcg(levels, lmax, index + 1, (p.T @ res) * _cols(levels[index + 1]["mask"], rhs), steps, degree, growth,
+               krylov_levels)
+    x = x + (p @ corr) * _cols(lv["mask"], rhs)
+    return chebyshev(lv, rhs, x, count, lmax[index])
+
+
+def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0, krylov_levels=0):
     """``steps`` iterations of flexible CG on A_index x = rhs from x = 0, each preconditioned by the V-cycle
-    from level ``index``: alpha = r.z / p.A p, beta = r_new.(z_new - z_old) / r_old.z_old."""
+    from level ``index`` (itself a K-cycle while index < krylov_levels): alpha = r.z / p.A p,
+    beta = r_new.(z_new - z_old) / r_old.z_old."""
     a = levels[index]["A"]
     x = np.zeros_like(rhs)
     r = rhs.copy()
-    z = vcycle(levels, lmax, r, index, degree, growth)
+    z = kvcycle(levels, lmax, r, index, steps, krylov_levels, degree, growth)
     p = z.copy()
     rz = r @ z
     for step in range(steps):
@@ -215,7 +233,7 @@ def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0):
         if step == steps - 1:
             break
         r = r - alpha * ap
-        znew = vcycle(levels, lmax, r, index, degree, growth)
+        znew = kvcycle(levels, lmax, r, index, steps, krylov_levels, degree, growth)
         beta = (r @ (znew - z)) / rz
         rz = r @ znew
         p = znew + beta * p
@@ -223,18 +241,13 @@ def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0):
     return x
 
 
-def kcycle(levels, lmax, rhs, steps=2, degree=3, growth=2.0):
-    """The fine level's V-cycle with its level-1 correction solved by ``fcg`` (``steps`` flexible-CG
-    steps) instead of one level-1 V-cycle; with two levels the level-1 solve is the exact coarsest one."""
+def kcycle(levels, lmax, rhs, steps=2, degree=3, growth=2.0, krylov_levels=1):
+    """The fine level's cycle with the coarse-grid corrections of the first ``krylov_levels`` levels solved by
+    ``fcg`` (``steps`` flexible-CG steps) instead of one V-cycle; with two levels the level-1 solve is the
+    exact coarsest one."""
     if len(levels) < 3:
         return vcycle(levels, lmax, rhs, 0, degree, growth)
-    lv = levels[0]
-    x = chebyshev(lv, rhs, None, degree, lmax[0])
-    res = (rhs - lv["A"] @ x) * lv["mask"]
-    p = levels[1]["P"]
-    corr = fcg(levels, lmax, 1, (p.T @ res) * levels[1]["mask"], steps, degree, growth)
-    x = x + (p @ corr) * lv["mask"]
-    return chebyshev(lv, rhs, x, degree, lmax[0])
+    return kvcycle(levels, lmax, rhs, 0, steps, krylov_levels, degree, growth)
 
 
 def pcg(levels, lmax, rhs, tol=1e-8, maxit=2000, precondition=None):

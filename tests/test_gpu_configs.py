@@ -23,8 +23,8 @@ from deep_frame_amd import element, problems
 
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
-MIXED = {"precision": "float32", "refine": True, "fine_smoother": "vector", "coarse_krylov": 2, "growth": 1.0,
-         "chebyshev_degree": 3, "inner_tolerance": 1e-4}
+MIXED = {"precision": "float32", "refine": True, "fine_smoother": "vector", "coarse_krylov": 2, "krylov_levels": 2,
+         "growth": 1.0, "chebyshev_degree": 3, "inner_tolerance": 1e-4}
 FP64 = {"precision": "float64"}
 
 

@@ -15,7 +15,8 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 # solver schemes: FP64 V-cycle, FP32 V-cycle inside FP64 PCG, FP32 PCG with reliable FP64 updates; the FP32
 # V-cycle's fine level runs on the FP32 vector units, or on the matrix cores (split bf16) for "-matrix";
 # "-krylov": the fine level's coarse-grid correction solves level 1 by two flexible-CG steps (K-cycle);
-# "refine-bench": the benchmark's mixed configuration (K-cycle, degree 3 on every level, 1e-4 inner drop);
+# "refine-bench": the benchmark's mixed configuration (K-cycle on the first two levels, degree 3 on every level,
+# 1e-4 inner drop);
 # "-block": one block PCG recurrence over all columns instead of one per column
 MODES = {"float64": {"precision": "float64"}, "float32": {"precision": "float32"},
          "refine": {"precision": "float32", "refine": True},
@@ -23,8 +24,8 @@ MODES = {"float64": {"precision": "float64"}, "float32": {"precision": "float32"
          "refine-matrix": {"precision": "float32", "refine": True, "fine_smoother": "matrix"},
          "float64-krylov": {"precision": "float64", "coarse_krylov": 2},
          "refine-krylov": {"precision": "float32", "refine": True, "coarse_krylov": 2},
-         "refine-bench": {"precision": "float32", "refine": True, "coarse_krylov": 2, "growth": 1.0,
-                          "inner_tolerance": 1e-4},
+         "refine-bench": {"precision": "float32", "refine": True, "coarse_krylov": 2, "krylov_levels": 2,
+                          "growth": 1.0, "inner_tolerance": 1e-4},
          "refine-block": {"precision": "float32", "refine": True, "block_krylov": True},
          "refine-block-krylov": {"precision": "float32", "refine": True, "block_krylov": True, "coarse_krylov": 2,
                                  "growth": 1.0, "inner_tolerance": 1e-4}}

@@ -137,3 +137,33 @@ def test_reference_cycle_settings_solve_with_history(settings, torch_gpu):
     assert len(report["history"][0]) == max(report["iterations"])
     assert report["history"][0][-1] < 1e-8 and report["kernel_check"] == []
     mg.close()
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+@pytest.mark.parametrize("krylov_levels", [2, 3])
+def test_recursive_kcycle_matches_oracle(precision, krylov_levels, torch_gpu):
+    """The Krylov coarse solve applied recursively (krylov_levels): the V-cycle inside each flexible-CG step
+    of level l solves its own coarse correction by flexible CG on level l + 1, on a 5-level hierarchy, against
+    oracle.gmg.kcycle (FP64 to 1e-9, FP32 to 1e-4) for 1, 2 and 7 columns."""
+    from deep_frame_amd.multigrid import GeometricMultigrid
+    print(f"recursive kcycle levels={krylov_levels} precision={precision}", flush=True)
+    shape, spacing = (32, 16, 16), (1.0, 1.0, 1.0)
+    ke = element.stiffness(spacing, element.isotropic(0.3))
+    rho = np.clip(np.random.default_rng(5).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
+    mod = 1e-6 + (1 - 1e-6) * rho ** 3
+    pts = element.node_points(shape, spacing)
+    fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
+    mg = GeometricMultigrid(shape, spacing, ke, {"levels": 5, "precision": precision, "coarse_krylov": 2,
+                                                 "krylov_levels": krylov_levels, "growth": 1.0})
+    mg.update(mod)
+    entry = mg.hierarchy("k", fixed)
+    levels = gmg.hierarchy(shape, ke, mod, fixed, (5, shape))
+    for cols in (1, 2, 7):
+        r = np.random.default_rng(6).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
+        ref = np.column_stack([gmg.kcycle(levels, entry["lambda_max"], r[:, c], 2, 3, 1.0, krylov_levels)
+                               for c in range(cols)])
+        z = mg.precondition_device("k", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
+        tol = 1e-9 if precision == "float64" else 1e-4
+        for c in range(cols):
+            assert np.linalg.norm(z[:, c] - ref[:, c]) < tol * np.linalg.norm(ref[:, c]), (cols, c)
+    mg.close()

@@ -367,6 +367,16 @@ def test_gmg_many_fcg_steps_solve_the_level_exactly_and_kcycle_pcg_converges():
     ref, _ = system.solve(dofs, conn, ke, mod, prob["fixed"], f, len(pts))
     assert rel < 1e-8 and it_k < it_v
     assert np.linalg.norm(x - ref[:, 0]) < 1e-7 * np.linalg.norm(ref)
+    # recursive K-cycle: the level-1 V-cycles solve their own level-2 correction by FCG; with many steps the
+    # level-1 solve is again exact, and PCG with two steps per level needs no more iterations
+    g1 = np.random.default_rng(7).normal(size=levels[1]["A"].shape[0]) * levels[1]["mask"]
+    live = np.flatnonzero(levels[1]["mask"] > 0)
+    exact = np.zeros_like(g1)
+    exact[live] = splu(levels[1]["A"][live][:, live].tocsc()).solve(g1[live])
+    deep = gmg.fcg(levels, lam, 1, g1, 40, krylov_levels=2)
+    assert np.linalg.norm(deep - exact) < 1e-8 * np.linalg.norm(exact)
+    x2, it_k2, rel2 = gmg.pcg(levels, lam, f, precondition=lambda r: gmg.kcycle(levels, lam, r, 2, krylov_levels=2))
+    assert rel2 < 1e-8 and it_k2 <= it_k and np.linalg.norm(x2 - ref[:, 0]) < 1e-7 * np.linalg.norm(ref)
 
 
 def test_sensitivity_matches_central_difference():
