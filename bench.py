@@ -149,16 +149,13 @@ def cpu_baseline():
                                     "peak_rss_gb": float(large["peak_rss_gb"] * (dof / large["dof"]) ** beta)}
     runs.append({"workload": "frame:256,256,48", "threads": 1, "status": "DNF (not attempted: the extrapolated time "
                  "exceeds any cap of a benchmark run)"})
-    full = ROOT / "profiles" / "cpu_superlu_frame_96x96x16_42rhs.json"
-    recorded = json.loads(full.read_text()) if full.exists() else None
     first = runs[0]
     return {"value": first.get("seconds"), "unit": "s", "cores": 1, "kind": "port",
             "sample": "48x24x24 cantilever (91,875 DOF, 1 RHS): assembly, SuperLU factor, solve, compliance and "
                       "sensitivity on one thread (the reference's documented thread setting); the other runs are "
                       "listed in runs",
             "host": {"cpu": _cpu_model(), "nproc": os.cpu_count(), "affinity": affinity, "threads_all_cores": cores},
-            "runs": runs, "extrapolation": extrapolation,
-            "recorded_uncapped": {"source": str(full.relative_to(ROOT)), "runs": recorded} if recorded else None}
+            "runs": runs, "extrapolation": extrapolation}
 
 
 def main():
@@ -448,20 +445,33 @@ def run_rows(args):
     mod = 1.0 * (1e-6 + (1 - 1e-6) * density.ravel() ** 3)
     points = element.node_points(shape)
     pins = problems.relief_support(points, np.flatnonzero(points[:, 2] == 0))
+    bench_mixed = {"precision": "float32", "refine": True, "coarse_krylov": 2, "krylov_levels": 2, "growth": 1.0,
+                   "inner_tolerance": 1e-4}
+    schemes = (("clamped", prob["fixed"], None, {"precision": "float32", "refine": True}),
+               ("clamped_fp64", prob["fixed"], None, {"precision": "float64"}),
+               ("clamped_flexible", prob["fixed"], None, {"precision": "float32"}),
+               ("clamped_bench_mixed", prob["fixed"], None, bench_mixed),
+               ("clamped_bench_mixed_block", prob["fixed"], None, {**bench_mixed, "block_krylov": True}),
+               ("floating", pins, pins, {"precision": "float32", "refine": True}))
     result = {}
-    for name, fixed, sup in (("clamped", prob["fixed"], None), ("floating", pins, pins)):
-        mg = GeometricMultigrid(shape, (1, 1, 1), ke, {"precision": "float32", "refine": True})
+    for name, fixed, sup, settings in schemes:
+        mg = GeometricMultigrid(shape, (1, 1, 1), ke, settings)
         mg.update(mod)
         mg.solve(name, fixed, prob["forces"], sup)
         t0 = time.perf_counter()
         _, rep = mg.solve(name, fixed, prob["forces"], sup)
         result[name] = {"s": time.perf_counter() - t0, "iterations": max(rep["iterations"]),
                         "max_rel_res": max(rep["relative_residual"]), "converged": rep["converged"],
-                        "retried_columns": rep["retried_columns"]}
+                        "retried_columns": rep["retried_columns"], "settings": settings}
         mg.close()
+        print(f"# exported_075mm {name}: {result[name]['s']:.3f} s, {result[name]['iterations']} iterations",
+              file=sys.stderr, flush=True)
     rows["exported_075mm"] = {"workload": "42 unit-wrench solves on the exported optimised 0.75 mm half-frame density "
-                                          "(91x170x48, 2,312,604 DOF, stiffness contrast 1e6), clamped pads and a "
-                                          "floating 3-2-1 support (host arrays in and out)", **result}
+                                          "(91x170x48, 2,312,604 DOF, stiffness contrast 1e6): clamped pads with the "
+                                          "mixed V-cycle, the reference's FP64 scheme, its default scheme (FP64 PCG, "
+                                          "FP32 V-cycle), the benchmark's mixed "
+                                          "configuration and the same as one block PCG; a floating 3-2-1 support "
+                                          "(host arrays in and out)", **result}
     print(f"# exported_075mm {rows['exported_075mm']}", file=sys.stderr, flush=True)
     print(json.dumps({"metric": "callers around the solve", "n_gpus": 1, "data": "synthetic", "rows": rows}))
 

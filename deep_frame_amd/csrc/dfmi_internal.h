@@ -94,6 +94,43 @@ enum ApplyMode : int {
                     // the moduli, Me from a.mdiag (Hadamard-diagonal form) and lumped in a.dinv (may be null)
 };
 
+// Three consecutive entries of a vector through a raw buffer load: byte offsets at or beyond the buffer's
+// size read zeros (no clamp or select needed for neighbours or slots outside the array), and the address is
+// one 32-bit offset from a scalar base.  Buffers must stay below 2 GiB.
+__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, unsigned byte, float& a, float& b, float& c) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, byte, 0, 0);
+  a = __uint_as_float(v[0]);
+  b = __uint_as_float(v[1]);
+  c = __uint_as_float(v[2]);
+}
+__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, unsigned byte, double& a, double& b, double& c) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte, 0, 0);
+  const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, byte + 16, 0, 0);
+  a = __hiloint2double((int)v[1], (int)v[0]);
+  b = __hiloint2double((int)v[3], (int)v[2]);
+  c = __hiloint2double((int)w[1], (int)w[0]);
+}
+// buffer descriptor word 3 for raw 32-bit loads on gfx9 (data format 32)
+constexpr int kBufferWord3 = 0x00020000;
+// a byte offset that lies beyond every buffer built by buffer() (sizes below 2 GiB)
+constexpr unsigned kOutside = 0x80000000u;
+
+// Three consecutive entries written through one (FP32) or two (FP64) buffer stores.
+__device__ __forceinline__ void store3(__amdgpu_buffer_rsrc_t r, unsigned byte, float a, float b, float c) {
+  __attribute__((ext_vector_type(3))) unsigned v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c)};
+  __builtin_amdgcn_raw_buffer_store_b96(v, r, byte, 0, 0);
+}
+__device__ __forceinline__ void store3(__amdgpu_buffer_rsrc_t r, unsigned byte, double a, double b, double c) {
+  __attribute__((ext_vector_type(4))) unsigned v = {(unsigned)__double2loint(a), (unsigned)__double2hiint(a),
+                                                    (unsigned)__double2loint(b), (unsigned)__double2hiint(b)};
+  __attribute__((ext_vector_type(2))) unsigned w = {(unsigned)__double2loint(c), (unsigned)__double2hiint(c)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, byte, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, byte + 16, 0, 0);
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer(const void* p, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kBufferWord3);
+}
+
 struct ApplyArgs {
   const void* x;      // operand, [k][nodes][3]
   void* y;            // output
@@ -108,6 +145,8 @@ struct ApplyArgs {
   const int* active;  // per-column flag (device), nullptr = all active
   int k;
   double mdiag[8];    // MASS only: unit-density element mass in the corner-Hadamard basis, divided by 64
+  int premasked;      // fine vector kernel: the operand (and for CHEB0 / DOT-with-pout the vectors the operand is
+                      // formed from) is already zero on every DOF the mask cuts, so staging skips the mask
 };
 
 // ---- fine level (matrix-free hex8) -------------------------------------------------

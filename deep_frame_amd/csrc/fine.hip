@@ -147,21 +147,32 @@ struct FineCfg {
 };
 
 // One workgroup owns a (TY-1) x (TZ-1) tile of node columns (TY * TZ <= NT element columns including
-// a one-element halo at the low side) and marches along x.  LDS is structure-of-arrays (component
-// planes), so consecutive lanes touch consecutive 4- or 8-byte words: no bank conflicts.  The FP32 DOT
+// a one-element halo at the low side) and marches along x.  The FP32 DOT
 // mode, which forms the fused PCG direction, is held to 80 VGPRs (six waves per SIMD, three workgroups per
-// CU).
-template <typename T, int MODE>
+// CU).  PRE: the operand is known to be zero on the DOFs the mask cuts (every vector the solver itself passes),
+// so the staging loads skip the mask and read through 32-bit buffer offsets; slots outside the grid read zeros.
+// KMODE: an ApplyMode, or kChebPartial for the FP32 Chebyshev step that also sums b.y per workgroup (a separate
+// instantiation, so that the other Chebyshev steps carry no dot-product work).
+constexpr int kChebPartial = 100;
+template <typename T, int KMODE, bool PRE>
 __global__ __launch_bounds__(FineCfg<T>::NT)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1))) void k_fine_apply(
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE == MODE_DOT ? 6 : 1))) void k_fine_apply(
     Geom g, const T* __restrict__ moduli, const uint8_t* __restrict__ mask, KeSym<T> ks, ApplyArgs a, int TY, int TZ,
     int xchunk, int tiles_y, int tiles_z) {
+  constexpr int MODE = KMODE == kChebPartial ? MODE_CHEB : KMODE;
+  constexpr bool kChebDot = KMODE == kChebPartial;
   constexpr int NT = FineCfg<T>::NT;
   constexpr int MAXNP = FineCfg<T>::MAXNP;
   // the FP32 PCG's fusions (direction formed inside DOT, r.z partials from the last CHEB step)
   constexpr bool kFusedPcg = sizeof(T) == sizeof(float);
-  __shared__ T U[2][3][MAXNP];  // node planes i and i+1 of the operand (masked)
-  __shared__ T OUT[3][3][NT];   // face corner results m = 1..3 (dy, dz, dy dz) at plane i
+  __shared__ T U[2][3 * MAXNP];  // node planes i and i+1 of the operand (masked)
+  __shared__ T OUT[9 * NT];      // face corner results m = 1..3 (dy, dz, dy dz) at plane i
+  // LDS layouts: FP32 node-major (a node's three components adjacent: one address per node and fewer LDS
+  // instructions; a 12-byte lane stride is conflict-free), FP64 component planes (8-byte words at a 24-byte
+  // lane stride would conflict)
+  constexpr bool kNodeMajor = sizeof(T) == sizeof(float);
+  auto u_at = [&](int buf, int c, int v) -> T& { return kNodeMajor ? U[buf][3 * v + c] : U[buf][c * MAXNP + v]; };
+  auto out_at = [&](int m, int c, int tt) -> T& { return kNodeMajor ? OUT[9 * tt + 3 * m + c] : OUT[(3 * m + c) * NT + tt]; };
   __shared__ double red[NT / 64];
 
   const int r = blockIdx.y;
@@ -200,8 +211,53 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1)))
     }
   }
   T nv[2][3];
+  // PRE staging: byte offset of each slot within a column at plane 0 (kOutside: no node), buffers of one column
+  unsigned sb[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) sb[k] = so[k] >= 0 ? (unsigned)so[k] * 3u * (unsigned)sizeof(T) : kOutside;
+  const unsigned plane_b = (unsigned)plane_nodes * 3u * (unsigned)sizeof(T);
+  const long long col_b = col * (long long)sizeof(T);
+  const __amdgpu_buffer_rsrc_t xr = buffer(X, col_b);
+  const __amdgpu_buffer_rsrc_t br = buffer(a.b ? reinterpret_cast<const T*>(a.b) + (long long)r * col : X, col_b);
+  const __amdgpu_buffer_rsrc_t dr = buffer(a.dinv ? a.dinv : (const void*)X, col_b);
+  const __amdgpu_buffer_rsrc_t xpr = buffer(a.xp ? reinterpret_cast<const T*>(a.xp) + (long long)r * col : X, col_b);
+  const __amdgpu_buffer_rsrc_t yr = buffer(Y, col_b);
+  const __amdgpu_buffer_rsrc_t pr = buffer(a.pout ? reinterpret_cast<const T*>(a.pout) + (long long)r * col : X, col_b);
+  const __amdgpu_buffer_rsrc_t mr = buffer(mask, g.nodes());
+  auto fetch_pre = [&](int ip) {
+    const bool in = ip >= 0 && ip < g.nx;  // uniform
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const unsigned off = in ? sb[k] + (unsigned)ip * plane_b : kOutside;
+      T v0, v1, v2;
+      if (MODE == MODE_CHEB0) {  // x1 = c0 Dinv b
+        T b0, b1, b2, d0, d1, d2;
+        load3(br, off, b0, b1, b2);
+        load3(dr, off, d0, d1, d2);
+        const T c0 = (T)a.c0;
+        v0 = c0 * d0 * b0;
+        v1 = c0 * d1 * b1;
+        v2 = c0 * d2 * b2;
+      } else if (MODE == MODE_DOT && kFusedPcg && a.pout) {  // fused PCG direction: p = z + beta p_old
+        load3(br, off, v0, v1, v2);
+        if (a.beta) {
+          const T bt = (T)a.beta[r];
+          T p0, p1, p2;
+          load3(xr, off, p0, p1, p2);
+          v0 += bt * p0;
+          v1 += bt * p1;
+          v2 += bt * p2;
+        }
+      } else {
+        load3(xr, off, v0, v1, v2);
+      }
+      nv[k][0] = v0;
+      nv[k][1] = v1;
+      nv[k][2] = v2;
+    }
+  };
   // branch-free staging: slots outside the grid read node 0 and are zeroed by their mask
-  auto fetch = [&](int ip) {
+  auto fetch_masked = [&](int ip) {
     const bool in = ip >= 0 && ip < g.nx;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -242,14 +298,20 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1)))
       }
     }
   };
+  auto fetch = [&](int ip) {
+    if constexpr (PRE)
+      fetch_pre(ip);
+    else
+      fetch_masked(ip);
+  };
   auto stage = [&](int buf) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       if (so[k] != -2) {
         const int v = t + k * NT;
-        U[buf][0][v] = nv[k][0];
-        U[buf][1][v] = nv[k][1];
-        U[buf][2][v] = nv[k][2];
+        u_at(buf, 0, v) = nv[k][0];
+        u_at(buf, 1, v) = nv[k][1];
+        u_at(buf, 2, v) = nv[k][2];
       }
     }
   };
@@ -279,11 +341,22 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1)))
     const int nn = i * plane_nodes + ej * g.nz + el;
     const bool out = own && i >= i0;
     const int nnc = out ? nn : 0;  // loads below are unconditional (node 0 when this thread has no output)
-    const uint8_t mkr = mask[nnc];
-    const uint8_t mk = out ? mkr : 0;
-    // epilogue operands of node (i, ej, el), requested early
+    uint8_t mk;
+    if constexpr (PRE) {
+      mk = (uint8_t)__builtin_amdgcn_raw_buffer_load_b8(mr, out ? (unsigned)nn : kOutside, 0, 0);
+    } else {
+      const uint8_t mkr = mask[nnc];
+      mk = out ? mkr : 0;
+    }
+    // epilogue operands of node (i, ej, el), requested early (PRE: 32-bit buffer offsets, zeros without a node)
     T bv[3] = {T(0), T(0), T(0)}, xpv[3] = {T(0), T(0), T(0)}, dv[3] = {T(0), T(0), T(0)};
-    {
+    const unsigned eoff = out ? (unsigned)nn * 3u * (unsigned)sizeof(T) : kOutside;
+    if constexpr (PRE) {
+      if (MODE == MODE_RESID || MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1)
+        load3(br, eoff, bv[0], bv[1], bv[2]);
+      if (MODE == MODE_CHEB && a.xp) load3(xpr, eoff, xpv[0], xpv[1], xpv[2]);
+      if (MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) load3(dr, eoff, dv[0], dv[1], dv[2]);
+    } else {
       if (MODE == MODE_RESID || MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
         const T* bp = reinterpret_cast<const T*>(a.b) + (long long)r * col + 3LL * nnc;
         bv[0] = bp[0];
@@ -306,18 +379,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1)))
     __syncthreads();
     const int b0 = i & 1, b1 = (i + 1) & 1;
     T xs[3];
-    xs[0] = U[b0][0][self];
-    xs[1] = U[b0][1][self];
-    xs[2] = U[b0][2][self];
+    xs[0] = u_at(b0, 0, self);
+    xs[1] = u_at(b0, 1, self);
+    xs[2] = u_at(b0, 2, self);
     // modes with epilogue operands in registers recompute the plane-i face instead of carrying it
     constexpr bool carry_face = !(MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1);
     if (!carry_face || i == i0 - 1) {  // face transform of plane i
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int v = self + (m & 1) * (TZ + 1) + (m >> 1);
-        fc[m][0] = U[b0][0][v];
-        fc[m][1] = U[b0][1][v];
-        fc[m][2] = U[b0][2][v];
+        fc[m][0] = u_at(b0, 0, v);
+        fc[m][1] = u_at(b0, 1, v);
+        fc[m][2] = u_at(b0, 2, v);
       }
       face_butterfly(fc);
     }
@@ -327,9 +400,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1)))
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int v = self + (m & 1) * (TZ + 1) + (m >> 1);
-        fn[m][0] = U[b1][0][v];
-        fn[m][1] = U[b1][1][v];
-        fn[m][2] = U[b1][2][v];
+        fn[m][0] = u_at(b1, 0, v);
+        fn[m][1] = u_at(b1, 1, v);
+        fn[m][2] = u_at(b1, 2, v);
       }
       face_butterfly(fn);
       T y[8][3];  // mode q = x + 2 m
@@ -372,38 +445,49 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1)))
     }
 #pragma unroll
     for (int m = 1; m < 4; ++m) {
-      OUT[m - 1][0][t] = g[m][0];
-      OUT[m - 1][1][t] = g[m][1];
-      OUT[m - 1][2][t] = g[m][2];
+      out_at(m - 1, 0, t) = g[m][0];
+      out_at(m - 1, 1, t) = g[m][1];
+      out_at(m - 1, 2, t) = g[m][2];
     }
     __syncthreads();
     if (own) {
       const int tb = t - TZ, tc = t - 1, td = t - TZ - 1;  // elements (ty-1,tz), (ty,tz-1), (ty-1,tz-1)
       T s0[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s0[c] = g[0][c] + OUT[0][c][tb] + OUT[1][c][tc] + OUT[2][c][td];
-      if (out) {
+      for (int c = 0; c < 3; ++c) s0[c] = g[0][c] + out_at(0, c, tb) + out_at(1, c, tc) + out_at(2, c, td);
+      if (PRE || out) {  // PRE: a thread without an output node stores to kOutside, which is dropped
+        T yv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const bool on = bit_live(mk, c);
-          if (MODE == MODE_APPLY) {
-            Y[3LL * nn + c] = on ? s0[c] : T(0);
-          } else if (MODE == MODE_DOT) {
-            const T yv = on ? s0[c] : T(0);
-            Y[3LL * nn + c] = yv;
-            if (kFusedPcg && a.pout) reinterpret_cast<T*>(a.pout)[(long long)r * col + 3LL * nn + c] = xs[c];
-            dot += (double)xs[c] * (double)yv;
+          if (MODE == MODE_APPLY || MODE == MODE_DOT) {
+            yv[c] = on ? s0[c] : T(0);
           } else if (MODE == MODE_RESID) {
-            Y[3LL * nn + c] = on ? bv[c] - s0[c] : T(0);
+            yv[c] = on ? bv[c] - s0[c] : T(0);
           } else if (MODE == MODE_CHEB || MODE == MODE_CHEB0 || MODE == MODE_CHEB1) {
             const T c1 = (T)a.c1, c2 = (T)a.c2;
             const T prev = MODE == MODE_CHEB1 ? (T)a.c0 * dv[c] * bv[c] : xpv[c];
-            const T yv = on ? xs[c] + c1 * (xs[c] - prev) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
-            Y[3LL * nn + c] = yv;
-            if (MODE == MODE_CHEB && kFusedPcg && a.partial) dot += (double)bv[c] * (double)yv;
+            yv[c] = on ? xs[c] + c1 * (xs[c] - prev) + c2 * dv[c] * (bv[c] - s0[c]) : T(0);
           } else if (MODE == MODE_MASS) {
-            const T lump = a.dinv ? reinterpret_cast<const T*>(a.dinv)[3LL * nn + c] : T(0);
-            Y[3LL * nn + c] = on ? s0[c] + lump * xs[c] : T(0);
+            const T lump = a.dinv ? reinterpret_cast<const T*>(a.dinv)[3LL * nnc + c] : T(0);
+            yv[c] = on ? s0[c] + lump * xs[c] : T(0);
+          }
+        }
+        // partial sums: x.y (DOT), b.y (FP32 CHEB with partials); yv is zero without an output node
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (MODE == MODE_DOT) dot += (double)xs[c] * (double)yv[c];
+          if (kChebDot) dot += (double)bv[c] * (double)yv[c];
+        }
+        if constexpr (PRE) {
+          store3(yr, eoff, yv[0], yv[1], yv[2]);
+          if (MODE == MODE_DOT && kFusedPcg && a.pout) store3(pr, eoff, xs[0], xs[1], xs[2]);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            Y[3LL * nn + c] = yv[c];
+            if (MODE == MODE_DOT && kFusedPcg && a.pout)
+              reinterpret_cast<T*>(a.pout)[(long long)r * col + 3LL * nn + c] = xs[c];
           }
         }
       }
@@ -411,7 +495,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MODE == MODE_DOT ? 6 : 1)))
     e_cur = e_next;
     (void)mk_next;
   }
-  if (MODE == MODE_DOT || (MODE == MODE_CHEB && kFusedPcg && a.partial)) {  // partials of x.y (DOT), b.y (CHEB)
+  if (MODE == MODE_DOT || kChebDot) {  // partials of x.y (DOT), b.y (CHEB)
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((t & 63) == 0) red[t >> 6] = dot;
     __syncthreads();
@@ -505,13 +589,25 @@ void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlo
   const KeSym<T> ks = sym_blocks<T>(kbd);
   dim3 grid(tl.tiles_y * tl.tiles_z * tl.chunks, a.k);
   if (nblocks_out) *nblocks_out = grid.x;
-#define DFMI_LAUNCH(M)                                                                                      \
-  hipLaunchKernelGGL((k_fine_apply<T, M>), grid, dim3(NT), 0, s, g, moduli, mask, ks, a, tl.TY, tl.TZ, \
-                     tl.xchunk, tl.tiles_y, tl.tiles_z)
+  // the premasked kernel addresses a column through 32-bit byte offsets
+  const bool pre = a.premasked && g.nodes() * 3 * (long long)sizeof(T) < (1LL << 31);
+#define DFMI_LAUNCH(M)                                                                                         \
+  if (pre)                                                                                                     \
+    hipLaunchKernelGGL((k_fine_apply<T, M, true>), grid, dim3(NT), 0, s, g, moduli, mask, ks, a, tl.TY, tl.TZ, \
+                       tl.xchunk, tl.tiles_y, tl.tiles_z);                                                     \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_fine_apply<T, M, false>), grid, dim3(NT), 0, s, g, moduli, mask, ks, a, tl.TY,       \
+                       tl.TZ, tl.xchunk, tl.tiles_y, tl.tiles_z)
   switch (mode) {
     case MODE_APPLY: DFMI_LAUNCH(MODE_APPLY); break;
     case MODE_RESID: DFMI_LAUNCH(MODE_RESID); break;
-    case MODE_CHEB: DFMI_LAUNCH(MODE_CHEB); break;
+    case MODE_CHEB:
+      if (sizeof(T) == sizeof(float) && a.partial) {
+        DFMI_LAUNCH(kChebPartial);
+      } else {
+        DFMI_LAUNCH(MODE_CHEB);
+      }
+      break;
     case MODE_CHEB0: DFMI_LAUNCH(MODE_CHEB0); break;
     case MODE_CHEB1: DFMI_LAUNCH(MODE_CHEB1); break;
     case MODE_DOT: DFMI_LAUNCH(MODE_DOT); break;

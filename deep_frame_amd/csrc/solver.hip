@@ -264,8 +264,10 @@ static void level_apply(VArgs& v, int idx, int mode, ApplyArgs& a, int* nblocks 
   if (idx == 0 && sizeof(T) == sizeof(float) && use_mfma(v.h, L.g, mode))
     fine_apply_mfma(L.g, v.h->m32.as<float>(), L.mask.as<uint8_t>(), v.h->atab.as<unsigned>(), mode, a,
                     fine_tile(v.h, v.k), v.h->stream);
-  else if (idx == 0)
+  else if (idx == 0) {
+    a.premasked = 1;  // V-cycle iterates are masked outputs of the operator kernels and the prolongation
     fine_apply<T>(L.g, level_moduli<T>(v.h), L.mask.as<uint8_t>(), v.h->kb, mode, a, v.h->stream, nblocks);
+  }
   else
     stencil_apply<T>(L.g, level_stencil<T>(L), L.mask.as<uint8_t>(), mode, a, interleaved(v.k), v.h->stream, nullptr);
 }
@@ -1045,6 +1047,7 @@ static void fine_fp64(Handle* h, Hierarchy& H, int mode, const double* x, double
   a.k = k;
   a.active = active;
   a.partial = partial;
+  a.premasked = 1;  // right-hand sides, start vectors and directions are masked before they reach the operator
   fine_apply<double>(h->pad, h->m64.as<double>(), H.levels[0]->mask.as<uint8_t>(), h->kb, mode, a, h->stream,
                      nblocks);
 }
@@ -1302,6 +1305,7 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
     a.k = k;
     a.active = act;
     a.partial = part;
+    a.premasked = 1;  // z (V-cycle output) and the previous direction are masked
     fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_DOT, a, s, &nbf);
     std::swap(p32, pn32);
     first = false;
@@ -2157,6 +2161,7 @@ int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision,
     a.x = xb.p;
     a.y = yb.p;
     a.k = k;
+    a.premasked = 1;  // the solver's form of the fine kernel (the zero operand is masked)
     auto launch = [&]() {
       if (level > 0 && precision == 0)
         stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, interleaved(k), h.stream,

@@ -22,40 +22,6 @@ __device__ inline double weight1(int t) { return t == 0 ? 1.0 : ((t == 1 || t ==
 constexpr int SNT = 512;
 constexpr int SP = 244;  // padded LDS row of 243 stencil entries
 
-// Three consecutive entries of a level vector through a raw buffer load: byte offsets outside the
-// buffer (a neighbour beyond either end of the array; its stencil entries are zero) read zeros, so the
-// gather needs no clamp, and the address is one 32-bit offset from a scalar base.
-__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, unsigned byte, float& a, float& b, float& c) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, byte, 0, 0);
-  a = __uint_as_float(v[0]);
-  b = __uint_as_float(v[1]);
-  c = __uint_as_float(v[2]);
-}
-__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, unsigned byte, double& a, double& b, double& c) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, byte, 0, 0);
-  const auto w = __builtin_amdgcn_raw_buffer_load_b64(r, byte + 16, 0, 0);
-  a = __hiloint2double((int)v[1], (int)v[0]);
-  b = __hiloint2double((int)v[3], (int)v[2]);
-  c = __hiloint2double((int)w[1], (int)w[0]);
-}
-// buffer descriptor word 3 for raw 32-bit loads on gfx9 (data format 32)
-constexpr int kBufferWord3 = 0x00020000;
-
-// Three consecutive entries written through one (FP32) or two (FP64) buffer stores.
-__device__ __forceinline__ void store3(__amdgpu_buffer_rsrc_t r, unsigned byte, float a, float b, float c) {
-  __attribute__((ext_vector_type(3))) unsigned v = {__float_as_uint(a), __float_as_uint(b), __float_as_uint(c)};
-  __builtin_amdgcn_raw_buffer_store_b96(v, r, byte, 0, 0);
-}
-__device__ __forceinline__ void store3(__amdgpu_buffer_rsrc_t r, unsigned byte, double a, double b, double c) {
-  __attribute__((ext_vector_type(4))) unsigned v = {(unsigned)__double2loint(a), (unsigned)__double2hiint(a),
-                                                    (unsigned)__double2loint(b), (unsigned)__double2hiint(b)};
-  __attribute__((ext_vector_type(2))) unsigned w = {(unsigned)__double2loint(c), (unsigned)__double2hiint(c)};
-  __builtin_amdgcn_raw_buffer_store_b128(v, r, byte, 0, 0);
-  __builtin_amdgcn_raw_buffer_store_b64(w, r, byte + 16, 0, 0);
-}
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer(const void* p, long long bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kBufferWord3);
-}
 
 template <typename T, int RP, int MODE>
 __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restrict__ st,
