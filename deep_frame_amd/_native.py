@@ -60,6 +60,12 @@ SIGNATURES = {
     "dfmi_history": ([c_void_p, POINTER(c_double), c_int32, POINTER(c_int32)], c_int32),
     "dfmi_kernel_check": ([c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)], c_int32),
     "dfmi_gram": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_void_p], c_int32),
+    "dfmi_block_work_doubles": ([c_int64, c_int32, c_int32, POINTER(c_int64)], c_int32),
+    "dfmi_block_gram": ([c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
+                         c_void_p], c_int32),
+    "dfmi_block_combine": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int64, c_void_p], c_int32),
+    "dfmi_block_update": ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
+                           c_void_p], c_int32),
     "dfmi_gram_static": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p], c_int32),
     "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_project": ([c_void_p, c_void_p, c_int64, c_double, POINTER(c_double), c_int32, c_int32, c_void_p, c_void_p,

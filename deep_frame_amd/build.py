@@ -45,7 +45,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
         objs = list(pool.map(compile_one, sources))
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-Wl,--no-undefined", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr[-6000:]}")

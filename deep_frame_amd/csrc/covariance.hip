@@ -312,25 +312,6 @@ static void combine_mfma(const T* u, long long n, int k, const double* m, int mc
   }
 }
 
-// FP32 operands, FP64 products and sums (the block Krylov solve): out = A^T B (ka, kb <= 48) and
-// Y (+)= U M (k <= 64)
-void gram32_device(const float* a, const float* b, long long n, int ka, int kb, double* out, double* work,
-                   int work_blocks, hipStream_t s) {
-  if (n < 1 || ka < 1 || kb < 1 || ka > 48 || kb > 48 || work_blocks < 1)
-    throw std::invalid_argument("gram32: need n >= 1 and 1 <= ka, kb <= 48");
-  const int blocks = (int)std::min<long long>((n + GRAM_CH - 1) / GRAM_CH, work_blocks);
-  gram_mfma<float>(a, b, n, ka, kb, work, blocks, s);
-  DFMI_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_blocks, dim3(ka * kb), dim3(256), 0, s, work, blocks, ka * kb, 1.0, out);
-  DFMI_HIP(hipGetLastError());
-}
-
-void combine32_device(const float* u, long long n, int k, const double* m, int mcols, float* y, int accumulate,
-                      hipStream_t s) {
-  if (n < 1 || k < 1 || k > GRAM_MAXK || mcols < 1) throw std::invalid_argument("combine32: need n >= 1, 1 <= k <= 64");
-  combine_mfma<float>(u, n, k, m, mcols, y, accumulate, s);
-}
-
 void gram_device(const double* a, const double* b, long long n, int ka, int kb, double* out, double* work,
                  int work_blocks, double scale, hipStream_t s) {
   if (n < 1 || ka < 1 || kb < 1 || ka > GRAM_MAXK || kb > GRAM_MAXK)

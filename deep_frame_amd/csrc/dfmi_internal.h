@@ -272,11 +272,18 @@ void from_work(const T* w, const double* scale, double* z, long long n, int k, c
 // in `work`, work_blocks * ka * kb doubles, summed in a fixed order)
 void gram_device(const double* a, const double* b, long long n, int ka, int kb, double* out, double* work,
                  int work_blocks, double scale, hipStream_t s);
+// passes of the block PCG recurrence over FP32 column-major multi-vectors (block.hip); coefficient matrices
+// of block_combine / block_update are FP32 with row stride 48
+int block_gram_blocks(long long n);
+int block_combine_blocks(long long n);
+void block_gram(const float* a, int ka, const float* b1, int kb1, const float* b2, int kb2, long long n, double* out,
+                double* work, hipStream_t s);
+void block_combine(const float* u, const float* x, float* y, const float* m, int kin, int kout, long long n,
+                   hipStream_t s);
+void block_update(const float* v, float* d, const float* w, float* r, const float* m, int k, long long n, double* rr,
+                  double* work, hipStream_t s);
+void block_small_product(const double* S, const double* H, int k, double scale, float* out, hipStream_t s);
 // y[m] (+)= sum_r m[r][m] u[r] for k (<= 64) rows u of length n and an mcols-column device matrix m
-void gram32_device(const float* a, const float* b, long long n, int ka, int kb, double* out, double* work,
-                   int work_blocks, hipStream_t s);
-void combine32_device(const float* u, long long n, int k, const double* m, int mcols, float* y, int accumulate,
-                      hipStream_t s);
 void combine_device(const double* u, long long n, int k, const double* m, int mcols, double* y, int accumulate,
                     hipStream_t s);
 // rigid-body kernel of floating fixtures (rigid.hip); coef is 6 x nk row-major over the rigid modes

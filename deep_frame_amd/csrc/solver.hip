@@ -113,7 +113,7 @@ struct Handle {
   // PCG workspace (padded grid, FP64)
   int k_cap = 0;
   DevBuf b, x, r, z, zn, p, ap, tmp, partial, scal, active;
-  // block Krylov solve: Gram partials and the small k x k matrices (T, alpha, beta)
+  // block Krylov solve: Gram and norm partials, and the small k x k matrices (S, W^T Z, step coefficients)
   DevBuf bk_work, bk_mat;
   std::vector<Geom> geoms;
   // learned coarse correction (not owned) and training-sample harvest (caller's device buffer)
@@ -1387,59 +1387,81 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
 }
 
 // ---- block Krylov solve -------------------------------------------------------------------
-// Symmetric eigendecomposition of a small n x n matrix (cyclic Jacobi, FP64, host): a is destroyed, w gets
-// the eigenvalues and v (row-major, column j = eigenvector j) the eigenvectors.
-static void jacobi_eigh(std::vector<double>& a, int n, std::vector<double>& w, std::vector<double>& v) {
-  v.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) (i == j ? diag : off) += a[(size_t)i * n + j] * a[(size_t)i * n + j];
-    if (off <= 1e-30 * diag) break;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a[(size_t)p * n + q];
-        if (apq == 0.0) continue;
-        const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
-        for (int r = 0; r < n; ++r) {  // columns p, q
-          const double arp = a[(size_t)r * n + p], arq = a[(size_t)r * n + q];
-          a[(size_t)r * n + p] = c * arp - sn * arq;
-          a[(size_t)r * n + q] = sn * arp + c * arq;
-        }
-        for (int r = 0; r < n; ++r) {  // rows p, q
-          const double apr = a[(size_t)p * n + r], aqr = a[(size_t)q * n + r];
-          a[(size_t)p * n + r] = c * apr - sn * aqr;
-          a[(size_t)q * n + r] = sn * apr + c * aqr;
-        }
-        for (int r = 0; r < n; ++r) {
-          const double vrp = v[(size_t)r * n + p], vrq = v[(size_t)r * n + q];
-          v[(size_t)r * n + p] = c * vrp - sn * vrq;
-          v[(size_t)r * n + q] = sn * vrp + c * vrq;
-        }
+constexpr int BLOCK_MAX_K = 48;
+
+// Pseudo-inverse of the block Gram matrix G = V^T A V (k x k, row-major, symmetrised here) over a
+// well-conditioned subset of the columns: with C = D^-1/2 G D^-1/2 (unit diagonal), a Cholesky factorisation
+// with symmetric pivoting on the largest remaining diagonal keeps columns while their A-energy orthogonal
+// to the kept ones, relative to their own, exceeds `drop`; then S = D^-1/2 Pi (L L^T)^-1 Pi^T D^-1/2 on the
+// kept columns and 0 elsewhere.  S = T T^T with T = D^-1/2 Pi L^-T, and V T is A-orthonormal.  Returns the
+// number of kept columns.  (k <= 48: a few tens of microseconds on the host.)
+static int block_pinv(const std::vector<double>& g, int k, double drop, std::vector<double>& S) {
+  S.assign((size_t)k * k, 0.0);
+  std::vector<int> idx;
+  for (int c = 0; c < k; ++c) {
+    const double d = g[(size_t)c * k + c];
+    if (d > 0 && std::isfinite(d)) idx.push_back(c);
+  }
+  const int nz = (int)idx.size();
+  if (nz == 0) return 0;
+  std::vector<double> dn(nz), C((size_t)nz * nz);
+  for (int i = 0; i < nz; ++i) dn[i] = std::sqrt(g[(size_t)idx[i] * k + idx[i]]);
+  for (int i = 0; i < nz; ++i)
+    for (int j = 0; j < nz; ++j)
+      C[(size_t)i * nz + j] =
+          0.5 * (g[(size_t)idx[i] * k + idx[j]] + g[(size_t)idx[j] * k + idx[i]]) / (dn[i] * dn[j]);
+  std::vector<int> perm(nz);
+  for (int i = 0; i < nz; ++i) perm[i] = i;
+  auto at = [&](int i, int j) -> double& { return C[(size_t)i * nz + j]; };
+  int m = 0;
+  for (; m < nz; ++m) {
+    int piv = m;
+    for (int q = m + 1; q < nz; ++q)
+      if (at(q, q) > at(piv, piv)) piv = q;
+    if (!(at(piv, piv) > drop)) break;
+    if (piv != m) {  // symmetric swap of rows and columns m, piv
+      for (int q = 0; q < nz; ++q) std::swap(at(m, q), at(piv, q));
+      for (int q = 0; q < nz; ++q) std::swap(at(q, m), at(q, piv));
+      std::swap(perm[m], perm[piv]);
+    }
+    const double lmm = std::sqrt(at(m, m));
+    at(m, m) = lmm;
+    for (int i = m + 1; i < nz; ++i) at(i, m) /= lmm;
+    for (int i = m + 1; i < nz; ++i)
+      for (int j = m + 1; j <= i; ++j) {
+        at(i, j) -= at(i, m) * at(j, m);
+        at(j, i) = at(i, j);
       }
   }
-  w.resize(n);
-  for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
+  if (m == 0) return 0;
+  // Li = L^-1 (m x m lower triangular) by forward substitution, then (L L^T)^-1 = Li^T Li
+  std::vector<double> Li((size_t)m * m, 0.0);
+  for (int c = 0; c < m; ++c) {
+    Li[(size_t)c * m + c] = 1.0 / at(c, c);
+    for (int i = c + 1; i < m; ++i) {
+      double acc = 0.0;
+      for (int q = c; q < i; ++q) acc += at(i, q) * Li[(size_t)q * m + c];
+      Li[(size_t)i * m + c] = -acc / at(i, i);
+    }
+  }
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b <= a; ++b) {
+      double acc = 0.0;
+      for (int q = a; q < m; ++q) acc += Li[(size_t)q * m + a] * Li[(size_t)q * m + b];
+      const int ia = perm[a], ib = perm[b];
+      const double v = acc / (dn[ia] * dn[ib]);
+      S[(size_t)idx[ia] * k + idx[ib]] = v;
+      S[(size_t)idx[ib] * k + idx[ia]] = v;
+    }
+  return m;
 }
-
-__global__ void k_negate_copy(const double* in, double* out, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = -in[i];
-}
-
-constexpr int BLOCK_MAX_K = 48;
-constexpr int BLOCK_GRAM_BLOCKS = 512;
 
 // Breakdown-free block PCG over the k columns (FP32 vectors and V-cycle, reliable FP64 updates as in
-// solve_refine).  Each iteration makes the search block A-orthonormal: W = A V, the eigendecomposition of
-// V^T W (normalised by its diagonal, directions with relative energy below 1e-6 dropped) gives T, and
-// P = V T, Q = W T satisfy P^T A P = I.  Then alpha = P^T R, D += P alpha, R -= Q alpha, Z = M R,
-// beta = -Q^T Z, and the next block V = Z + P beta.  Products over the DOFs run on the FP64 matrix cores;
-// only the k x k eigenproblem is on the host.
+// solve_refine).  With W = A V, G = V^T W and S its pseudo-inverse (block_pinv), the A-orthonormal block
+// P = V T (S = T T^T) is never formed: the step is D += V a, R -= W a with a = S V^T R, and the next block is
+// V' = Z - V S W^T Z for Z = M R (= Z + P beta, beta = -(A P)^T Z).  One iteration therefore makes four passes
+// over the DOFs besides the operator apply and the V-cycle: the Gram V^T [W | R], the paired update (which
+// also returns |R_j|^2), the Gram W^T Z and the combination (block.hip).  Only S (k x k) is formed on the host.
 static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_flat, int k, dfmi_report* rep) {
   if (h->st.precision != 1) throw std::invalid_argument("block Krylov needs the fp32 V-cycle precision");
   if (k > BLOCK_MAX_K) throw std::invalid_argument("block Krylov takes at most 48 columns");
@@ -1455,15 +1477,18 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
   double* r = h->r.as<double>();
   float* r32 = h->z.as<float>();
   float* d32 = r32 + kn;
-  float* p32 = h->zn.as<float>();
-  float* q32 = p32 + kn;
+  float* vcur = h->zn.as<float>();  // V and the next V alternate between the halves of zn
+  float* vnext = vcur + kn;
   float* w32 = reinterpret_cast<float*>(h->p.as<double>());
-  h->bk_work.ensure(sizeof(double) * BLOCK_MAX_K * BLOCK_MAX_K * BLOCK_GRAM_BLOCKS);
-  h->bk_mat.ensure(sizeof(double) * BLOCK_MAX_K * BLOCK_MAX_K * 4);
-  double* gmat = h->bk_mat.as<double>();            // k x k Gram
-  double* tmat = gmat + BLOCK_MAX_K * BLOCK_MAX_K;   // k x m
-  double* amat = tmat + BLOCK_MAX_K * BLOCK_MAX_K;   // m x k (alpha or beta)
-  double* nmat = amat + BLOCK_MAX_K * BLOCK_MAX_K;   // -alpha
+  const int gblocks = block_gram_blocks(n), cblocks = block_combine_blocks(n);
+  h->bk_work.ensure(sizeof(double) * std::max((size_t)BLOCK_MAX_K * 2 * BLOCK_MAX_K * gblocks,
+                                              (size_t)BLOCK_MAX_K * cblocks));
+  h->bk_mat.ensure(sizeof(double) * BLOCK_MAX_K * BLOCK_MAX_K * 6);
+  double* g1 = h->bk_mat.as<double>();                  // k x 2k: V^T [W | R]
+  double* smat = g1 + 2 * BLOCK_MAX_K * BLOCK_MAX_K;     // k x k: S
+  double* hmat = smat + BLOCK_MAX_K * BLOCK_MAX_K;       // k x k: W^T Z
+  float* amat = reinterpret_cast<float*>(hmat + BLOCK_MAX_K * BLOCK_MAX_K);  // k x 48 FP32: step coefficients
+  float* bmat = amat + BLOCK_MAX_K * BLOCK_MAX_K;                            // k x 48 FP32: next-block coefficients
   double* work = h->bk_work.as<double>();
   double* sc = h->scal.as<double>();
   const size_t kc = (size_t)h->k_cap;
@@ -1473,8 +1498,6 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
   int* act = h->active.as<int>();
   Level& F = *H.levels[0];
   VArgs v{h, &H, k, act};
-  const int nbr = reduce_blocks(n);
-  double* part = h->partial.as<double>();
 
   flat_to_grid(h->pad, h->ux, h->uy, h->uz, rhs_flat, b, k, s);
   mask_vector(F.mask.as<uint8_t>(), b, h->pad.nodes(), k, s);
@@ -1523,61 +1546,47 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
     sync(h);
     return;
   }
-  // V = M R, with the columns of finished right-hand sides zero
+  // Z = M R, with the columns of finished right-hand sides zero (in the fine level's x vector)
   auto precondition_block = [&]() -> float* {
     vcycle<float>(v, 0, r32);
     float* z = F.x.as<float>();
     zero_columns(z, inactive_of());
     return z;
   };
-  float* vb = precondition_block();
+  {
+    const float* z = precondition_block();
+    DFMI_HIP(hipMemcpyAsync(vcur, z, sizeof(float) * kn, hipMemcpyDeviceToDevice, s));
+  }
   int step = 0;
-  std::vector<double> g((size_t)k * k), w, evec, t;
+  std::vector<double> g((size_t)k * 2 * k), S;
+  std::vector<float> af((size_t)k * BLOCK_MAX_K);
   while (count(active) && step < h->st.max_iterations) {
-    // W = A V over all k columns, G = V^T W
-    {
+    {  // W = A V over all k columns
       ApplyArgs a{};
-      a.x = vb;
+      a.x = vcur;
       a.y = w32;
       a.k = k;
+      a.premasked = 1;  // V is a combination of masked V-cycle outputs
       fine_apply<float>(h->pad, h->m32.as<float>(), F.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, s, nullptr);
     }
-    gram32_device(vb, w32, n, k, k, gmat, work, BLOCK_GRAM_BLOCKS, s);
-    DFMI_HIP(hipMemcpyAsync(g.data(), gmat, sizeof(double) * k * k, hipMemcpyDeviceToHost, s));
+    block_gram(vcur, k, w32, k, r32, k, n, g1, work, s);  // [V^T W | V^T R]
+    DFMI_HIP(hipMemcpyAsync(g.data(), g1, sizeof(double) * k * 2 * k, hipMemcpyDeviceToHost, s));
     sync(h);
-    // T = D^-1 U Lambda^-1/2 over the kept directions of the diagonal-normalised V^T A V
-    std::vector<int> idx;
-    for (int c = 0; c < k; ++c)
-      if (g[(size_t)c * k + c] > 0 && std::isfinite(g[(size_t)c * k + c])) idx.push_back(c);
-    const int nz = (int)idx.size();
-    if (nz == 0) break;
-    std::vector<double> dn(nz), cm((size_t)nz * nz);
-    for (int i = 0; i < nz; ++i) dn[i] = std::sqrt(g[(size_t)idx[i] * k + idx[i]]);
-    for (int i = 0; i < nz; ++i)
-      for (int j = 0; j < nz; ++j) cm[(size_t)i * nz + j] = g[(size_t)idx[i] * k + idx[j]] / (dn[i] * dn[j]);
-    jacobi_eigh(cm, nz, w, evec);
-    double wmax = 0.0;
-    for (double e : w) wmax = std::max(wmax, e);
-    std::vector<int> keep;
-    for (int j = 0; j < nz; ++j)
-      if (w[j] > 1e-6 * wmax) keep.push_back(j);
-    const int m = (int)keep.size();
-    if (m == 0) break;
-    t.assign((size_t)k * m, 0.0);
-    for (int i = 0; i < nz; ++i)
-      for (int q = 0; q < m; ++q)
-        t[(size_t)idx[i] * m + q] = evec[(size_t)i * nz + keep[q]] / (dn[i] * std::sqrt(w[keep[q]]));
-    DFMI_HIP(hipMemcpyAsync(tmat, t.data(), sizeof(double) * k * m, hipMemcpyHostToDevice, s));
-    combine32_device(vb, n, k, tmat, m, p32, 0, s);   // P = V T
-    combine32_device(w32, n, k, tmat, m, q32, 0, s);  // Q = W T = A P
-    // alpha = P^T R; D += P alpha; R -= Q alpha
-    gram32_device(p32, r32, n, m, k, amat, work, BLOCK_GRAM_BLOCKS, s);
-    hipLaunchKernelGGL(k_negate_copy, dim3((m * k + 255) / 256), dim3(256), 0, s, amat, nmat, m * k);
-    DFMI_HIP(hipGetLastError());
-    combine32_device(p32, n, m, amat, k, d32, 1, s);
-    combine32_device(q32, n, m, nmat, k, r32, 1, s);
-    column_dot<float>(r32, r32, n, k, part, nbr, act, s);
-    reduce_partials(part, nbr, k, rr, s);
+    std::vector<double> gw((size_t)k * k);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) gw[(size_t)i * k + j] = g[(size_t)i * 2 * k + j];
+    if (block_pinv(gw, k, 1e-5, S) == 0) break;
+    // a = S V^T R (rows: input columns of V, columns: right-hand sides)
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < BLOCK_MAX_K; ++j) {
+        double acc = 0.0;
+        if (j < k)
+          for (int q = 0; q < k; ++q) acc += S[(size_t)i * k + q] * g[(size_t)q * 2 * k + k + j];
+        af[(size_t)i * BLOCK_MAX_K + j] = (float)acc;
+      }
+    DFMI_HIP(hipMemcpyAsync(amat, af.data(), sizeof(float) * k * BLOCK_MAX_K, hipMemcpyHostToDevice, s));
+    DFMI_HIP(hipMemcpyAsync(smat, S.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, s));
+    block_update(vcur, d32, w32, r32, amat, k, n, rr, work, s);  // D += V a, R -= W a, |R_j|^2
     DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     sync(h);
     ++step;
@@ -1623,13 +1632,12 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
       upload(act, active);
     }
     if (!count(active)) break;
-    // Z = M R; beta = -Q^T Z; V = Z + P beta
-    float* z = precondition_block();
-    gram32_device(q32, z, n, m, k, amat, work, BLOCK_GRAM_BLOCKS, s);
-    hipLaunchKernelGGL(k_negate_copy, dim3((m * k + 255) / 256), dim3(256), 0, s, amat, nmat, m * k);
-    DFMI_HIP(hipGetLastError());
-    combine32_device(p32, n, m, nmat, k, z, 1, s);
-    vb = z;
+    // Z = M R; V' = Z - V S W^T Z
+    const float* z = precondition_block();
+    block_gram(w32, k, z, k, nullptr, 0, n, hmat, work, s);
+    block_small_product(smat, hmat, k, -1.0, bmat, s);
+    block_combine(vcur, z, vnext, bmat, k, k, n, s);
+    std::swap(vcur, vnext);
   }
   if (count(active)) {  // iteration limit: flush and report the true residual
     upload(act, active);

@@ -225,6 +225,25 @@ int dfmi_gram_static(const double* a_dev, const double* b_dev, int64_t n, int32_
 int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev, int32_t mcols, double* y_dev,
                  int32_t accumulate, void* hip_stream);
 
+/* Passes over the DOFs of the block PCG recurrence that dfmi_solve runs with settings.block_krylov (no
+ * counterpart in the reference, whose _pcg, topology_multigrid.py:407-454, treats the columns separately;
+ * exported for tests and for callers that run their own block recurrence).  Operands are FP32 column-major
+ * multi-vectors (column j at dev + j * n), at most 48 columns; coefficient matrices are FP32 with row stride 48
+ * (entries past the used columns zero).  Sums over DOFs are FP32 within a wave's share and FP64 across waves
+ * and workgroups, deterministic.
+ *   dfmi_block_work_doubles: size of the work buffer of a Gram with ka x kb outputs and of an update;
+ *   dfmi_block_gram: out_dev (ka x (kb1 + kb2), FP64 row-major) = A^T [B1 | B2]; b2_dev may be NULL with kb2 = 0,
+ *     otherwise ceil(ka / 16), ceil(kb1 / 16) and ceil(kb2 / 16) must be equal (and so must the first two);
+ *   dfmi_block_combine: Y = X + U M for kin input and kout output columns (y may alias x, not u);
+ *   dfmi_block_update: D += V M, R -= W M (k columns) and rr_dev[j] = |R_j|^2 (FP64) after the update. */
+int dfmi_block_work_doubles(int64_t n, int32_t ka, int32_t kb, int64_t* count);
+int dfmi_block_gram(const float* a_dev, int32_t ka, const float* b1_dev, int32_t kb1, const float* b2_dev,
+                    int32_t kb2, int64_t n, double* out_dev, double* work_dev, void* hip_stream);
+int dfmi_block_combine(const float* u_dev, const float* x_dev, float* y_dev, const float* m_dev, int32_t kin,
+                       int32_t kout, int64_t n, void* hip_stream);
+int dfmi_block_update(const float* v_dev, float* d_dev, const float* w_dev, float* r_dev, const float* m_dev,
+                      int32_t k, int64_t n, double* rr_dev, double* work_dev, void* hip_stream);
+
 /* SIMP interpolation E = E0 (eps + (1 - eps) rho^p), dE — topology_optimization.py:657-661;
  * inactive cells (active_dev[e] == 0) get 0. */
 int dfmi_simp(const double* rho_dev, const uint8_t* active_dev, int64_t n, double young, double penal, double eps,

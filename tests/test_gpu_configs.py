@@ -1,7 +1,7 @@
 """The workloads at their real sizes on the GPU:
 
 * frame 96 x 96 x 16 (479,859 DOF, 42 unit-wrench RHS): the FP64 scheme and the mixed configuration of the
-  benchmark against oracle fixtures (tests/golden/frame_96x96x16_42rhs.npz, written by
+  benchmark (block PCG, and the same with one recurrence per column) against oracle fixtures (tests/golden/frame_96x96x16_42rhs.npz, written by
   tests/golden/make_golden_frame.py with the oracle's FP64 GMG-PCG to 1e-12): 42 compliances to 1e-6 and the
   summed sensitivity to 1e-5;
 * frame 256 x 256 x 48 (9,709,203 DOF, 42 RHS): the benchmark's mixed configuration and the reference's FP64
@@ -23,8 +23,10 @@ from deep_frame_amd import element, problems
 
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
+# the benchmark's mixed configuration: block PCG over the columns; MIXED_COLUMNS: one recurrence per column
 MIXED = {"precision": "float32", "refine": True, "fine_smoother": "vector", "coarse_krylov": 2, "krylov_levels": 2,
-         "growth": 1.0, "chebyshev_degree": 3, "inner_tolerance": 1e-4}
+         "growth": 1.0, "chebyshev_degree": 3, "inner_tolerance": 1e-4, "block_krylov": True}
+MIXED_COLUMNS = {**MIXED, "block_krylov": False}
 FP64 = {"precision": "float64"}
 
 
@@ -55,7 +57,7 @@ def _true_residual(mg, u, f):
     return float(res.max())
 
 
-@pytest.mark.parametrize("settings", [FP64, MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("settings", [FP64, MIXED, MIXED_COLUMNS], ids=["fp64", "mixed", "mixed-per-column"])
 def test_frame_96x96x16_matches_oracle_fixture(settings, torch_gpu):
     print(f"frame 96x96x16 {settings}", flush=True)
     path = GOLDEN / "frame_96x96x16_42rhs.npz"

@@ -189,12 +189,13 @@ def test_kcycle_solves_frame_wrenches_in_fewer_iterations(torch_gpu):
     # block PCG against the per-column recurrence with the same preconditioner (V-cycle; the bench's K-cycle)
     assert its["refine-block"] < its["refine"] and its["refine-block-krylov"] < its["refine-bench"]
     # every scheme meets the same true residual; at this 1e9 stiffness contrast that fixes the displacements to
-    # about 1e-6 and the compliances f.u (the quantity the caller uses) to well within 1e-6
+    # about 1e-6 to 1e-5 (the residual bounds them only through the conditioning) and the compliances f.u (the
+    # quantity the caller uses) to well within 1e-6
     forces = np.ascontiguousarray(prob["forces"].T)
     for mode in ("float64-krylov", "refine", "refine-krylov", "refine-bench", "refine-block", "refine-block-krylov"):
         for c in range(42):
             a, b = out["float64"][c], out[mode][c]
-            assert np.linalg.norm(a - b) <= 5e-6 * np.linalg.norm(a), (mode, c)
+            assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a), (mode, c)
             assert abs(forces[c] @ b - forces[c] @ a) <= 1e-6 * abs(forces[c] @ a), (mode, c)
 
 
