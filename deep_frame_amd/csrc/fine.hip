@@ -156,7 +156,7 @@ struct FineCfg {
 constexpr int kChebPartial = 100;
 template <typename T, int KMODE, bool PRE>
 __global__ __launch_bounds__(FineCfg<T>::NT)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE == MODE_DOT ? 6 : 1))) void k_fine_apply(
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE != MODE_MASS ? 6 : 1))) void k_fine_apply(
     Geom g, const T* __restrict__ moduli, const uint8_t* __restrict__ mask, KeSym<T> ks, ApplyArgs a, int TY, int TZ,
     int xchunk, int tiles_y, int tiles_z) {
   constexpr int MODE = KMODE == kChebPartial ? MODE_CHEB : KMODE;
@@ -424,22 +424,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE == MODE_DOT ? 6 : 1))
           w[q][2] = m * y[q][2];
         }
       } else {
-#pragma unroll
-        for (int q = 1; q < 8; ++q) {
-          y[q][0] *= e_cur;
-          y[q][1] *= e_cur;
-          y[q][2] *= e_cur;
-        }
-        sym_mul(ks, y, w);
+        sym_mul(ks, y, w);  // unscaled: the modulus enters in the x stage below
       }
-      // x stage of the inverse transform: plane-i face (this element) plus the previous element's
-      // plane-i face, then one inverse face transform gives the four corner sums at plane i
+      // x stage of the inverse transform: plane-i face (this element, times its modulus: 12 products instead
+      // of 21 on the modes) plus the previous element's plane-i face, then one inverse face transform gives the
+      // four corner sums at plane i
+      const T es = MODE == MODE_MASS ? T(1) : e_cur;  // the mass path scaled its modes already
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          g[m][c] = gc[m][c] + (w[2 * m][c] + w[2 * m + 1][c]);
-          gc[m][c] = w[2 * m][c] - w[2 * m + 1][c];
+          g[m][c] = fma(es, w[2 * m][c] + w[2 * m + 1][c], gc[m][c]);
+          gc[m][c] = es * (w[2 * m][c] - w[2 * m + 1][c]);
         }
       face_butterfly(g);
     }
