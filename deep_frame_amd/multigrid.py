@@ -135,12 +135,13 @@ class GeometricMultigrid:
             self.hierarchies.pop(next(iter(self.hierarchies)))
         used = set(h["slot"] for h in self.hierarchies.values())
         slot = next(i for i in range(SLOTS) if i not in used)
-        flags = np.zeros(self.ndof, dtype=np.uint8)
         fixed = np.asarray(fixed, dtype=np.int64)
         if fixed.size and (fixed.min() < 0 or fixed.max() >= self.ndof):
             raise ValueError("fixed DOF index out of range")
-        flags[fixed] = 1
-        dev = self._tensor(flags, self.torch.uint8)
+        # the DOF flags are scattered on the device: only the fixed indices cross PCIe
+        dev = self.torch.zeros(self.ndof, dtype=self.torch.uint8, device=self.device)
+        if fixed.size:
+            dev[self._tensor(fixed, self.torch.int64)] = 1
         coefficients = self.rigid_coefficients(fixed)
         nk = coefficients.shape[1] if self.settings["projection"] else 0
         self._stream()
@@ -164,7 +165,11 @@ class GeometricMultigrid:
         self.hierarchies[key] = entry
         return entry
 
-    def _points(self):
+    def _points(self, nodes=None):
+        """Coordinates (n, 3) of all nodes, or of the given flat node indices."""
+        if nodes is not None:
+            return np.stack(np.unravel_index(np.asarray(nodes, dtype=np.int64), self.nodes), axis=1) * \
+                np.asarray(self.spacing, float)
         axes = [np.arange(n) * h for n, h in zip(self.nodes, self.spacing)]
         return np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
 
@@ -186,7 +191,7 @@ class GeometricMultigrid:
         if not constrained.size:
             return np.eye(6)
         nodes = np.unique(constrained // 3)
-        modes = rigid_modes(self._points()[nodes], self.center)
+        modes = rigid_modes(self._points(nodes), self.center)
         local = np.searchsorted(nodes, constrained // 3) * 3 + constrained % 3
         fixed_modes = modes[:, local]
         values, vectors = np.linalg.eigh(fixed_modes @ fixed_modes.T)
