@@ -20,7 +20,8 @@ class Settings(Structure):
                 ("max_iterations", c_int32), ("dead_ratio", c_double), ("seed", c_int32), ("refine", c_int32),
                 ("inner_tolerance", c_double), ("fine_smoother", c_int32), ("retries", c_int32),
                 ("coarse_krylov", c_int32), ("block_krylov", c_int32), ("smoother", c_int32), ("sweeps", c_int32),
-                ("damping", c_double), ("cycle", c_int32), ("coarse", c_int32), ("krylov_levels", c_int32)]
+                ("damping", c_double), ("cycle", c_int32), ("coarse", c_int32), ("krylov_levels", c_int32),
+                ("chebyshev_kind", c_int32)]
 
 
 class Report(Structure):

@@ -282,52 +282,80 @@ static void final_dot(VArgs& v, int idx, bool last, ApplyArgs& a) {
   a.partial = v.final_dot;
 }
 
-// Chebyshev smoothing of A x = b (topology_multigrid.py:284-305) in the three-term form
-// x_{j+1} = x_j + rho_j rho_{j-1} (x_j - x_{j-1}) + (2 rho_j / delta) Dinv (b - A x_j).
-// On exit the iterate is in L.x.
-template <typename T>
-static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) {
-  Level& L = *v.H->levels[idx];
-  const double upper = 1.1 * L.lambda_max, lower = upper / v.h->st.chebyshev_ratio;
+// Step coefficients of the Chebyshev smoother (topology_multigrid.py:284-305 in the three-term form) for `count` steps: the first step is x1 = x0 + c2[0] Dinv r0, step
+// j >= 1 is x_{j+1} = x_j + c1[j] (x_j - x_{j-1}) + c2[j] Dinv (b - A x_j).
+//   kind 1 (the reference's polynomial, topology_multigrid.py:291-304): bounds [upper / ratio, upper], theta and
+//     delta their centre and half-width, sigma = theta / delta, rho_0 = 1 / sigma, c2[0] = 1 / theta and
+//     rho_j = 1 / (2 sigma - rho_{j-1}), c1[j] = rho_j rho_{j-1}, c2[j] = 2 rho_j / delta;
+//   kind 4 (Chebyshev polynomials of the fourth kind, Lottes 2023): c2[0] = 4 / (3 upper) and
+//     c1[j] = (2j - 1) / (2j + 3), c2[j] = (8j + 4) / ((2j + 3) upper); the error polynomial after k steps is
+//     W_k(1 - 2t) / (2k + 1), t = eigenvalue / upper, W_k the fourth-kind Chebyshev polynomial.
+// upper = 1.1 lambda_max in both.
+static void chebyshev_coefficients(int kind, double lambda_max, double ratio, int count, std::vector<double>& c1,
+                                   std::vector<double>& c2) {
+  const double upper = 1.1 * lambda_max;
+  c1.assign(std::max(count, 1), 0.0);
+  c2.assign(std::max(count, 1), 0.0);
+  if (kind == 4) {
+    c2[0] = 4.0 / (3.0 * upper);
+    for (int j = 1; j < count; ++j) {
+      c1[j] = (2.0 * j - 1.0) / (2.0 * j + 3.0);
+      c2[j] = (8.0 * j + 4.0) / ((2.0 * j + 3.0) * upper);
+    }
+    return;
+  }
+  const double lower = upper / ratio;
   const double theta = 0.5 * (upper + lower), delta = 0.5 * (upper - lower);
   const double sigma = theta / delta;
   double rho = 1.0 / sigma;
+  c2[0] = 1.0 / theta;
+  for (int j = 1; j < count; ++j) {
+    const double nr = 1.0 / (2.0 * sigma - rho);
+    c1[j] = nr * rho;
+    c2[j] = 2.0 * nr / delta;
+    rho = nr;
+  }
+}
+
+// Chebyshev smoothing of A x = b on level idx (coefficients above), from x = 0 or from the level's x.  On exit
+// the iterate is in L.x.
+template <typename T>
+static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) {
+  Level& L = *v.H->levels[idx];
+  std::vector<double> c1, c2;
+  chebyshev_coefficients(v.h->st.chebyshev_kind, L.lambda_max, v.h->st.chebyshev_ratio, count, c1, c2);
   T* cur = L.x.as<T>();
   T* other = L.xa.as<T>();
   const T* dinv = level_dinv<T>(L);
   bool prev_zero;
   int j0 = 1;
   if (from_zero && idx == 0 && count >= 2) {
-    // fine level: x1 = Dinv b / theta is formed inside the first step (no separate pass) and re-formed
-    // from b as the previous iterate of the second step
-    const double nr = 1.0 / (2.0 * sigma - rho);
+    // fine level: x1 = c2[0] Dinv b is formed inside the first step (no separate pass) and re-formed from b as
+    // the previous iterate of the second step
     ApplyArgs a{};
     a.b = b;
     a.dinv = dinv;
-    a.c0 = 1.0 / theta;
-    a.c1 = nr * rho;
-    a.c2 = 2.0 * nr / delta;
+    a.c0 = c2[0];
+    a.c1 = c1[1];
+    a.c2 = c2[1];
     a.y = cur;
     level_apply<T>(v, idx, MODE_CHEB0, a);
-    rho = nr;
     if (count >= 3) {
-      const double nr2 = 1.0 / (2.0 * sigma - rho);
       ApplyArgs a2{};
       a2.x = cur;
       a2.y = other;
       a2.b = b;
       a2.dinv = dinv;
-      a2.c0 = 1.0 / theta;
-      a2.c1 = nr2 * rho;
-      a2.c2 = 2.0 * nr2 / delta;
+      a2.c0 = c2[0];
+      a2.c1 = c1[2];
+      a2.c2 = c2[2];
       level_apply<T>(v, idx, MODE_CHEB1, a2);
       std::swap(cur, other);
-      rho = nr2;
     }
     prev_zero = false;
     j0 = std::min(count, 3);
   } else if (from_zero) {
-    cheb_start<T>(b, dinv, 1.0 / theta, cur, L.g.nodes(), v.k, v.active, level_layout(L, idx, v.k), v.h->stream);
+    cheb_start<T>(b, dinv, c2[0], cur, L.g.nodes(), v.k, v.active, level_layout(L, idx, v.k), v.h->stream);
     prev_zero = true;
   } else {
     ApplyArgs a{};
@@ -337,27 +365,25 @@ static void chebyshev(VArgs& v, int idx, const T* b, bool from_zero, int count) 
     a.xp = nullptr;
     a.dinv = dinv;
     a.c1 = 0.0;
-    a.c2 = 1.0 / theta;
+    a.c2 = c2[0];
     final_dot(v, idx, count == 1, a);
     level_apply<T>(v, idx, MODE_CHEB, a, v.final_blocks);
     std::swap(cur, other);
     prev_zero = false;
   }
   for (int j = j0; j < count; ++j) {
-    const double nr = 1.0 / (2.0 * sigma - rho);
     ApplyArgs a{};
     a.x = cur;
     a.y = other;
     a.b = b;
     a.xp = prev_zero ? nullptr : other;
     a.dinv = dinv;
-    a.c1 = nr * rho;
-    a.c2 = 2.0 * nr / delta;
+    a.c1 = c1[j];
+    a.c2 = c2[j];
     final_dot(v, idx, !from_zero && j == count - 1, a);
     level_apply<T>(v, idx, MODE_CHEB, a, v.final_blocks);
     std::swap(cur, other);
     prev_zero = false;
-    rho = nr;
   }
   if (cur != L.x.as<T>()) std::swap(L.x.p, L.xa.p), std::swap(L.x.bytes, L.xa.bytes);
 }
@@ -1710,6 +1736,7 @@ void dfmi_default_settings(dfmi_settings* o) {
   o->cycle = 0;
   o->coarse = 0;
   o->krylov_levels = 1;
+  o->chebyshev_kind = 1;
 }
 
 int dfmi_create(const int32_t shape[3], const double spacing[3], const double* ke, const dfmi_settings* settings,
@@ -1730,6 +1757,7 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
     if (st.retries < 0) throw std::invalid_argument("retries must be >= 0");
     if (st.coarse_krylov < 0 || st.coarse_krylov > 2) throw std::invalid_argument("coarse_krylov must be 0, 1 or 2");
     if (st.krylov_levels < 1) throw std::invalid_argument("krylov_levels must be >= 1");
+    if (st.chebyshev_kind != 1 && st.chebyshev_kind != 4) throw std::invalid_argument("chebyshev_kind must be 1 or 4");
     if (st.block_krylov != 0 && st.block_krylov != 1) throw std::invalid_argument("block_krylov must be 0 or 1");
     if (st.chebyshev_degree < 1 || st.max_iterations < 1 || !(st.tolerance > 0))
       throw std::invalid_argument("invalid multigrid settings");

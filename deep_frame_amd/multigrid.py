@@ -20,7 +20,7 @@ MULTIGRID = {"coarse": "galerkin", "coarsest_dofs": 80000, "max_levels": 8, "smo
              "power_iterations": 20, "precision": "float32", "tolerance": 1e-8, "max_iterations": 2000,
              "batch": 64, "projection": True, "coarsest_projection": True, "dead_ratio": 1e-12, "zero_rhs": 1e-12, "levels": 0, "seed": 0,
              "refine": False, "inner_tolerance": 1e-2, "device": "cuda", "fine_smoother": "vector", "retries": 2,
-             "coarse_krylov": 0, "krylov_levels": 1, "block_krylov": False,
+             "coarse_krylov": 0, "krylov_levels": 1, "block_krylov": False, "chebyshev_kind": 1,
              "eigen_tolerance": 1e-8, "eigen_iterations": 300, "eigen_guard": 2, "eigen_refresh": 10,
              "eigen_drop": 1e-12, "eigen_seed": 0}
 PRECISIONS = {"float64": 0, "float32": 1}
@@ -104,6 +104,7 @@ class GeometricMultigrid:
         n.smoother, n.sweeps, n.damping = SMOOTHERS[s["smoother"]], int(s["sweeps"]), float(s["damping"])
         n.cycle, n.coarse = CYCLES[s["cycle"]], COARSE[s["coarse"]]
         n.krylov_levels = int(s["krylov_levels"])
+        n.chebyshev_kind = int(s["chebyshev_kind"])
         return n
 
     def _stream(self):

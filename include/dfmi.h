@@ -71,6 +71,12 @@ typedef struct dfmi_settings {
                                 (1, default: the fine level only) are Krylov solves on the next coarser level, so
                                 the V-cycles inside a Krylov solve use one again (a recursive K-cycle); levels
                                 whose coarser level is the coarsest keep the direct solve */
+  int32_t chebyshev_kind;    /* smoother = chebyshev: 1 (default) the reference's first-kind polynomial on
+                                [1.1 lambda_max / chebyshev_ratio, 1.1 lambda_max] (topology_multigrid.py:291-304);
+                                4 the fourth-kind polynomial (Lottes 2023), which needs only the upper bound
+                                1.1 lambda_max: error factor sin((2k+1) asin(sqrt t)) / ((2k+1) sqrt t) at
+                                t = eigenvalue of Dinv A / (1.1 lambda_max) after k steps; no counterpart in the
+                                reference */
 } dfmi_settings;
 
 /* Per-column outcome of a solve. */

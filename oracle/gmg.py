@@ -142,9 +142,22 @@ def _cols(v, like):
     return v if np.ndim(like) == 1 else v[:, None]
 
 
-def chebyshev(level, rhs, x, count, lmax, ratio=20.0):
+def chebyshev(level, rhs, x, count, lmax, ratio=20.0, kind=1):
+    """``count`` Chebyshev smoothing steps on level A x = rhs from x (None: zero), Jacobi-preconditioned.
+    kind 1: the reference's first-kind polynomial on [1.1 lmax / ratio, 1.1 lmax] (topology_multigrid.py:291-304);
+    kind 4: the fourth-kind polynomial (Lottes 2023, no reference counterpart), d_0 = 4 / (3 upper) Dinv r_0,
+    d_i = (2i - 1) / (2i + 3) d_{i-1} + (8i + 4) / ((2i + 3) upper) Dinv r_i, upper = 1.1 lmax."""
     a, dinv = level["A"], _cols(level["dinv"], rhs)
     upper = 1.1 * lmax
+    if kind == 4:
+        r = rhs.copy() if x is None else rhs - a @ x
+        d = 4.0 / (3.0 * upper) * dinv * r
+        x = d.copy() if x is None else x + d
+        for i in range(1, count):
+            r = r - a @ d
+            d = (2 * i - 1) / (2 * i + 3) * d + (8 * i + 4) / ((2 * i + 3) * upper) * dinv * r
+            x = x + d
+        return x
     lower = upper / ratio
     theta, delta = (upper + lower) / 2, (upper - lower) / 2
     sigma = theta / delta
@@ -171,7 +184,7 @@ def jacobi(level, rhs, x, count, lmax, damping=1.0):
 
 
 def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0, smoother="chebyshev", sweeps=2, damping=1.0,
-           cycle="V"):
+           cycle="V", kind=1):
     lv = levels[index]
     if index == len(levels) - 1:
         live = np.flatnonzero(lv["mask"] > 0)
@@ -184,43 +197,43 @@ def vcycle(levels, lmax, rhs, index=0, degree=3, growth=2.0, smoother="chebyshev
     def smooth(x):
         if smoother == "jacobi":
             return jacobi(lv, rhs, x, count, lmax[index], damping)
-        return chebyshev(lv, rhs, x, count, lmax[index])
+        return chebyshev(lv, rhs, x, count, lmax[index], kind=kind)
 
     x = smooth(None)
     p = levels[index + 1]["P"]
     for _ in range(2 if cycle == "W" and index + 2 < len(levels) else 1):
         res = (rhs - lv["A"] @ x) * _cols(lv["mask"], rhs)
         corr = vcycle(levels, lmax, (p.T @ res) * _cols(levels[index + 1]["mask"], rhs), index + 1, degree, growth,
-                      smoother, sweeps, damping, cycle)
+                      smoother, sweeps, damping, cycle, kind)
         x = x + (p @ corr) * _cols(lv["mask"], rhs)
     return smooth(x)
 
 
-def kvcycle(levels, lmax, rhs, index, steps, krylov_levels, degree=3, growth=2.0):
+def kvcycle(levels, lmax, rhs, index, steps, krylov_levels, degree=3, growth=2.0, kind=1):
     """V-cycle from level ``index`` whose coarse-grid correction on the first ``krylov_levels`` levels (counted
     from the fine level 0) is ``steps`` flexible-CG steps on the next level (``fcg``), as long as that level is
     not the coarsest; below, the plain V-cycle."""
     lv = levels[index]
     if index == len(levels) - 1 or not (index < krylov_levels and index + 2 < len(levels)):
-        return vcycle(levels, lmax, rhs, index, degree, growth)
+        return vcycle(levels, lmax, rhs, index, degree, growth, kind=kind)
     count = max(1, int(round(degree * growth ** index)))
-    x = chebyshev(lv, rhs, None, count, lmax[index])
+    x = chebyshev(lv, rhs, None, count, lmax[index], kind=kind)
     res = (rhs - lv["A"] @ x) * _cols(lv["mask"], rhs)
     p = levels[index + 1]["P"]
     corr = fcg(levels, lmax, index + 1, (p.T @ res) * _cols(levels[index + 1]["mask"], rhs), steps, degree, growth,
-               krylov_levels)
+               krylov_levels, kind)
     x = x + (p @ corr) * _cols(lv["mask"], rhs)
-    return chebyshev(lv, rhs, x, count, lmax[index])
+    return chebyshev(lv, rhs, x, count, lmax[index], kind=kind)
 
 
-def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0, krylov_levels=0):
+def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0, krylov_levels=0, kind=1):
     """``steps`` iterations of flexible CG on A_index x = rhs from x = 0, each preconditioned by the V-cycle
     from level ``index`` (itself a K-cycle while index < krylov_levels): alpha = r.z / p.A p,
     beta = r_new.(z_new - z_old) / r_old.z_old."""
     a = levels[index]["A"]
     x = np.zeros_like(rhs)
     r = rhs.copy()
-    z = kvcycle(levels, lmax, r, index, steps, krylov_levels, degree, growth)
+    z = kvcycle(levels, lmax, r, index, steps, krylov_levels, degree, growth, kind)
     p = z.copy()
     rz = r @ z
     for step in range(steps):
@@ -233,7 +246,7 @@ def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0, krylov_levels=0):
         if step == steps - 1:
             break
         r = r - alpha * ap
-        znew = kvcycle(levels, lmax, r, index, steps, krylov_levels, degree, growth)
+        znew = kvcycle(levels, lmax, r, index, steps, krylov_levels, degree, growth, kind)
         beta = (r @ (znew - z)) / rz
         rz = r @ znew
         p = znew + beta * p
@@ -241,13 +254,13 @@ def fcg(levels, lmax, index, rhs, steps, degree=3, growth=2.0, krylov_levels=0):
     return x
 
 
-def kcycle(levels, lmax, rhs, steps=2, degree=3, growth=2.0, krylov_levels=1):
+def kcycle(levels, lmax, rhs, steps=2, degree=3, growth=2.0, krylov_levels=1, kind=1):
     """The fine level's cycle with the coarse-grid corrections of the first ``krylov_levels`` levels solved by
     ``fcg`` (``steps`` flexible-CG steps) instead of one V-cycle; with two levels the level-1 solve is the
     exact coarsest one."""
     if len(levels) < 3:
-        return vcycle(levels, lmax, rhs, 0, degree, growth)
-    return kvcycle(levels, lmax, rhs, 0, steps, krylov_levels, degree, growth)
+        return vcycle(levels, lmax, rhs, 0, degree, growth, kind=kind)
+    return kvcycle(levels, lmax, rhs, 0, steps, krylov_levels, degree, growth, kind)
 
 
 def pcg(levels, lmax, rhs, tol=1e-8, maxit=2000, precondition=None):

@@ -83,12 +83,13 @@ def test_kcycle_matches_oracle(shape, spacing, precision, steps, torch_gpu):
 
 @pytest.mark.parametrize("settings", [{"smoother": "jacobi"}, {"smoother": "jacobi", "sweeps": 3, "damping": 0.8},
                                       {"cycle": "W"}, {"smoother": "jacobi", "cycle": "W"},
-                                      {"coarse": "rediscretize"}, {"coarse": "rediscretize", "cycle": "W"}])
+                                      {"coarse": "rediscretize"}, {"coarse": "rediscretize", "cycle": "W"},
+                                      {"chebyshev_kind": 4}, {"chebyshev_kind": 4, "cycle": "W"}])
 @pytest.mark.parametrize("precision", ["float64", "float32"])
 def test_reference_cycle_settings_match_oracle(settings, precision, torch_gpu):
     """The reference's other MULTIGRID choices (topology_multigrid.py:51, :160-166, :284-290, :312): damped Jacobi
-    smoothing, the W-cycle and rediscretised coarse levels, against the oracle's cycle on the same hierarchy
-    (FP64 to 1e-9, FP32 to 1e-4) for 1, 2 and 7 columns."""
+    smoothing, the W-cycle and rediscretised coarse levels, and the fourth-kind Chebyshev smoother, against the
+    oracle's cycle on the same hierarchy (FP64 to 1e-9, FP32 to 1e-4) for 1, 2 and 7 columns."""
     from deep_frame_amd.multigrid import GeometricMultigrid
     print(f"cycle settings={settings} precision={precision}", flush=True)
     shape, spacing = (16, 8, 8), (1.0, 1.0, 1.0)
@@ -106,7 +107,8 @@ def test_reference_cycle_settings_match_oracle(settings, precision, torch_gpu):
         r = np.random.default_rng(2).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
         ref = np.column_stack([gmg.vcycle(levels, entry["lambda_max"], r[:, c], smoother=settings.get("smoother", "chebyshev"),
                                           sweeps=settings.get("sweeps", 2), damping=settings.get("damping", 1.0),
-                                          cycle=settings.get("cycle", "V")) for c in range(cols)])
+                                          cycle=settings.get("cycle", "V"), kind=settings.get("chebyshev_kind", 1))
+                               for c in range(cols)])
         z = mg.precondition_device("v", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
         tol = 1e-9 if precision == "float64" else 1e-4
         for c in range(cols):
@@ -140,13 +142,14 @@ def test_reference_cycle_settings_solve_with_history(settings, torch_gpu):
 
 
 @pytest.mark.parametrize("precision", ["float64", "float32"])
-@pytest.mark.parametrize("krylov_levels", [2, 3])
-def test_recursive_kcycle_matches_oracle(precision, krylov_levels, torch_gpu):
+@pytest.mark.parametrize("krylov_levels,kind", [(2, 1), (3, 1), (2, 4)])
+def test_recursive_kcycle_matches_oracle(precision, krylov_levels, kind, torch_gpu):
     """The Krylov coarse solve applied recursively (krylov_levels): the V-cycle inside each flexible-CG step
     of level l solves its own coarse correction by flexible CG on level l + 1, on a 5-level hierarchy, against
-    oracle.gmg.kcycle (FP64 to 1e-9, FP32 to 1e-4) for 1, 2 and 7 columns."""
+    oracle.gmg.kcycle (FP64 to 1e-9, FP32 to 1e-4) for 1, 2 and 7 columns; first- and fourth-kind Chebyshev
+    smoothing."""
     from deep_frame_amd.multigrid import GeometricMultigrid
-    print(f"recursive kcycle levels={krylov_levels} precision={precision}", flush=True)
+    print(f"recursive kcycle levels={krylov_levels} kind={kind} precision={precision}", flush=True)
     shape, spacing = (32, 16, 16), (1.0, 1.0, 1.0)
     ke = element.stiffness(spacing, element.isotropic(0.3))
     rho = np.clip(np.random.default_rng(5).uniform(0.3, 1.2, int(np.prod(shape))), 0, 1)
@@ -154,13 +157,14 @@ def test_recursive_kcycle_matches_oracle(precision, krylov_levels, torch_gpu):
     pts = element.node_points(shape, spacing)
     fixed = problems.clamp_dofs(np.flatnonzero(np.abs(pts[:, 0]) < 1e-9))
     mg = GeometricMultigrid(shape, spacing, ke, {"levels": 5, "precision": precision, "coarse_krylov": 2,
-                                                 "krylov_levels": krylov_levels, "growth": 1.0})
+                                                 "krylov_levels": krylov_levels, "growth": 1.0,
+                                                 "chebyshev_kind": kind})
     mg.update(mod)
     entry = mg.hierarchy("k", fixed)
     levels = gmg.hierarchy(shape, ke, mod, fixed, (5, shape))
     for cols in (1, 2, 7):
         r = np.random.default_rng(6).normal(size=(levels[0]["A"].shape[0], cols)) * levels[0]["mask"][:, None]
-        ref = np.column_stack([gmg.kcycle(levels, entry["lambda_max"], r[:, c], 2, 3, 1.0, krylov_levels)
+        ref = np.column_stack([gmg.kcycle(levels, entry["lambda_max"], r[:, c], 2, 3, 1.0, krylov_levels, kind)
                                for c in range(cols)])
         z = mg.precondition_device("k", torch_gpu.as_tensor(r.T.copy(), device="cuda")).cpu().numpy().T
         tol = 1e-9 if precision == "float64" else 1e-4

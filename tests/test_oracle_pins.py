@@ -63,6 +63,26 @@ def test_chebyshev_error_is_the_scaled_chebyshev_polynomial(count):
         assert np.linalg.norm((e - x) - factor * 0.7 * e) < 1e-10 * np.linalg.norm(e)
 
 
+@pytest.mark.parametrize("count", [1, 2, 3, 6])
+def test_fourth_kind_chebyshev_error_is_the_closed_form(count):
+    """The fourth-kind smoother's error polynomial is W_k(1 - 2t) / (2k + 1), t = lam / upper, which for
+    t in (0, 1] equals sin((2k + 1) asin(sqrt t)) / ((2k + 1) sqrt t); evaluated here from the sine, not
+    from the three-term recurrence the smoother runs."""
+    level = small_level(2)
+    live, lam, vec = live_eigenpairs(level)
+    lmax = lam.max() * 0.97
+    upper = 1.1 * lmax
+    for i in (0, len(lam) // 2, len(lam) - 1):
+        e = np.zeros(level["A"].shape[0])
+        e[live] = vec[:, i]
+        t = lam[i] / upper
+        factor = np.sin((2 * count + 1) * np.arcsin(np.sqrt(t))) / ((2 * count + 1) * np.sqrt(t))
+        x = gmg.chebyshev(level, level["A"] @ e, None, count, lmax, kind=4)
+        assert np.linalg.norm((e - x) - factor * e) < 1e-10 * np.linalg.norm(e), (i, factor)
+        x = gmg.chebyshev(level, level["A"] @ e, 0.3 * e, count, lmax, kind=4)
+        assert np.linalg.norm((e - x) - factor * 0.7 * e) < 1e-10 * np.linalg.norm(e)
+
+
 @pytest.mark.parametrize("count,damping", [(1, 1.0), (2, 1.0), (4, 0.7)])
 def test_jacobi_error_factor(count, damping):
     level = small_level(1)
