@@ -668,40 +668,86 @@ __global__ void k_gather_live(const T* __restrict__ v, VLay lay, const int* __re
       (active && !active[r]) ? 0.0 : (double)v[(d / 3) * lay.ns + (long long)r * lay.rs + d % 3];
 }
 
-// Coarsest solve out = in A^-1 as a split-K GEMM: block (64 columns j) x (up to 64 right-hand sides)
-// x (one slice of the i range); A^-1 is read once per right-hand-side tile, and the slices are
-// summed in a fixed order afterwards.  Thread (j, q) accumulates right-hand sides q, q + 4, ...
-constexpr int DJ = 64, DI = 32, DR = 64, DSPLIT = 16;
-__global__ __launch_bounds__(256) void k_dense_gemm(const double* __restrict__ in, const double* __restrict__ ainv,
-                                                    int n, int k, int per, double* __restrict__ partial) {
-  __shared__ double tile[DR][DI + 1];
-  const int tj = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int j = blockIdx.x * DJ + tj, r0 = blockIdx.y * DR, split = blockIdx.z;
-  const int ib = split * per, ie = min(n, ib + per);
-  double acc[DR / 4];
+// Coarsest solve out = in A^-1 (A^-1 symmetric, row-major n x n) on the FP64 matrix cores, split over the i
+// range: each wavefront owns TJ 16-column tiles of A^-1 and accumulates the 16 x 16 output tiles of up to RT
+// 16-row blocks of right-hand sides with v_mfma_f64_16x16x4f64 (one read of the right-hand sides serves TJ
+// tiles).  In a 16-row step lane l (c = l % 16, q = l / 16) reads in[r][i0 + 4q + s] (r the lane's right-hand
+// side c of the block) and A^-1[i0 + 4q + s][j] (j the lane's column c of the tile) and feeds product s with
+// them, so both operands of a product meet at the same i; the lane then holds C[q + 4v][c], v = 0..3.  The next
+// step's operands are loaded before the current step's products.  The slices are summed in a fixed order
+// afterwards, so results are deterministic.
+constexpr int DSPLIT = 32, DTJ = 2;
+typedef double dense_d4 __attribute__((ext_vector_type(4)));
+
+template <int RT>
+__global__ __launch_bounds__(256) void k_dense_mfma(const double* __restrict__ in, const double* __restrict__ ainv,
+                                                    int n, int k, int r0, int per, double* __restrict__ partial) {
+  constexpr int TJ = DTJ;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, c = l & 15, q = l >> 4;
+  const int split = blockIdx.y;
+  const int nb = (n + 15) / 16;
+  const int b0 = split * per, b1 = min(nb, b0 + per);
+  int j[TJ];
+  bool jin[TJ];
 #pragma unroll
-  for (int m = 0; m < DR / 4; ++m) acc[m] = 0.0;
-  for (int i0 = ib; i0 < ie; i0 += DI) {
-    for (int v = threadIdx.x; v < DR * DI; v += 256) {
-      const int rr = v / DI, ii = v % DI;
-      tile[rr][ii] = (r0 + rr < k && i0 + ii < ie) ? in[(long long)(r0 + rr) * n + i0 + ii] : 0.0;
-    }
-    __syncthreads();
-    if (j < n) {
-      const int lim = min(DI, ie - i0);
-      for (int ii = 0; ii < lim; ++ii) {
-        const double a = ainv[(long long)(i0 + ii) * n + j];
-#pragma unroll
-        for (int m = 0; m < DR / 4; ++m) acc[m] = fma(tile[q + 4 * m][ii], a, acc[m]);
-      }
-    }
-    __syncthreads();
+  for (int u = 0; u < TJ; ++u) {
+    j[u] = ((blockIdx.x * 4 + wave) * TJ + u) * 16 + c;
+    jin[u] = j[u] < n;
   }
-  if (j >= n) return;
+  const double* arow[RT];
+  bool rin[RT];
 #pragma unroll
-  for (int m = 0; m < DR / 4; ++m) {
-    const int r = r0 + q + 4 * m;
-    if (r < k) partial[((long long)split * k + r) * n + j] = acc[m];
+  for (int t = 0; t < RT; ++t) {
+    const int r = r0 + 16 * t + c;
+    rin[t] = r < k;
+    arow[t] = in + (long long)(rin[t] ? r : 0) * n;
+  }
+  dense_d4 acc[RT][TJ];
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int u = 0; u < TJ; ++u) acc[t][u] = dense_d4{0.0, 0.0, 0.0, 0.0};
+  double av[RT][4], bv[TJ][4];
+  auto load = [&](int blk) {
+    const int i = blk * 16 + 4 * q;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const bool ok = i + s < n;
+#pragma unroll
+      for (int u = 0; u < TJ; ++u) bv[u][s] = ok && jin[u] ? ainv[(long long)(i + s) * n + j[u]] : 0.0;
+#pragma unroll
+      for (int t = 0; t < RT; ++t) av[t][s] = ok && rin[t] ? arow[t][i + s] : 0.0;
+    }
+  };
+  if (b0 < b1) load(b0);
+  for (int blk = b0; blk < b1; ++blk) {
+    double ca[RT][4], cb[TJ][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int u = 0; u < TJ; ++u) cb[u][s] = bv[u][s];
+#pragma unroll
+      for (int t = 0; t < RT; ++t) ca[t][s] = av[t][s];
+    }
+    if (blk + 1 < b1) load(blk + 1);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int u = 0; u < TJ; ++u)
+          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[t][s], cb[u][s], acc[t][u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < TJ; ++u) {
+    if (!jin[u]) continue;
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = r0 + 16 * t + q + 4 * v;
+        if (r < k) partial[((long long)split * k + r) * n + j[u]] = acc[t][u][v];
+      }
   }
 }
 
@@ -732,10 +778,18 @@ void coarsest_solve(const Geom& g, const double* ainv, const int* live_dofs, int
   if (n == 0) return;
   const unsigned b = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(k_gather_live<T>, dim3(b, k), dim3(256), 0, s, rhs, lay, live_dofs, n, work_in, active);
-  const int per = ((n + DSPLIT - 1) / DSPLIT + DI - 1) / DI * DI;
-  const int splits = (n + per - 1) / per;
-  hipLaunchKernelGGL(k_dense_gemm, dim3((n + DJ - 1) / DJ, (k + DR - 1) / DR, splits), dim3(256), 0, s, work_in,
-                     ainv, n, k, per, work_split);
+  const int nb = (n + 15) / 16;
+  const int per = (nb + DSPLIT - 1) / DSPLIT;
+  const int splits = (nb + per - 1) / per;
+  const dim3 grid((unsigned)((nb + 4 * DTJ - 1) / (4 * DTJ)), (unsigned)splits);
+  for (int r0 = 0; r0 < k; r0 += 64) {  // up to four 16-row blocks of right-hand sides per launch
+    switch (std::min(4, (k - r0 + 15) / 16)) {
+      case 1: hipLaunchKernelGGL(k_dense_mfma<1>, grid, dim3(256), 0, s, work_in, ainv, n, k, r0, per, work_split); break;
+      case 2: hipLaunchKernelGGL(k_dense_mfma<2>, grid, dim3(256), 0, s, work_in, ainv, n, k, r0, per, work_split); break;
+      case 3: hipLaunchKernelGGL(k_dense_mfma<3>, grid, dim3(256), 0, s, work_in, ainv, n, k, r0, per, work_split); break;
+      default: hipLaunchKernelGGL(k_dense_mfma<4>, grid, dim3(256), 0, s, work_in, ainv, n, k, r0, per, work_split);
+    }
+  }
   hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)(((long long)k * n + 255) / 256)), dim3(256), 0, s, work_split,
                      splits, (long long)k * n, work_out);
   hipLaunchKernelGGL(k_scatter_live<T>, dim3(b, k), dim3(256), 0, s, work_out, lay, live_dofs, n, out, active);
