@@ -67,6 +67,8 @@ SIGNATURES = {
     "dfmi_block_combine": ([c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int64, c_void_p], c_int32),
     "dfmi_block_update": ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p,
                            c_void_p], c_int32),
+    "dfmi_block_combine2": ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int64,
+                             c_void_p], c_int32),
     "dfmi_gram_static": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p], c_int32),
     "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_project": ([c_void_p, c_void_p, c_int64, c_double, POINTER(c_double), c_int32, c_int32, c_void_p, c_void_p,

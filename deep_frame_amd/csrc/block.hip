@@ -137,16 +137,17 @@ __global__ __launch_bounds__(256) void k_bgram(const float* __restrict__ a, int 
 }
 
 constexpr int BK_MAXK = 48;
-constexpr int BK_OG = 16;  // output columns per wave of the combination
 
-// Y[j] = X[j] + sgn sum_i U[i] M[i][j] for j < kout (M FP32, row i at m + 48 i, zero past kout).  NORM: also
+// Y[j] = X[j] + sgn sum_i U[i] M[i][j] for j < kout (M FP32, row i at m + 48 i, zero past kout), for one or two
+// output sets (SETS) that share U: set 0 is (x0, y0, m0), set 1 is (x1, y1, m1).  NORM (one set): also
 // per-workgroup partials of |Y[j]|^2.  A workgroup of NW waves takes chunks of 256 consecutive rows: it stages
 // the chunk of every input column in LDS (one 16-byte buffer load per lane and column, all issued before the
-// first use), then wave w forms output columns 16 w .. 16 w + 15 for four rows per lane, reading U from LDS
-// and the wave-uniform coefficients through scalar loads.  Every HBM access is 1 KB contiguous per wave and
-// each input entry is read from HBM once.  Rows at or past n read zeros and their stores are dropped by the
-// buffer's range check; the last chunk, which may end inside a lane's four rows, takes single accesses.  Y may
-// alias X: an entry of X is read only by the lane that writes it.  U must not alias Y.
+// first use), then wave w forms OG output columns of set w / (NW / SETS) for four rows per lane, reading U from
+// LDS and the wave-uniform coefficients through scalar loads.  Every HBM access is 1 KB contiguous per wave and
+// each input entry is read from HBM once.  Only the last chunk of the grid can be partial; it is peeled off the
+// loop and takes single accesses (rows at or past n read zeros, their stores are dropped by the buffer's range
+// check), so the loop itself carries no tests around its loads.  Y may alias X of its set: an entry of X is
+// read only by the lane that writes it.  U must not alias an output.
 __device__ __forceinline__ void bk_load4(__amdgpu_buffer_rsrc_t r, unsigned off, bool whole, float (&v)[4]) {
   if (whole) {
     const auto w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
@@ -168,28 +169,32 @@ __device__ __forceinline__ void bk_store4(__amdgpu_buffer_rsrc_t r, unsigned off
   }
 }
 
-template <int NW, bool NORM>
-__global__ __launch_bounds__(64 * NW) void k_bcombine(const float* __restrict__ u, const float* x, float* y,
-                                                       const float* __restrict__ m, float sgn, int kin, int kout,
+template <bool B>
+struct bk_tag {
+  static constexpr bool value = B;
+};
+
+template <int NW, int OG, int SETS, bool NORM>
+__global__ __launch_bounds__(64 * NW) void k_bcombine(const float* __restrict__ u, const float* x0, float* y0,
+                                                       const float* __restrict__ m0, const float* x1, float* y1,
+                                                       const float* __restrict__ m1, float sgn, int kin, int kout,
                                                        long long n, double* __restrict__ partial) {
+  static_assert(!NORM || SETS == 1, "column norms are formed for one output set");
+  constexpr int WPS = NW / SETS;  // waves per output set
   __shared__ __align__(16) float Us[BK_MAXK][256];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
-  const int j0 = wave * BK_OG;
-  const int jn = min(BK_OG, kout - j0);  // wave-uniform, >= 1
+  const int set = wave / WPS, j0 = (wave - set * WPS) * OG;
+  const int jn = min(OG, kout - j0);  // wave-uniform; waves past kout only stage
+  const float* x = set ? x1 : x0;
+  float* y = set ? y1 : y0;
+  const float* m = set ? m1 : m0;
   const long long cbytes = n * (long long)sizeof(float);
-  float ss[NORM ? BK_OG : 1];
+  float ss[NORM ? OG : 1];
 #pragma unroll
-  for (int o = 0; o < (NORM ? BK_OG : 1); ++o) ss[o] = 0.f;
-  // each workgroup sweeps one contiguous range of chunks, so that its accesses stay within a few pages of every
-  // column (a grid-stride order would move every column to a new page at every step)
-  const long long chunks = (n + 255) / 256, per = (chunks + gridDim.x - 1) / gridDim.x;
-  const long long c0 = (long long)blockIdx.x * per, c1 = min(chunks, c0 + per);
-  for (long long r0 = c0 * 256; r0 < c1 * 256; r0 += 256) {
-    const long long rl = r0 + 4LL * l;  // this lane's first row
-    // wave-uniform: only the last chunk of the grid is partial (a per-lane test would put every access of the
-    // common case under divergent control flow, which serialises the loads)
-    const bool whole = r0 + 256 <= n;
-    const unsigned off = (unsigned)(rl * (long long)sizeof(float));
+  for (int o = 0; o < (NORM ? OG : 1); ++o) ss[o] = 0.f;
+  auto chunk = [&](long long r0, auto whole_tag) {
+    constexpr bool whole = decltype(whole_tag)::value;
+    const unsigned off = (unsigned)((r0 + 4LL * l) * (long long)sizeof(float));
     {  // stage the chunk of every input column: all loads first, then the LDS stores
       constexpr int PER = (BK_MAXK + NW - 1) / NW;
       float v[PER][4];
@@ -205,27 +210,30 @@ __global__ __launch_bounds__(64 * NW) void k_bcombine(const float* __restrict__ 
           *reinterpret_cast<float4*>(&Us[i][4 * l]) = make_float4(sgn * v[q][0], sgn * v[q][1], sgn * v[q][2], sgn * v[q][3]);
       }
     }
-    float acc[BK_OG][4];  // the sums start from X, read before any store of this lane
+    float acc[OG][4];  // the sums start from X, read before any store of this lane
 #pragma unroll
-    for (int o = 0; o < BK_OG; ++o) bk_load4(buffer(x + (long long)min(j0 + o, kout - 1) * n, cbytes), off, whole, acc[o]);
+    for (int o = 0; o < OG; ++o)
+      bk_load4(buffer(x + (long long)max(0, min(j0 + o, kout - 1)) * n, cbytes), off, whole, acc[o]);
     __syncthreads();
+    if (jn > 0) {
 #pragma unroll 2
-    for (int i = 0; i < kin; ++i) {
-      const float4 uv = *reinterpret_cast<const float4*>(&Us[i][4 * l]);
-      const float* mi = m + (long long)i * BK_MAXK + j0;
+      for (int i = 0; i < kin; ++i) {
+        const float4 uv = *reinterpret_cast<const float4*>(&Us[i][4 * l]);
+        const float* mi = m + (long long)i * BK_MAXK + j0;
 #pragma unroll
-      for (int o = 0; o < BK_OG; ++o) {
-        const float c = mi[o];
-        acc[o][0] = fmaf(c, uv.x, acc[o][0]);
-        acc[o][1] = fmaf(c, uv.y, acc[o][1]);
-        acc[o][2] = fmaf(c, uv.z, acc[o][2]);
-        acc[o][3] = fmaf(c, uv.w, acc[o][3]);
+        for (int o = 0; o < OG; ++o) {
+          const float c = mi[o];
+          acc[o][0] = fmaf(c, uv.x, acc[o][0]);
+          acc[o][1] = fmaf(c, uv.y, acc[o][1]);
+          acc[o][2] = fmaf(c, uv.z, acc[o][2]);
+          acc[o][3] = fmaf(c, uv.w, acc[o][3]);
+        }
       }
     }
     __syncthreads();  // the next chunk overwrites Us
 #pragma unroll
-    for (int o = 0; o < BK_OG; ++o) {  // fully unrolled with a wave-uniform guard (a break keeps a loop that
-                                       // indexes acc dynamically)
+    for (int o = 0; o < OG; ++o) {  // fully unrolled with a wave-uniform guard (a break keeps a loop that
+                                    // indexes acc dynamically)
       if (o < jn) {
         bk_store4(buffer(y + (long long)(j0 + o) * n, cbytes), off, whole, acc[o]);
         if (NORM) {  // rows past n were read as zeros
@@ -234,10 +242,17 @@ __global__ __launch_bounds__(64 * NW) void k_bcombine(const float* __restrict__ 
         }
       }
     }
-  }
+  };
+  // each workgroup sweeps one contiguous range of chunks, so that its accesses stay within a few pages of every
+  // column (a grid-stride order would move every column to a new page at every step)
+  const long long chunks = (n + 255) / 256, per = (chunks + gridDim.x - 1) / gridDim.x;
+  const long long c0 = (long long)blockIdx.x * per, c1 = min(chunks, c0 + per);
+  const long long cw = min(c1, n / 256);  // chunks of this range that are whole
+  for (long long c = c0; c < cw; ++c) chunk(c * 256, bk_tag<true>{});
+  if (c0 < c1 && cw < c1) chunk(cw * 256, bk_tag<false>{});  // the partial last chunk (empty ranges take none)
   if (NORM) {  // each wave owns its columns: lanes are summed in FP64, one partial per column and workgroup
 #pragma unroll
-    for (int o = 0; o < BK_OG; ++o) {
+    for (int o = 0; o < OG; ++o) {
       double v = (double)ss[o];
       for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
       if (l == 0 && o < jn) partial[(long long)(j0 + o) * gridDim.x + blockIdx.x] = v;
@@ -303,10 +318,19 @@ static void launch_bcombine(const float* u, const float* x, float* y, const floa
   if ((long long)n * (long long)sizeof(float) > (long long)INT32_MAX)
     throw std::invalid_argument("block combination: columns above 2 GiB");
   const dim3 grid(block_combine_blocks(n));
-  switch ((kout + BK_OG - 1) / BK_OG) {
-    case 1: hipLaunchKernelGGL((k_bcombine<1, NORM>), grid, dim3(64), 0, s, u, x, y, m, sgn, kin, kout, n, partial); break;
-    case 2: hipLaunchKernelGGL((k_bcombine<2, NORM>), grid, dim3(128), 0, s, u, x, y, m, sgn, kin, kout, n, partial); break;
-    default: hipLaunchKernelGGL((k_bcombine<3, NORM>), grid, dim3(192), 0, s, u, x, y, m, sgn, kin, kout, n, partial); break;
+  switch ((kout + 15) / 16) {  // 16 output columns per wave
+    case 1:
+      hipLaunchKernelGGL((k_bcombine<1, 16, 1, NORM>), grid, dim3(64), 0, s, u, x, y, m, x, y, m, sgn, kin, kout, n,
+                         partial);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_bcombine<2, 16, 1, NORM>), grid, dim3(128), 0, s, u, x, y, m, x, y, m, sgn, kin, kout, n,
+                         partial);
+      break;
+    default:
+      hipLaunchKernelGGL((k_bcombine<3, 16, 1, NORM>), grid, dim3(192), 0, s, u, x, y, m, x, y, m, sgn, kin, kout, n,
+                         partial);
+      break;
   }
   DFMI_HIP(hipGetLastError());
 }
@@ -318,12 +342,44 @@ void block_combine(const float* u, const float* x, float* y, const float* m, int
   launch_bcombine<false>(u, x, y, m, 1.f, kin, kout, n, nullptr, s);
 }
 
+// Y0 = X0 + U M0 and Y1 = X1 + U M1 in one pass over U (k columns in and out; M row stride 48): the next search
+// block and the deferred solution update of the block recurrence.  8 output columns per wave, two sets of waves.
+void block_combine2(const float* u, const float* x0, float* y0, const float* m0, const float* x1, float* y1,
+                    const float* m1, int k, long long n, hipStream_t s) {
+  if (n < 1 || k < 1 || k > BK_MAXK) throw std::invalid_argument("block_combine2: need n >= 1 and 1 <= k <= 48");
+  if ((long long)n * (long long)sizeof(float) > (long long)INT32_MAX)
+    throw std::invalid_argument("block combination: columns above 2 GiB");
+  const dim3 grid(block_combine_blocks(n));
+  switch ((k + 15) / 16) {  // waves per set: 2, 4 or 6 of 8 columns
+    case 1:
+      hipLaunchKernelGGL((k_bcombine<4, 8, 2, false>), grid, dim3(256), 0, s, u, x0, y0, m0, x1, y1, m1, 1.f, k, k, n,
+                         nullptr);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_bcombine<8, 8, 2, false>), grid, dim3(512), 0, s, u, x0, y0, m0, x1, y1, m1, 1.f, k, k, n,
+                         nullptr);
+      break;
+    default:
+      hipLaunchKernelGGL((k_bcombine<12, 8, 2, false>), grid, dim3(768), 0, s, u, x0, y0, m0, x1, y1, m1, 1.f, k, k,
+                         n, nullptr);
+      break;
+  }
+  DFMI_HIP(hipGetLastError());
+}
+
 // D += V M and R -= W M (k columns, M row stride 48), rr[j] = |R_j|^2 after the update; work holds
 // k * block_combine_blocks(n) doubles.
 void block_update(const float* v, float* d, const float* w, float* r, const float* m, int k, long long n, double* rr,
                   double* work, hipStream_t s) {
   if (n < 1 || k < 1 || k > BK_MAXK) throw std::invalid_argument("block_update: need n >= 1 and 1 <= k <= 48");
   launch_bcombine<false>(v, d, d, m, 1.f, k, k, n, nullptr, s);
+  block_residual_update(w, r, m, k, n, rr, work, s);
+}
+
+// R -= W M alone, with rr[j] = |R_j|^2 (the solution update deferred to the next combination pass)
+void block_residual_update(const float* w, float* r, const float* m, int k, long long n, double* rr, double* work,
+                           hipStream_t s) {
+  if (n < 1 || k < 1 || k > BK_MAXK) throw std::invalid_argument("block_update: need n >= 1 and 1 <= k <= 48");
   launch_bcombine<true>(w, r, r, m, -1.f, k, k, n, work, s);
   reduce_partials(work, block_combine_blocks(n), k, rr, s);
 }
@@ -376,6 +432,18 @@ int dfmi_block_combine(const float* u_dev, const float* x_dev, float* y_dev, con
     if (!u_dev || !x_dev || !y_dev || !m_dev) throw std::invalid_argument("block_combine: null operand");
     if (u_dev == y_dev) throw std::invalid_argument("block_combine: y must not alias u");
     dfmi::block_combine(u_dev, x_dev, y_dev, m_dev, kin, kout, (long long)n, (hipStream_t)hip_stream);
+  })
+}
+
+int dfmi_block_combine2(const float* u_dev, const float* x0_dev, float* y0_dev, const float* m0_dev,
+                        const float* x1_dev, float* y1_dev, const float* m1_dev, int32_t k, int64_t n,
+                        void* hip_stream) {
+  DFMI_BLOCK_GUARD({
+    if (!u_dev || !x0_dev || !y0_dev || !m0_dev || !x1_dev || !y1_dev || !m1_dev)
+      throw std::invalid_argument("block_combine2: null operand");
+    if (u_dev == y0_dev || u_dev == y1_dev) throw std::invalid_argument("block_combine2: y must not alias u");
+    dfmi::block_combine2(u_dev, x0_dev, y0_dev, m0_dev, x1_dev, y1_dev, m1_dev, k, (long long)n,
+                         (hipStream_t)hip_stream);
   })
 }
 

@@ -282,6 +282,12 @@ void block_combine(const float* u, const float* x, float* y, const float* m, int
                    hipStream_t s);
 void block_update(const float* v, float* d, const float* w, float* r, const float* m, int k, long long n, double* rr,
                   double* work, hipStream_t s);
+// R -= W M with rr[j] = |R_j|^2 (work: k * block_combine_blocks(n) doubles)
+void block_residual_update(const float* w, float* r, const float* m, int k, long long n, double* rr, double* work,
+                           hipStream_t s);
+// Y0 = X0 + U M0 and Y1 = X1 + U M1 in one pass over U (k <= 48 columns in and out)
+void block_combine2(const float* u, const float* x0, float* y0, const float* m0, const float* x1, float* y1,
+                    const float* m1, int k, long long n, hipStream_t s);
 void block_small_product(const double* S, const double* H, int k, double scale, float* out, hipStream_t s);
 // y[m] (+)= sum_r m[r][m] u[r] for k (<= 64) rows u of length n and an mcols-column device matrix m
 void combine_device(const double* u, long long n, int k, const double* m, int mcols, double* y, int accumulate,

@@ -1612,7 +1612,9 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
       }
     DFMI_HIP(hipMemcpyAsync(amat, af.data(), sizeof(float) * k * BLOCK_MAX_K, hipMemcpyHostToDevice, s));
     DFMI_HIP(hipMemcpyAsync(smat, S.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, s));
-    block_update(vcur, d32, w32, r32, amat, k, n, rr, work, s);  // D += V a, R -= W a, |R_j|^2
+    // R -= W a, |R_j|^2; D += V a waits for the pass that forms the next block (it reads V as well), unless a
+    // column is flushed below
+    block_residual_update(w32, r32, amat, k, n, rr, work, s);
     DFMI_HIP(hipMemcpyAsync(rrh.data(), rr, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     sync(h);
     ++step;
@@ -1626,7 +1628,10 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
       if (est * sh[c] < tol * nb[c] || est < delta) upd[c] = 1, any_upd = true;
     }
     h->history.push_back(worst);
+    bool pending = true;  // D += V a not yet applied
     if (any_upd) {  // flush D into x, true FP64 residual, restart the column's scaled residual at norm 1
+      block_combine(vcur, d32, d32, amat, k, k, n, s);
+      pending = false;
       upload(act, upd);
       add_scaled(x, d32, scale, n, k, act, s);
       for (int c = 0; c < k; ++c) fh[c] = 0.0;
@@ -1658,11 +1663,14 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
       upload(act, active);
     }
     if (!count(active)) break;
-    // Z = M R; V' = Z - V S W^T Z
+    // Z = M R; V' = Z - V S W^T Z, with the deferred D += V a in the same pass over V
     const float* z = precondition_block();
     block_gram(w32, k, z, k, nullptr, 0, n, hmat, work, s);
     block_small_product(smat, hmat, k, -1.0, bmat, s);
-    block_combine(vcur, z, vnext, bmat, k, k, n, s);
+    if (pending)
+      block_combine2(vcur, z, vnext, bmat, d32, d32, amat, k, n, s);
+    else
+      block_combine(vcur, z, vnext, bmat, k, k, n, s);
     std::swap(vcur, vnext);
   }
   if (count(active)) {  // iteration limit: flush and report the true residual

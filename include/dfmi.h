@@ -241,7 +241,9 @@ int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev,
  *   dfmi_block_gram: out_dev (ka x (kb1 + kb2), FP64 row-major) = A^T [B1 | B2]; b2_dev may be NULL with kb2 = 0,
  *     otherwise ceil(ka / 16), ceil(kb1 / 16) and ceil(kb2 / 16) must be equal (and so must the first two);
  *   dfmi_block_combine: Y = X + U M for kin input and kout output columns (y may alias x, not u);
- *   dfmi_block_update: D += V M, R -= W M (k columns) and rr_dev[j] = |R_j|^2 (FP64) after the update. */
+ *   dfmi_block_update: D += V M, R -= W M (k columns) and rr_dev[j] = |R_j|^2 (FP64) after the update;
+ *   dfmi_block_combine2: Y0 = X0 + U M0 and Y1 = X1 + U M1 (k columns each) in one pass over U (each y may alias
+ *     its x, neither may alias u). */
 int dfmi_block_work_doubles(int64_t n, int32_t ka, int32_t kb, int64_t* count);
 int dfmi_block_gram(const float* a_dev, int32_t ka, const float* b1_dev, int32_t kb1, const float* b2_dev,
                     int32_t kb2, int64_t n, double* out_dev, double* work_dev, void* hip_stream);
@@ -249,6 +251,9 @@ int dfmi_block_combine(const float* u_dev, const float* x_dev, float* y_dev, con
                        int32_t kout, int64_t n, void* hip_stream);
 int dfmi_block_update(const float* v_dev, float* d_dev, const float* w_dev, float* r_dev, const float* m_dev,
                       int32_t k, int64_t n, double* rr_dev, double* work_dev, void* hip_stream);
+int dfmi_block_combine2(const float* u_dev, const float* x0_dev, float* y0_dev, const float* m0_dev,
+                        const float* x1_dev, float* y1_dev, const float* m1_dev, int32_t k, int64_t n,
+                        void* hip_stream);
 
 /* SIMP interpolation E = E0 (eps + (1 - eps) rho^p), dE — topology_optimization.py:657-661;
  * inactive cells (active_dev[e] == 0) get 0. */

@@ -11,7 +11,7 @@ HEADER = Path(__file__).resolve().parents[1] / "include" / "dfmi.h"
 
 def declared():
     text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"\b(dfmi_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(dfmi_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -107,3 +107,36 @@ def test_block_update_matches_numpy(torch_gpu, k, n):
     assert np.max(np.abs(got_r - want_r) / sr) < 1e-6
     # squares summed in FP32 per lane, FP64 across lanes and workgroups
     np.testing.assert_allclose(rr.cpu().numpy(), np.sum(got_r.astype(np.float64) ** 2, axis=1), rtol=1e-6)
+
+
+@pytest.mark.parametrize("k,n", [(42, 100003), (20, 4097), (7, 300), (48, 65536), (1, 5)])
+def test_block_combine2_matches_numpy(torch_gpu, k, n):
+    """The fused pass of the recurrence: Y0 = X0 + U M0 out of place (the next search block) and Y1 = X1 + U M1 in
+    place (the deferred solution update), against NumPy in FP64; U, X0 and X1 are left unchanged where not
+    written."""
+    from deep_frame_amd import _native
+    torch = torch_gpu
+    rng = np.random.default_rng(100 + k)
+    u, x0, x1 = (rng.normal(size=(k, n)).astype(np.float32) for _ in range(3))
+    m0, m1 = ((0.2 * rng.normal(size=(k, k))).astype(np.float32) for _ in range(2))
+    tu, tx0, tx1 = (torch.as_tensor(a, device="cuda") for a in (u, x0, x1))
+    ty0 = torch.full((k, n), np.nan, dtype=torch.float32, device="cuda")
+    tm0, tm1 = _coef(torch, m0), _coef(torch, m1)  # both alive across the call
+    _native.call("dfmi_block_combine2", _native.ptr(tu), _native.ptr(tx0), _native.ptr(ty0), _native.ptr(tm0),
+                 _native.ptr(tx1), _native.ptr(tx1), _native.ptr(tm1), k, n, _stream(torch))
+    u64 = u.astype(np.float64)
+    for got, x, m in ((ty0, x0, m0), (tx1, x1, m1)):
+        want = x + m.astype(np.float64).T @ u64
+        scale = np.abs(x) + np.abs(m.astype(np.float64)).T @ np.abs(u64)
+        assert np.max(np.abs(got.cpu().numpy() - want) / scale) < 1e-6
+    assert np.array_equal(tu.cpu().numpy(), u) and np.array_equal(tx0.cpu().numpy(), x0)
+
+
+def test_block_combine2_rejects_aliased_input(torch_gpu):
+    from deep_frame_amd import _native
+    torch = torch_gpu
+    t = torch.zeros((4, 64), dtype=torch.float32, device="cuda")
+    m = _coef(torch, np.zeros((4, 4), np.float32))
+    with pytest.raises(RuntimeError, match="alias"):
+        _native.call("dfmi_block_combine2", _native.ptr(t), _native.ptr(t), _native.ptr(t), _native.ptr(m), _native.ptr(t),
+                     _native.ptr(t), _native.ptr(m), 4, 64, _stream(torch))
