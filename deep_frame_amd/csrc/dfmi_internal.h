@@ -92,6 +92,8 @@ enum ApplyMode : int {
   MODE_CHEB1 = 6,   // fine only: Chebyshev step whose previous iterate is x1 = c0 Dinv b (a.xp unused)
   MODE_MASS = 7,    // fine only: y = M (sum_e m_e Me + diag(lumped)) M x with the element field m_e in place of
                     // the moduli, Me from a.mdiag (Hadamard-diagonal form) and lumped in a.dinv (may be null)
+  MODE_RESID_WORK = 8,  // fine FP64 only: r = M (b - A M x) written in FP32 as r * beta[c] to a.pout (a.y unused),
+                        // with per-workgroup partial sums of |r|^2 (FP64) in a.partial
 };
 
 // Three consecutive entries of a vector through a raw buffer load: byte offsets at or beyond the buffer's
@@ -140,8 +142,10 @@ struct ApplyArgs {
   double c1, c2;      // Chebyshev coefficients, CHEB only
   double c0;          // first Chebyshev iterate x1 = c0 Dinv b, CHEB0 / CHEB1 only
   double* partial;    // [k][nblocks] partial dot products: x.y for DOT; b.y for FP32 fine CHEB when not null
-  const double* beta; // FP32 fine DOT with pout: per-column beta of the fused direction (nullptr: p = b)
-  void* pout;         // FP32 fine DOT: when not null the operand is p = b + beta x, written here for owned nodes
+  const double* beta; // FP32 fine DOT with pout: per-column beta of the fused direction (nullptr: p = b);
+                      // FP64 fine RESID_WORK: per-column factor of the FP32 residual
+  void* pout;         // FP32 fine DOT: when not null the operand is p = b + beta x, written here for owned nodes;
+                      // FP64 fine RESID_WORK: the FP32 residual (same column stride as x)
   const int* active;  // per-column flag (device), nullptr = all active
   int k;
   double mdiag[8];    // MASS only: unit-density element mass in the corner-Hadamard basis, divided by 64

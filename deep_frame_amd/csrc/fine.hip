@@ -153,14 +153,17 @@ struct FineCfg {
 // so the staging loads skip the mask and read through 32-bit buffer offsets; slots outside the grid read zeros.
 // KMODE: an ApplyMode, or kChebPartial for the FP32 Chebyshev step that also sums b.y per workgroup (a separate
 // instantiation, so that the other Chebyshev steps carry no dot-product work).
-constexpr int kChebPartial = 100;
+// kResidWork: the FP64 residual written in FP32 (times a per-column factor) with partial sums of |r|^2, the
+// restart of the FP32 recurrence after a reliable update.
+constexpr int kChebPartial = 100, kResidWork = 102;
 template <typename T, int KMODE, bool PRE>
 __global__ __launch_bounds__(FineCfg<T>::NT)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE != MODE_MASS ? 6 : 1))) void k_fine_apply(
     Geom g, const T* __restrict__ moduli, const uint8_t* __restrict__ mask, KeSym<T> ks, ApplyArgs a, int TY, int TZ,
     int xchunk, int tiles_y, int tiles_z) {
-  constexpr int MODE = KMODE == kChebPartial ? MODE_CHEB : KMODE;
+  constexpr int MODE = KMODE == kChebPartial ? MODE_CHEB : (KMODE == kResidWork ? MODE_RESID : KMODE);
   constexpr bool kChebDot = KMODE == kChebPartial;
+  constexpr bool kWork = KMODE == kResidWork;
   constexpr int NT = FineCfg<T>::NT;
   constexpr int MAXNP = FineCfg<T>::MAXNP;
   // the FP32 PCG's fusions (direction formed inside DOT, r.z partials from the last CHEB step)
@@ -224,6 +227,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE != MODE_MASS ? 6 : 1)
   const __amdgpu_buffer_rsrc_t yr = buffer(Y, col_b);
   const __amdgpu_buffer_rsrc_t pr = buffer(a.pout ? reinterpret_cast<const T*>(a.pout) + (long long)r * col : X, col_b);
   const __amdgpu_buffer_rsrc_t mr = buffer(mask, g.nodes());
+  const __amdgpu_buffer_rsrc_t wr =
+      buffer(kWork ? reinterpret_cast<const float*>(a.pout) + (long long)r * col : reinterpret_cast<const float*>(X),
+             col * (long long)sizeof(float));
   auto fetch_pre = [&](int ip) {
     const bool in = ip >= 0 && ip < g.nx;  // uniform
 #pragma unroll
@@ -474,8 +480,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE != MODE_MASS ? 6 : 1)
         for (int c = 0; c < 3; ++c) {
           if (MODE == MODE_DOT) dot += (double)xs[c] * (double)yv[c];
           if (kChebDot) dot += (double)bv[c] * (double)yv[c];
+          if (kWork) dot += (double)yv[c] * (double)yv[c];
         }
-        if constexpr (PRE) {
+        if constexpr (kWork) {  // the FP32 residual only (PRE: offsets in floats, dropped without a node)
+          const T f = (T)a.beta[r];
+          store3(wr, out ? (unsigned)nn * 3u * (unsigned)sizeof(float) : kOutside, (float)(f * yv[0]),
+                 (float)(f * yv[1]), (float)(f * yv[2]));
+        } else if constexpr (PRE) {
           store3(yr, eoff, yv[0], yv[1], yv[2]);
           if (MODE == MODE_DOT && kFusedPcg && a.pout) store3(pr, eoff, xs[0], xs[1], xs[2]);
         } else {
@@ -491,7 +502,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KMODE != MODE_MASS ? 6 : 1)
     e_cur = e_next;
     (void)mk_next;
   }
-  if (MODE == MODE_DOT || kChebDot) {  // partials of x.y (DOT), b.y (CHEB)
+  if (MODE == MODE_DOT || kChebDot || kWork) {  // partials of x.y (DOT), b.y (CHEB), r.r (RESID_WORK)
     for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
     if ((t & 63) == 0) red[t >> 6] = dot;
     __syncthreads();
@@ -608,6 +619,16 @@ void fine_apply(const Geom& g, const T* moduli, const uint8_t* mask, const KeBlo
     case MODE_CHEB1: DFMI_LAUNCH(MODE_CHEB1); break;
     case MODE_DOT: DFMI_LAUNCH(MODE_DOT); break;
     case MODE_MASS: DFMI_LAUNCH(MODE_MASS); break;
+    case MODE_RESID_WORK:
+      if constexpr (sizeof(T) == sizeof(double)) {
+        if (!pre || !a.pout || !a.beta || !a.partial)
+          throw std::runtime_error("fine_apply: RESID_WORK needs the premasked form, pout, beta and partials");
+        hipLaunchKernelGGL((k_fine_apply<T, kResidWork, true>), grid, dim3(NT), 0, s, g, moduli, mask, ks, a, tl.TY,
+                           tl.TZ, tl.xchunk, tl.tiles_y, tl.tiles_z);
+      } else {
+        throw std::runtime_error("fine_apply: RESID_WORK is an FP64 mode");
+      }
+      break;
     default: throw std::runtime_error("fine_apply: bad mode");
   }
 #undef DFMI_LAUNCH

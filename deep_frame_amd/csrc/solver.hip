@@ -1521,6 +1521,8 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
   double* rr = sc + 6 * kc;
   double* scale = sc + 8 * kc;
   double* factor = sc + 9 * kc;
+  double* invs = sc + 10 * kc;  // per-column factor of a restarted FP32 residual
+  double* tnd = sc + 11 * kc;   // |r|^2 of the flushed columns
   int* act = h->active.as<int>();
   Level& F = *H.levels[0];
   VArgs v{h, &H, k, act};
@@ -1637,10 +1639,37 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
       for (int c = 0; c < k; ++c) fh[c] = 0.0;
       DFMI_HIP(hipMemcpyAsync(factor, fh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
       scale_columns<float>(d32, n, k, factor, act, s);
-      fine_fp64(h, H, MODE_RESID, x, r, b, k, act, nullptr, nullptr);
-      column_norms(h, r, n, k, act, tn);
+      // true FP64 residual of the flushed columns, written straight into R in FP32 scaled by the recursive estimate
+      // of its norm, with the partial sums of |r|^2; the continuing columns are then brought to unit norm
+      std::vector<double> shn(sh), inv(k, 1.0);
+      for (int c = 0; c < k; ++c) {
+        if (!upd[c]) continue;
+        const double e = std::sqrt(rrh[c]) * sh[c];
+        shn[c] = e > 1e-300 ? e : 1.0;
+        inv[c] = 1.0 / shn[c];
+      }
+      DFMI_HIP(hipMemcpyAsync(invs, inv.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      {
+        ApplyArgs a{};
+        a.x = x;
+        a.b = b;
+        a.k = k;
+        a.active = act;
+        a.partial = h->partial.as<double>();
+        a.beta = invs;
+        a.pout = r32;
+        a.premasked = 1;
+        int nbk = 0;
+        fine_apply<double>(h->pad, h->m64.as<double>(), F.mask.as<uint8_t>(), h->kb, MODE_RESID_WORK, a, s, &nbk);
+        reduce_partials(h->partial.as<double>(), nbk, k, tnd, s);
+        tn.resize(k);
+        DFMI_HIP(hipMemcpyAsync(tn.data(), tnd, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        sync(h);
+        for (auto& t : tn) t = std::sqrt(t);
+      }
       std::vector<int> cont(k, 0), done(k, 0);
       for (int c = 0; c < k; ++c) {
+        fh[c] = 1.0;
         if (!upd[c]) continue;
         const double tr = tn[c] / nb[c];
         rep[c].iterations = step;
@@ -1649,16 +1678,18 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
         if (tr < tol || !(tn[c] > 0)) {
           active[c] = 0;
           done[c] = 1;
-        } else {
+        } else {  // restart at unit norm: r / |r|
           cont[c] = 1;
+          fh[c] = shn[c] / tn[c];
           sh[c] = tn[c];
         }
       }
       if (count(cont)) {
+        DFMI_HIP(hipMemcpyAsync(factor, fh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
         upload(act, cont);
-        DFMI_HIP(hipMemcpyAsync(scale, sh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        to_work<float>(r, scale, r32, n, k, act, s);  // r / |r|
+        scale_columns<float>(r32, n, k, factor, act, s);
       }
+      DFMI_HIP(hipMemcpyAsync(scale, sh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
       zero_columns(r32, done);
       upload(act, active);
     }
