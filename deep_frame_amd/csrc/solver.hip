@@ -1630,6 +1630,11 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
       if (est * sh[c] < tol * nb[c] || est < delta) upd[c] = 1, any_upd = true;
     }
     h->history.push_back(worst);
+    // a flush also takes every column whose residual has dropped by sqrt(inner_tolerance) since its last
+    // restart: fewer flush iterations (each costs a separate pass for D += V a) and fewer restarts of the block
+    if (any_upd)
+      for (int c = 0; c < k; ++c)
+        if (active[c] && !upd[c] && std::sqrt(rrh[c]) < std::sqrt(delta)) upd[c] = 1;
     bool pending = true;  // D += V a not yet applied
     if (any_upd) {  // flush D into x, true FP64 residual, restart the column's scaled residual at norm 1
       block_combine(vcur, d32, d32, amat, k, k, n, s);
