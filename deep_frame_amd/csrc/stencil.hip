@@ -118,6 +118,34 @@ __global__ __launch_bounds__(SNT) void k_stencil_apply(Geom g, const T* __restri
   }
 }
 
+// One column (the power iterations of the set-up): a thread per node reads its 243 stencil entries straight
+// from the structure-of-arrays stencil (coalesced across the wavefront) and gathers its 27 neighbours; the
+// workgroup-staged form above would leave all but one thread per node idle.  Neighbour offsets that leave
+// the array read zeros through the buffer's range check; those that wrap around a grid line meet zero entries.
+template <typename T>
+__global__ __launch_bounds__(256) void k_stencil_one(Geom g, const T* __restrict__ st, const uint8_t* __restrict__ mask,
+                                                     const T* __restrict__ x, T* __restrict__ y) {
+  const int N = (int)g.nodes();
+  const int n = (int)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (n >= N) return;
+  const __amdgpu_buffer_rsrc_t xr = buffer(x, (long long)N * 3 * (long long)sizeof(T));
+  const int py = g.nz, px = g.ny * g.nz;
+  T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll 3
+  for (int d = 0; d < 27; ++d) {
+    const int od = (d / 9 - 1) * px + ((d / 3) % 3 - 1) * py + (d % 3 - 1);
+    T x0, x1, x2;
+    load3(xr, (unsigned)((n + od) * 3 * (int)sizeof(T)), x0, x1, x2);
+    const T* sd = st + (long long)d * 9 * N + n;
+    acc[0] = fma(sd[0], x0, fma(sd[(long long)N], x1, fma(sd[2LL * N], x2, acc[0])));
+    acc[1] = fma(sd[3LL * N], x0, fma(sd[4LL * N], x1, fma(sd[5LL * N], x2, acc[1])));
+    acc[2] = fma(sd[6LL * N], x0, fma(sd[7LL * N], x1, fma(sd[8LL * N], x2, acc[2])));
+  }
+  const uint8_t mn = mask[n];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[3LL * n + c] = live_bit(mn, c) ? acc[c] : T(0);
+}
+
 // right-hand sides per thread and nodes per workgroup (staged stencils kept within 48 KB of LDS)
 // (2 per thread measured fastest at 42 right-hand sides on the level-1 grid of 256 x 256 x 48: 3 and 6
 // per thread trade L2 re-reads for register pressure and lose)
@@ -142,6 +170,14 @@ void stencil_apply(const Geom& g, const T* st, const uint8_t* mask, int mode, co
   if (groups > SNT) throw std::runtime_error("stencil_apply: too many right-hand sides per call");
   if (g.nodes() * 3 * (long long)a.k * (long long)sizeof(T) > (long long)INT32_MAX)
     throw std::runtime_error("stencil_apply: level vectors too large for 32-bit byte offsets");
+  if (a.k == 1 && mode == MODE_APPLY && !a.active && lay.ns == 3) {
+    const unsigned b1 = (unsigned)((g.nodes() + 255) / 256);
+    if (nblocks_out) *nblocks_out = (int)b1;
+    hipLaunchKernelGGL(k_stencil_one<T>, dim3(b1), dim3(256), 0, s, g, st, mask, reinterpret_cast<const T*>(a.x),
+                       reinterpret_cast<T*>(a.y));
+    DFMI_HIP(hipGetLastError());
+    return;
+  }
   const int npb = stencil_npb<T>(a.k);
   const unsigned bx = (unsigned)((g.nodes() + npb - 1) / npb);
   const size_t lds = sizeof(T) * SP * (size_t)npb;
