@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 
 #include "dfmi_internal.h"
@@ -26,13 +27,51 @@ const char* get_error() { return g_error.c_str(); }
 // reference corner k -> corner bits q = a + 2b + 4c
 static const int kRefToBits[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 
+// Device buffers are allocated and released stream-ordered (hipMallocAsync / hipFreeAsync from the device's
+// memory pool, which keeps released memory for reuse) on the stream of the entry point that runs, so that
+// releasing a temporary does not wait for the whole device: hipFree does, which stalls every other stream
+// (concurrent handles on their own streams) and empties the queue of this one.  Every entry point of a handle
+// sets that stream (StreamScope) and returns with its stream synchronised, so a buffer is never released on a
+// stream other than the one its last use was queued on while that use is in flight.  Outside an entry point
+// of a handle the plain allocator is used.
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local bool t_stream_set = false;
+
+struct StreamScope {
+  hipStream_t prev;
+  bool prev_set;
+  explicit StreamScope(hipStream_t s) : prev(t_stream), prev_set(t_stream_set) {
+    t_stream = s;
+    t_stream_set = true;
+  }
+  ~StreamScope() {
+    t_stream = prev;
+    t_stream_set = prev_set;
+  }
+};
+
+// keep the memory the stream-ordered allocator releases (the default threshold returns it to the system at
+// every synchronisation)
+static void keep_pool_memory() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0;
+    hipMemPool_t pool;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  });
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool pooled = false;           // allocated from the stream-ordered pool
+  hipStream_t home = nullptr;    // the stream it was allocated on
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), pooled(o.pooled), home(o.home) {
     o.p = nullptr;
     o.bytes = 0;
   }
@@ -41,6 +80,8 @@ struct DevBuf {
       release();
       p = o.p;
       bytes = o.bytes;
+      pooled = o.pooled;
+      home = o.home;
       o.p = nullptr;
       o.bytes = 0;
     }
@@ -49,13 +90,32 @@ struct DevBuf {
   void ensure(size_t n) {
     if (n <= bytes) return;
     release();
-    DFMI_HIP(hipMalloc(&p, n));
+    if (t_stream_set) {
+      DFMI_HIP(hipMallocAsync(&p, n, t_stream));
+      pooled = true;
+      home = t_stream;
+    } else {
+      DFMI_HIP(hipMalloc(&p, n));
+      pooled = false;
+    }
     bytes = n;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (!pooled)
+        (void)hipFree(p);
+      else if (t_stream_set) {
+        if (home != t_stream) (void)hipStreamSynchronize(home);  // a handle whose stream was changed
+        (void)hipFreeAsync(p, t_stream);
+      }
+      else {  // released outside an entry point of a handle: the device is idle for it only after a full wait
+        (void)hipDeviceSynchronize();
+        (void)hipFree(p);
+      }
+    }
     p = nullptr;
     bytes = 0;
+    pooled = false;
   }
   template <typename T>
   T* as() const {
@@ -1750,6 +1810,21 @@ struct dfmi_net {
     return 1;                             \
   }
 
+// entry points of a handle: device buffers are allocated and released on the handle's stream
+#define DFMI_GUARD_MG(...)                                     \
+  try {                                                        \
+    if (!mg) throw std::invalid_argument("null handle");       \
+    dfmi::StreamScope dfmi_scope_(mg->h.stream);               \
+    __VA_ARGS__;                                               \
+    return 0;                                                  \
+  } catch (const std::exception& e) {                          \
+    dfmi::set_error(e.what());                                 \
+    return 1;                                                  \
+  } catch (...) {                                              \
+    dfmi::set_error("unknown failure");                        \
+    return 1;                                                  \
+  }
+
 extern "C" {
 
 const char* dfmi_last_error(void) { return dfmi::get_error(); }
@@ -1787,6 +1862,8 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
                 dfmi_mg** out) {
   DFMI_GUARD({
     if (!shape || !spacing || !ke || !out) throw std::invalid_argument("null argument");
+    keep_pool_memory();
+    StreamScope scope(nullptr);  // the handle's buffers come from the stream-ordered pool as well
     for (int a = 0; a < 3; ++a) {
       if (shape[a] < 1) throw std::invalid_argument("grid shape must contain three positive integers");
       if (!(spacing[a] > 0) || !std::isfinite(spacing[a])) throw std::invalid_argument("spacing must be positive");
@@ -1846,16 +1923,17 @@ int dfmi_create(const int32_t shape[3], const double spacing[3], const double* k
 int dfmi_destroy(dfmi_mg* mg) {
   DFMI_GUARD({
     if (mg) {
+      StreamScope scope(mg->h.stream);
       (void)hipStreamSynchronize(mg->h.stream);
       delete mg;
     }
   })
 }
 
-int dfmi_set_stream(dfmi_mg* mg, void* stream) { DFMI_GUARD({ mg->h.stream = (hipStream_t)stream; }) }
+int dfmi_set_stream(dfmi_mg* mg, void* stream) { DFMI_GUARD_MG({ mg->h.stream = (hipStream_t)stream; }) }
 
 int dfmi_levels(dfmi_mg* mg, int32_t* levels, int32_t padded[3]) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     *levels = mg->h.nlev;
     padded[0] = mg->h.pad.ex;
     padded[1] = mg->h.pad.ey;
@@ -1864,7 +1942,7 @@ int dfmi_levels(dfmi_mg* mg, int32_t* levels, int32_t padded[3]) {
 }
 
 int dfmi_update(dfmi_mg* mg, const double* moduli) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     pad_moduli(h.pad, h.ux, h.uy, h.uz, moduli, h.m64.as<double>(), h.m32.as<float>(), h.stream);
     for (auto& H : h.slots) H = Hierarchy();
@@ -1874,7 +1952,7 @@ int dfmi_update(dfmi_mg* mg, const double* moduli) {
 }
 
 int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed, double* setup_seconds) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     build(&mg->h, slot, fixed);
     if (setup_seconds) *setup_seconds = mg->h.slots[slot].setup_s;
@@ -1883,7 +1961,7 @@ int dfmi_build_hierarchy(dfmi_mg* mg, int32_t slot, const uint8_t* fixed, double
 
 int dfmi_build_hierarchy_kernel(dfmi_mg* mg, int32_t slot, const uint8_t* fixed, const double* coefficients,
                                 int32_t nk, double* setup_seconds) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     if (nk < 0 || nk > 6 || (nk > 0 && !coefficients)) throw std::invalid_argument("nk must be in [0, 6]");
     build(&mg->h, slot, fixed, coefficients, nk);
@@ -1892,7 +1970,7 @@ int dfmi_build_hierarchy_kernel(dfmi_mg* mg, int32_t slot, const uint8_t* fixed,
 }
 
 int dfmi_kernel_component(dfmi_mg* mg, int32_t slot, const double* rhs, int32_t k, int32_t* nk, double* component) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
     Hierarchy& H = h.slots[slot];
@@ -1974,7 +2052,7 @@ static void retry_failed(Handle* h, int slot, const double* rhs, double* x, int 
 }
 
 int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t k, dfmi_report* report) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     if (k < 1 || !rhs || !x || !report) throw std::invalid_argument("solve needs k >= 1 and buffers");
     mg->h.history.clear();
@@ -1989,7 +2067,7 @@ int dfmi_solve(dfmi_mg* mg, int32_t slot, const double* rhs, double* x, int32_t 
 }
 
 int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x, double* y, int32_t k) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (!h.have_moduli) throw std::runtime_error("update(moduli) must precede apply");
     if (k < 1) throw std::invalid_argument("k must be >= 1");
@@ -2038,7 +2116,7 @@ int dfmi_apply(dfmi_mg* mg, int32_t slot, int32_t precision, const double* x, do
 }
 
 int dfmi_set_mass(dfmi_mg* mg, const double* me) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (!me) throw std::invalid_argument("null mass matrix");
     Handle& h = mg->h;
     double mb[24][24];
@@ -2079,7 +2157,7 @@ int dfmi_set_mass(dfmi_mg* mg, const double* me) {
 
 int dfmi_mass_apply(dfmi_mg* mg, int32_t slot, const double* mass, const double* lumped, const double* x, double* y,
                     int32_t k) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (!h.have_mass) throw std::runtime_error("dfmi_set_mass must precede mass_apply");
     if (k < 1 || !mass || !x || !y) throw std::invalid_argument("mass_apply needs k >= 1 and buffers");
@@ -2110,7 +2188,7 @@ int dfmi_mass_apply(dfmi_mg* mg, int32_t slot, const double* mass, const double*
 }
 
 int dfmi_mask(dfmi_mg* mg, int32_t slot, double* x, int32_t k) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (k < 1 || !x) throw std::invalid_argument("mask needs k >= 1 and a buffer");
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
@@ -2124,7 +2202,7 @@ int dfmi_mask(dfmi_mg* mg, int32_t slot, double* x, int32_t k) {
 }
 
 int dfmi_element_energy(dfmi_mg* mg, const double* u, int32_t k, double* energy) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     ensure_workspace(&h, k);
     flat_to_grid(h.pad, h.ux, h.uy, h.uz, u, h.b.as<double>(), k, h.stream);
@@ -2135,7 +2213,7 @@ int dfmi_element_energy(dfmi_mg* mg, const double* u, int32_t k, double* energy)
 
 int dfmi_compliance_sensitivity(dfmi_mg* mg, const double* u, const double* f, int32_t k, const double* dmoduli,
                                 double factor, double* compliance, double* sensitivity) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     ensure_workspace(&h, k);
     const long long ne = (long long)h.ux * h.uy * h.uz;
@@ -2152,7 +2230,7 @@ int dfmi_compliance_sensitivity(dfmi_mg* mg, const double* u, const double* f, i
 
 int dfmi_modal_sensitivity(dfmi_mg* mg, const double* u, int32_t k, const double* weight, const double* value,
                            const double* stiffness_slope, const double* mass_slope, double target, double* out) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (!h.have_mass) throw std::runtime_error("dfmi_set_mass must precede modal_sensitivity");
     if (!u || !weight || !value || !stiffness_slope || !mass_slope || !out) throw std::invalid_argument("null buffer");
@@ -2166,7 +2244,7 @@ int dfmi_modal_sensitivity(dfmi_mg* mg, const double* u, int32_t k, const double
 }
 
 int dfmi_level_apply(dfmi_mg* mg, int32_t slot, int32_t level, const double* x, double* y, int32_t k) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
     Hierarchy& H = h.slots[slot];
@@ -2186,7 +2264,7 @@ int dfmi_level_apply(dfmi_mg* mg, int32_t slot, int32_t level, const double* x, 
 }
 
 int dfmi_precondition(dfmi_mg* mg, int32_t slot, const double* r, double* z, int32_t k) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
     Hierarchy& H = h.slots[slot];
@@ -2202,7 +2280,7 @@ int dfmi_precondition(dfmi_mg* mg, int32_t slot, const double* r, double* z, int
 
 int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, const double* in, double* out,
                   int32_t k) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
     Hierarchy& H = h.slots[slot];
@@ -2223,7 +2301,7 @@ int dfmi_transfer(dfmi_mg* mg, int32_t slot, int32_t level, int32_t direction, c
 
 int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision, int32_t k, int32_t reps,
                     double* ms_per_launch) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
     if (k < 1 || reps < 1) throw std::invalid_argument("k and reps must be positive");
@@ -2276,7 +2354,7 @@ int dfmi_time_apply(dfmi_mg* mg, int32_t slot, int32_t level, int32_t precision,
 }
 
 int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double* ms_per_launch) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     Handle& h = mg->h;
     if (slot < 0 || slot >= 8 || !h.slots[slot].built) throw std::invalid_argument("slot not built");
     if (k < 1) throw std::invalid_argument("k must be positive");
@@ -2297,7 +2375,7 @@ int dfmi_tune_fine(dfmi_mg* mg, int32_t slot, int32_t k, int32_t tile[2], double
 }
 
 int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coarsest_dofs) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (slot < 0 || slot >= 8) throw std::invalid_argument("slot must be in [0, 8)");
     Hierarchy& H = mg->h.slots[slot];
     if (!H.built) throw std::invalid_argument("slot not built");
@@ -2307,7 +2385,7 @@ int dfmi_diagnostics(dfmi_mg* mg, int32_t slot, double* lambda_max, int32_t* coa
 }
 
 int dfmi_history(dfmi_mg* mg, double* out, int32_t capacity, int32_t* count) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     const auto& hist = mg->h.history;
     if (count) *count = (int32_t)hist.size();
     if (out)
@@ -2316,7 +2394,7 @@ int dfmi_history(dfmi_mg* mg, double* out, int32_t capacity, int32_t* count) {
 }
 
 int dfmi_kernel_check(dfmi_mg* mg, int32_t slot, double* out, int32_t* count) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (slot < 0 || slot >= 8 || !mg->h.slots[slot].built) throw std::invalid_argument("slot not built");
     const auto& kc = mg->h.slots[slot].kernel_check;
     if (count) *count = (int32_t)kc.size();
@@ -2388,7 +2466,7 @@ int dfmi_net_adam(dfmi_net* net, double lr, double beta1, double beta2, double e
 
 int dfmi_net_train_step(dfmi_net* net, dfmi_mg* mg, int32_t slot, const float* g, int32_t n, double lr,
                         double* loss) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (slot < 0 || slot >= 8 || !mg->h.slots[slot].built) throw std::invalid_argument("hierarchy slot not built");
     Hierarchy& H = mg->h.slots[slot];
     if (H.levels.size() < 2) throw std::invalid_argument("training needs at least two levels");
@@ -2410,7 +2488,7 @@ int dfmi_net_train_step(dfmi_net* net, dfmi_mg* mg, int32_t slot, const float* g
 }
 
 int dfmi_attach_net(dfmi_mg* mg, dfmi_net* net) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (net && mg->h.st.precision != 1) throw std::invalid_argument("the learned correction needs precision fp32");
     mg->h.net = net ? net->impl : nullptr;
     if (net) net_invalidate_feature(net->impl);
@@ -2418,7 +2496,7 @@ int dfmi_attach_net(dfmi_mg* mg, dfmi_net* net) {
 }
 
 int dfmi_harvest(dfmi_mg* mg, float* buffer, int64_t capacity) {
-  DFMI_GUARD({
+  DFMI_GUARD_MG({
     if (capacity < 0 || (capacity > 0 && !buffer)) throw std::invalid_argument("harvest needs a buffer");
     mg->h.harvest = capacity > 0 ? buffer : nullptr;
     mg->h.harvest_cap = capacity;
@@ -2426,5 +2504,5 @@ int dfmi_harvest(dfmi_mg* mg, float* buffer, int64_t capacity) {
   })
 }
 
-int dfmi_harvest_count(dfmi_mg* mg, int64_t* count) { DFMI_GUARD({ *count = mg->h.harvest_count; }) }
+int dfmi_harvest_count(dfmi_mg* mg, int64_t* count) { DFMI_GUARD_MG({ *count = mg->h.harvest_count; }) }
 }  // extern "C"
