@@ -9,7 +9,7 @@ The mixed solver is one block PCG recurrence over the 42 columns in FP32 with re
 FP64 updates, around an FP32 V-cycle whose level-1 correction is two flexible-CG steps
 preconditioned by the level-1 V-cycle, whose own level-2 correction is two flexible-CG
 steps again (coarse_krylov 2, krylov_levels 2), with Chebyshev degree 3 on every level
-(growth 1.0) and reliable updates after a 1e-4 drop of the FP32 residual
+(growth 1.0) and reliable updates after a 3e-3 drop of the FP32 residual
 (inner_tolerance); the FP64 solve is the reference's scheme (FP64 V-cycle, degree
 3 * 2^level, one PCG recurrence per column), timed with the same warm-up and steps.  Two
 more legs separate the gains: the mixed solver with one recurrence per column, and FP64
@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--scheme", default="refine", choices=["refine", "flexible"],
                     help="mixed solver: FP32 PCG inside FP64 iterative refinement, or FP64 PCG with an FP32 V-cycle")
-    ap.add_argument("--inner-tolerance", type=float, default=1e-4,
+    ap.add_argument("--inner-tolerance", type=float, default=3e-3,
                     help="refine scheme: FP32 residual drop between reliable FP64 updates")
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--coarse-krylov", type=int, default=2, choices=[0, 1, 2],
@@ -461,7 +461,7 @@ def run_rows(args):
     points = element.node_points(shape)
     pins = problems.relief_support(points, np.flatnonzero(points[:, 2] == 0))
     bench_mixed = {"precision": "float32", "refine": True, "coarse_krylov": 2, "krylov_levels": 2, "growth": 1.0,
-                   "inner_tolerance": 1e-4}
+                   "inner_tolerance": 3e-3}
     schemes = (("clamped", prob["fixed"], None, {"precision": "float32", "refine": True}),
                ("clamped_fp64", prob["fixed"], None, {"precision": "float64"}),
                ("clamped_flexible", prob["fixed"], None, {"precision": "float32"}),

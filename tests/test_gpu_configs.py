@@ -25,7 +25,7 @@ pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 # the benchmark's mixed configuration: block PCG over the columns; MIXED_COLUMNS: one recurrence per column
 MIXED = {"precision": "float32", "refine": True, "fine_smoother": "vector", "coarse_krylov": 2, "krylov_levels": 2,
-         "growth": 1.0, "chebyshev_degree": 3, "inner_tolerance": 1e-4, "block_krylov": True}
+         "growth": 1.0, "chebyshev_degree": 3, "inner_tolerance": 3e-3, "block_krylov": True}
 MIXED_COLUMNS = {**MIXED, "block_krylov": False}
 FP64 = {"precision": "float64"}
 
