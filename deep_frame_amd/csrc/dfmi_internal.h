@@ -230,6 +230,7 @@ void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int 
                 hipStream_t s);
 void power_init(const uint8_t* mask, double* v, long long nodes, unsigned seed, hipStream_t s);
 void inv_sqrt(const double* in, double* out, hipStream_t s);  // out[0] = 1 / sqrt(in[0])
+// v = dinv * y * inv_norm[0] (dinv null: 1; inv_norm null: 1)
 void dinv_scale(const double* y, const double* dinv, const double* inv_norm, double* v, long long nodes,
                 hipStream_t s);
 void pcg_beta(const double* rzn, const double* rzc, double* rz, double* beta, int k, const int* active,

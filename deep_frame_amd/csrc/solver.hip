@@ -182,6 +182,16 @@ struct Handle {
   long long harvest_cap = 0, harvest_count = 0;
   // largest relative recursive residual over the active columns after each PCG iteration of the last solve
   std::vector<double> history;
+  // side streams of the hierarchy set-up (default priority): the coarse levels' power iterations and the
+  // coarsest inverse run beside the fine level's power iteration
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  Handle() = default;
+  Handle(const Handle&) = delete;
+  Handle& operator=(const Handle&) = delete;
+  ~Handle() {
+    for (hipStream_t& q : side)
+      if (q) (void)hipStreamDestroy(q), q = nullptr;
+  }
 };
 
 static void hex8_blocks(const double* ke_bits, KeBlocks& kb) {
@@ -831,43 +841,39 @@ static double device_max(Handle* h, const double* v, long long n) {
   return out;
 }
 
-// lambda_max(Dinv A) by power iteration (topology_multigrid.py:214-223), in FP64.  The iterate is
-// normalised on the device (w = Dinv A v, v = w / |w|), so the host waits once per level.
-static double spectrum(Handle* h, Hierarchy& H, int idx) {
+// lambda_max(Dinv A) by power iteration (topology_multigrid.py:214-223), in FP64, queued on stream `st` with the
+// caller's buffers (v, y: one level vector each; nrm: 2 doubles, |w|^2 of the last step left in nrm[0]; part:
+// reduce_blocks partials).  The iterate is normalised on the device (w = Dinv A v, v = w / |w|), so nothing waits
+// for the host; lambda_max = sqrt(nrm[0]) once the stream has run.
+struct PowerBufs {
+  DevBuf v, y, nrm, part;
+};
+
+static void spectrum_async(Handle* h, Hierarchy& H, int idx, hipStream_t st, PowerBufs& pb) {
   Level& L = *H.levels[idx];
   const long long n = col(L.g);
-  DevBuf v, y, nrm;
-  v.ensure(sizeof(double) * n);
-  y.ensure(sizeof(double) * n);
-  nrm.ensure(sizeof(double) * 4);
-  double* sq = nrm.as<double>();       // |w|^2
-  double* inv = nrm.as<double>() + 1;  // 1 / |w|
-  double* one = nrm.as<double>() + 2;
-  const double host_one = 1.0;
-  DFMI_HIP(hipMemcpyAsync(one, &host_one, sizeof(double), hipMemcpyHostToDevice, h->stream));
-  power_init(L.mask.as<uint8_t>(), v.as<double>(), L.g.nodes(), (unsigned)h->st.seed + 7919u * idx, h->stream);
+  double* sq = pb.nrm.as<double>();       // |w|^2
+  double* inv = pb.nrm.as<double>() + 1;  // 1 / |w|
+  // no host-to-device copy here: a pageable copy would wait for the other streams of the set-up
+  power_init(L.mask.as<uint8_t>(), pb.v.as<double>(), L.g.nodes(), (unsigned)h->st.seed + 7919u * idx, st);
   const int nb = reduce_blocks(n);
-  require_workspace(h);
+  double* part = pb.part.as<double>();
   for (int it = 0; it < h->st.power_iterations; ++it) {
     ApplyArgs a{};
-    a.x = v.as<double>();
-    a.y = y.as<double>();
+    a.x = pb.v.as<double>();
+    a.y = pb.y.as<double>();
     a.k = 1;
     if (idx == 0)
-      fine_apply<double>(L.g, h->m64.as<double>(), L.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, h->stream, nullptr);
+      fine_apply<double>(L.g, h->m64.as<double>(), L.mask.as<uint8_t>(), h->kb, MODE_APPLY, a, st, nullptr);
     else
-      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, column_major(L.g),
-                            h->stream, nullptr);
-    dinv_scale(y.as<double>(), L.dinv64.as<double>(), one, y.as<double>(), L.g.nodes(), h->stream);
-    column_dot<double>(y.as<double>(), y.as<double>(), n, 1, h->partial.as<double>(), nb, nullptr, h->stream);
-    reduce_partials(h->partial.as<double>(), nb, 1, sq, h->stream);
-    inv_sqrt(sq, inv, h->stream);
-    dinv_scale(y.as<double>(), nullptr, inv, v.as<double>(), L.g.nodes(), h->stream);
+      stencil_apply<double>(L.g, L.st64.as<double>(), L.mask.as<uint8_t>(), MODE_APPLY, a, column_major(L.g), st,
+                            nullptr);
+    dinv_scale(pb.y.as<double>(), L.dinv64.as<double>(), nullptr, pb.y.as<double>(), L.g.nodes(), st);
+    column_dot<double>(pb.y.as<double>(), pb.y.as<double>(), n, 1, part, nb, nullptr, st);
+    reduce_partials(part, nb, 1, sq, st);
+    inv_sqrt(sq, inv, st);
+    dinv_scale(pb.y.as<double>(), nullptr, inv, pb.v.as<double>(), L.g.nodes(), st);
   }
-  double out = 0.0;
-  DFMI_HIP(hipMemcpyAsync(&out, sq, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  sync(h);
-  return std::sqrt(out);
 }
 
 // Euclidean norm of n contiguous FP64 values (set-up diagnostics; own partial buffers)
@@ -1028,8 +1034,22 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
   }
   fst.release();
   ensure_workspace(h, 1);
-  for (int l = 0; l + 1 < h->nlev; ++l) H.levels[l]->lambda_max = spectrum(h, H, l);
-  // coarsest dense inverse over live DOFs
+  // side streams only for large grids: creating a stream costs milliseconds, more than the overlap gains on
+  // small ones (many small handles, as in a batch of variants, keep everything on their own stream)
+  const bool fork = F.g.nodes() >= (1LL << 20);
+  if (fork)
+    for (hipStream_t& q : h->side)
+      if (!q) DFMI_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+  const hipStream_t side_inv = fork ? h->side[2] : s, side_l1 = fork ? h->side[0] : s, side_lc = fork ? h->side[1] : s;
+  auto event_on = [](hipStream_t st) {
+    hipEvent_t e;
+    DFMI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    DFMI_HIP(hipEventRecord(e, st));
+    return e;
+  };
+  std::vector<hipEvent_t> joins;
+  // coarsest dense inverse over live DOFs: assembled on the handle's stream, inverted on a side stream (clamped
+  // fixtures) while the power iterations run
   Level& C = *H.levels.back();
   const long long nc = C.g.nodes();
   std::vector<uint8_t> mk(nc);
@@ -1044,8 +1064,8 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
   H.coarsest_n = (int)live.size();
   if (H.coarsest_n > DENSE_COARSEST_MAX)
     throw std::runtime_error("coarsest level too large for the dense solve; add levels");
+  DevBuf idx;
   if (H.coarsest_n > 0) {
-    DevBuf idx;
     idx.ensure(sizeof(int) * nc * 3);
     H.live_dofs.ensure(sizeof(int) * live.size());
     DFMI_HIP(hipMemcpyAsync(idx.p, index.data(), sizeof(int) * nc * 3, hipMemcpyHostToDevice, s));
@@ -1069,11 +1089,45 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
       pin_dense(H.ainv.as<double>(), n, pd.as<int>(), (int)pins.size(), s);
       dense_sweep_inverse(H.ainv.as<double>(), n, s);
       project_dense_inverse(H.ainv.as<double>(), n, pd.as<int>(), (int)pins.size(), kernel, nk, s);
+      sync(h);
     } else {
-      dense_sweep_inverse(H.ainv.as<double>(), n, s);
+      const hipEvent_t ready = event_on(s);
+      DFMI_HIP(hipStreamWaitEvent(side_inv, ready, 0));
+      DFMI_HIP(hipEventDestroy(ready));
+      dense_sweep_inverse(H.ainv.as<double>(), n, side_inv);
+      joins.push_back(event_on(side_inv));
     }
-    sync(h);
   }
+  // lambda_max of every level but the coarsest: level 0 on the handle's stream, level 1 and the coarser levels
+  // on side streams; every buffer is allocated here, before the streams fork, and released after they join
+  const int np = h->nlev - 1;
+  std::vector<PowerBufs> pbs(np);
+  for (int l = 0; l < np; ++l) {
+    const long long n = col(H.levels[l]->g);
+    pbs[l].v.ensure(sizeof(double) * n);
+    pbs[l].y.ensure(sizeof(double) * n);
+    pbs[l].nrm.ensure(sizeof(double) * 2);
+    pbs[l].part.ensure(sizeof(double) * reduce_blocks(n));
+  }
+  {
+    const hipEvent_t ready = event_on(s);
+    DFMI_HIP(hipStreamWaitEvent(side_l1, ready, 0));
+    DFMI_HIP(hipStreamWaitEvent(side_lc, ready, 0));
+    DFMI_HIP(hipEventDestroy(ready));
+  }
+  for (int l = 1; l < np; ++l) spectrum_async(h, H, l, l == 1 ? side_l1 : side_lc, pbs[l]);
+  if (np > 1) joins.push_back(event_on(side_l1));
+  if (np > 2) joins.push_back(event_on(side_lc));
+  if (np > 0) spectrum_async(h, H, 0, s, pbs[0]);
+  for (hipEvent_t e : joins) {
+    DFMI_HIP(hipStreamWaitEvent(s, e, 0));
+    DFMI_HIP(hipEventDestroy(e));
+  }
+  std::vector<double> sqs(np, 0.0);
+  for (int l = 0; l < np; ++l)
+    DFMI_HIP(hipMemcpyAsync(&sqs[l], pbs[l].nrm.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  sync(h);
+  for (int l = 0; l < np; ++l) H.levels[l]->lambda_max = std::sqrt(sqs[l]);
   if (nk > 0) kernel_check(h, H, coef, nk, center);
   H.built = true;
   H.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

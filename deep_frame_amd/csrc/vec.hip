@@ -147,7 +147,7 @@ void power_init(const uint8_t* mask, double* v, long long nodes, unsigned seed, 
 __global__ void k_dinv_scale(const double* y, const double* __restrict__ dinv, const double* inv_norm, double* v,
                              long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = (dinv ? dinv[i] : 1.0) * y[i] * inv_norm[0];
+  if (i < n) v[i] = (dinv ? dinv[i] : 1.0) * y[i] * (inv_norm ? inv_norm[0] : 1.0);
 }
 
 void dinv_scale(const double* y, const double* dinv, const double* inv_norm, double* v, long long nodes,
