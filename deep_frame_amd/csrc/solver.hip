@@ -147,7 +147,19 @@ struct Hierarchy {
   DevBuf kern, kwork, kcoef;
   // |A_l B_l| / |A_l R_l| per level for slots with a free kernel (empty otherwise)
   std::vector<double> kernel_check;
+  // completion of the coarsest inverse when it runs on a side stream (null: ordered on the handle's stream);
+  // the coarsest solve waits for it on the device
+  hipEvent_t ainv_ready = nullptr;
 };
+
+// Before a hierarchy is replaced or dropped: order the handle's stream after its pending coarsest inverse, so
+// that the stream-ordered release of its buffers cannot hand them out while the side stream still writes them.
+static void retire(Hierarchy& H, hipStream_t s) {
+  if (!H.ainv_ready) return;
+  (void)hipStreamWaitEvent(s, H.ainv_ready, 0);
+  (void)hipEventDestroy(H.ainv_ready);
+  H.ainv_ready = nullptr;
+}
 
 constexpr int KERNEL_GRAM_BLOCKS = 512;
 
@@ -189,8 +201,10 @@ struct Handle {
   Handle(const Handle&) = delete;
   Handle& operator=(const Handle&) = delete;
   ~Handle() {
+    for (Hierarchy& H : slots)
+      if (H.ainv_ready) (void)hipEventSynchronize(H.ainv_ready), (void)hipEventDestroy(H.ainv_ready);
     for (hipStream_t& q : side)
-      if (q) (void)hipStreamDestroy(q), q = nullptr;
+      if (q) (void)hipStreamSynchronize(q), (void)hipStreamDestroy(q), q = nullptr;
   }
 };
 
@@ -632,6 +646,7 @@ static void vcycle(VArgs& v, int idx, const T* b) {
   Level& L = *H.levels[idx];
   hipStream_t s = v.h->stream;
   if (idx == (int)H.levels.size() - 1) {
+    if (H.ainv_ready) DFMI_HIP(hipStreamWaitEvent(s, H.ainv_ready, 0));
     coarsest_solve<T>(L.g, H.ainv.as<double>(), H.live_dofs.as<int>(), H.coarsest_n, b, L.x.as<T>(),
                       level_layout(L, idx, v.k), H.work_in.as<double>(), H.work_out.as<double>(),
                       H.work_split.as<double>(), v.k, v.active, s);
@@ -950,8 +965,9 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
   if (!h->have_moduli) throw std::runtime_error("update(moduli) must precede hierarchy construction");
   auto t0 = std::chrono::steady_clock::now();
   Hierarchy& H = h->slots[slot];
-  H = Hierarchy();
   hipStream_t s = h->stream;
+  retire(H, s);
+  H = Hierarchy();
   ensure_workspace(h, 1);  // reductions below use the partial-sum and scalar buffers
   for (int l = 0; l < h->nlev; ++l) {
     auto L = std::make_unique<Level>();
@@ -1095,7 +1111,7 @@ static void build(Handle* h, int slot, const uint8_t* fixed_dev, const double* c
       DFMI_HIP(hipStreamWaitEvent(side_inv, ready, 0));
       DFMI_HIP(hipEventDestroy(ready));
       dense_sweep_inverse(H.ainv.as<double>(), n, side_inv);
-      joins.push_back(event_on(side_inv));
+      if (fork) H.ainv_ready = event_on(side_inv);  // joined by the first coarsest solve, not here
     }
   }
   // lambda_max of every level but the coarsest: level 0 on the handle's stream, level 1 and the coarser levels
@@ -1999,7 +2015,7 @@ int dfmi_update(dfmi_mg* mg, const double* moduli) {
   DFMI_GUARD_MG({
     Handle& h = mg->h;
     pad_moduli(h.pad, h.ux, h.uy, h.uz, moduli, h.m64.as<double>(), h.m32.as<float>(), h.stream);
-    for (auto& H : h.slots) H = Hierarchy();
+    for (auto& H : h.slots) retire(H, h.stream), H = Hierarchy();
     h.have_moduli = true;
     sync(&h);
   })
