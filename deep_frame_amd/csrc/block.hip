@@ -169,17 +169,56 @@ __device__ __forceinline__ void bk_store4(__amdgpu_buffer_rsrc_t r, unsigned off
   }
 }
 
+// Four consecutive FP64 entries (32 bytes) per lane; the partial form takes single dwords, so rows at or past the
+// buffer's end read zeros and their stores are dropped.
+__device__ __forceinline__ void bk_load4d(__amdgpu_buffer_rsrc_t r, unsigned off, bool whole, double (&v)[4]) {
+  unsigned w[8];
+  if (whole) {
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = a[q], w[4 + q] = b[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, off + 4u * q, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = __hiloint2double((int)w[2 * q + 1], (int)w[2 * q]);
+}
+__device__ __forceinline__ void bk_store4d(__amdgpu_buffer_rsrc_t r, unsigned off, bool whole, const double (&v)[4]) {
+  unsigned w[8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) w[2 * q] = (unsigned)__double2loint(v[q]), w[2 * q + 1] = (unsigned)__double2hiint(v[q]);
+  if (whole) {
+    __attribute__((ext_vector_type(4))) unsigned a = {w[0], w[1], w[2], w[3]}, b = {w[4], w[5], w[6], w[7]};
+    __builtin_amdgcn_raw_buffer_store_b128(a, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(b, r, off + 16u, 0, 0);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) __builtin_amdgcn_raw_buffer_store_b32(w[q], r, off + 4u * q, 0, 0);
+  }
+}
+
+// Flushed output columns of a one-set combination (FLUSH): an output column j with flag[j] != 0 adds
+// scale[j] * Y[j] into the FP64 vector x (same row layout, column stride n) and stores zero as Y[j].
+struct BkFlush {
+  double* x;
+  const double* scale;
+  const int* flag;
+};
+
 template <bool B>
 struct bk_tag {
   static constexpr bool value = B;
 };
 
-template <int NW, int OG, int SETS, bool NORM>
+template <int NW, int OG, int SETS, bool NORM, bool FLUSH = false>
 __global__ __launch_bounds__(64 * NW) void k_bcombine(const float* __restrict__ u, const float* x0, float* y0,
                                                        const float* __restrict__ m0, const float* x1, float* y1,
                                                        const float* __restrict__ m1, float sgn, int kin, int kout,
-                                                       long long n, double* __restrict__ partial) {
+                                                       long long n, double* __restrict__ partial, BkFlush fl) {
   static_assert(!NORM || SETS == 1, "column norms are formed for one output set");
+  static_assert(!FLUSH || (SETS == 1 && !NORM), "flushed columns are formed by a plain one-set pass");
   constexpr int WPS = NW / SETS;  // waves per output set
   __shared__ __align__(16) float Us[BK_MAXK][256];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
@@ -235,6 +274,20 @@ __global__ __launch_bounds__(64 * NW) void k_bcombine(const float* __restrict__ 
     for (int o = 0; o < OG; ++o) {  // fully unrolled with a wave-uniform guard (a break keeps a loop that
                                     // indexes acc dynamically)
       if (o < jn) {
+        if constexpr (FLUSH) {
+          if (fl.flag[j0 + o]) {  // wave-uniform: x += s Y, and Y restarts at zero
+            const double sj = fl.scale[j0 + o];
+            const __amdgpu_buffer_rsrc_t xr = buffer(fl.x + (long long)(j0 + o) * n, 2 * cbytes);
+            double xv[4];
+            bk_load4d(xr, 2u * off, whole, xv);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              xv[q] += sj * (double)acc[o][q];
+              acc[o][q] = 0.f;
+            }
+            bk_store4d(xr, 2u * off, whole, xv);
+          }
+        }
         bk_store4(buffer(y + (long long)(j0 + o) * n, cbytes), off, whole, acc[o]);
         if (NORM) {  // rows past n were read as zeros
 #pragma unroll
@@ -321,15 +374,15 @@ static void launch_bcombine(const float* u, const float* x, float* y, const floa
   switch ((kout + 15) / 16) {  // 16 output columns per wave
     case 1:
       hipLaunchKernelGGL((k_bcombine<1, 16, 1, NORM>), grid, dim3(64), 0, s, u, x, y, m, x, y, m, sgn, kin, kout, n,
-                         partial);
+                         partial, BkFlush{});
       break;
     case 2:
       hipLaunchKernelGGL((k_bcombine<2, 16, 1, NORM>), grid, dim3(128), 0, s, u, x, y, m, x, y, m, sgn, kin, kout, n,
-                         partial);
+                         partial, BkFlush{});
       break;
     default:
       hipLaunchKernelGGL((k_bcombine<3, 16, 1, NORM>), grid, dim3(192), 0, s, u, x, y, m, x, y, m, sgn, kin, kout, n,
-                         partial);
+                         partial, BkFlush{});
       break;
   }
   DFMI_HIP(hipGetLastError());
@@ -353,15 +406,43 @@ void block_combine2(const float* u, const float* x0, float* y0, const float* m0,
   switch ((k + 15) / 16) {  // waves per set: 2, 4 or 6 of 8 columns
     case 1:
       hipLaunchKernelGGL((k_bcombine<4, 8, 2, false>), grid, dim3(256), 0, s, u, x0, y0, m0, x1, y1, m1, 1.f, k, k, n,
-                         nullptr);
+                         nullptr, BkFlush{});
       break;
     case 2:
       hipLaunchKernelGGL((k_bcombine<8, 8, 2, false>), grid, dim3(512), 0, s, u, x0, y0, m0, x1, y1, m1, 1.f, k, k, n,
-                         nullptr);
+                         nullptr, BkFlush{});
       break;
     default:
       hipLaunchKernelGGL((k_bcombine<12, 8, 2, false>), grid, dim3(768), 0, s, u, x0, y0, m0, x1, y1, m1, 1.f, k, k,
-                         n, nullptr);
+                         n, nullptr, BkFlush{});
+      break;
+  }
+  DFMI_HIP(hipGetLastError());
+}
+
+// D += U M (k columns, M row stride 48), and for every column j with flag[j] != 0 also x_j += scale[j] D_j (FP64,
+// column stride n) followed by D_j = 0, in the same pass: the reliable update of the block recurrence, which moves
+// the FP32 solution increment of the flushed columns into the FP64 solution.
+void block_combine_flush(const float* u, float* d, const float* m, int k, long long n, double* x, const double* scale,
+                         const int* flag, hipStream_t s) {
+  if (n < 1 || k < 1 || k > BK_MAXK) throw std::invalid_argument("block_combine_flush: need n >= 1 and 1 <= k <= 48");
+  if (!x || !scale || !flag) throw std::invalid_argument("block_combine_flush: need x, scale and flags");
+  if ((long long)n * (long long)sizeof(double) > (long long)INT32_MAX)
+    throw std::invalid_argument("block combination: FP64 columns above 2 GiB");
+  const dim3 grid(block_combine_blocks(n));
+  const BkFlush fl{x, scale, flag};
+  switch ((k + 15) / 16) {
+    case 1:
+      hipLaunchKernelGGL((k_bcombine<1, 16, 1, false, true>), grid, dim3(64), 0, s, u, d, d, m, d, d, m, 1.f, k, k, n,
+                         nullptr, fl);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_bcombine<2, 16, 1, false, true>), grid, dim3(128), 0, s, u, d, d, m, d, d, m, 1.f, k, k,
+                         n, nullptr, fl);
+      break;
+    default:
+      hipLaunchKernelGGL((k_bcombine<3, 16, 1, false, true>), grid, dim3(192), 0, s, u, d, d, m, d, d, m, 1.f, k, k,
+                         n, nullptr, fl);
       break;
   }
   DFMI_HIP(hipGetLastError());

@@ -245,7 +245,9 @@ void pcg_update(T* x, T* r, const T* p, const T* ap, const double* alpha, long l
                 double* partial_rr, int nblocks, hipStream_t s, const T* z = nullptr);
 template <typename T>
 void pcg_direction(T* p, const T* z, const double* beta, long long n, int k, const int* active, hipStream_t s);
-void add_scaled(double* x, const float* d, const double* scale, long long n, int k, const int* active, hipStream_t s);
+// x += scale d on the active columns; with zero_d the FP32 increment d of those columns is then set to zero
+void add_scaled(double* x, float* d, const double* scale, long long n, int k, const int* active, hipStream_t s,
+                bool zero_d = false);
 // Krylov coarse solve on interleaved level vectors [node][r][3]: dots of np (2 or 4) pairs a[i].b[i] per
 // column into partial[(i * k + r) * nblocks + block]; scalar steps on the rows of ks (9 k doubles); y -= f x;
 // x = a x + b p
@@ -264,6 +266,7 @@ void level_axpy(T* y, const T* x, const double* f, long long nodes, int k, const
 template <typename T>
 void level_combine(T* x, const T* p, const double* a, const double* b, long long nodes, int k, const int* active,
                    hipStream_t s);
+// v_c *= f[c] on the selected columns; a zero factor, or f == nullptr, stores zeros without reading v
 template <typename T>
 void scale_columns(T* v, long long n, int k, const double* f, const int* select, hipStream_t s);
 // dst[c] = src[c] (square_scale false) or dst[c] *= src[c]^2 (true) for selected columns
@@ -293,6 +296,9 @@ void block_residual_update(const float* w, float* r, const float* m, int k, long
 // Y0 = X0 + U M0 and Y1 = X1 + U M1 in one pass over U (k <= 48 columns in and out)
 void block_combine2(const float* u, const float* x0, float* y0, const float* m0, const float* x1, float* y1,
                     const float* m1, int k, long long n, hipStream_t s);
+// D += U M; columns j with flag[j] != 0 then x_j += scale[j] D_j (FP64) and D_j = 0, in the same pass
+void block_combine_flush(const float* u, float* d, const float* m, int k, long long n, double* x, const double* scale,
+                         const int* flag, hipStream_t s);
 void block_small_product(const double* S, const double* H, int k, double scale, float* out, hipStream_t s);
 // y[m] (+)= sum_r m[r][m] u[r] for k (<= 64) rows u of length n and an mcols-column device matrix m
 void combine_device(const double* u, long long n, int k, const double* m, int mcols, double* y, int accumulate,

@@ -1486,10 +1486,7 @@ static void solve_refine(Handle* h, int slot, const double* rhs_flat, double* x_
     h->history.push_back(worst);
     if (any_upd) {
       upload(act, upd);
-      add_scaled(x, d32, scale, n, k, act, s);
-      for (int c = 0; c < k; ++c) fh[c] = 0.0;
-      DFMI_HIP(hipMemcpyAsync(factor, fh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      scale_columns<float>(d32, n, k, factor, act, s);  // d = 0
+      add_scaled(x, d32, scale, n, k, act, s, true);  // x += s d, then d = 0
       fine_fp64(h, H, MODE_RESID, x, r, b, k, act, nullptr, nullptr);
       column_norms(h, r, n, k, act, tn);
       std::vector<int> cont(k, 0);
@@ -1680,13 +1677,11 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
   auto upload = [&](int* dst, const std::vector<int>& src) {
     DFMI_HIP(hipMemcpyAsync(dst, src.data(), sizeof(int) * k, hipMemcpyHostToDevice, s));
   };
-  // zero the columns whose flag in `which` is set (factor 0 on the selected columns)
+  // zero the columns whose flag in `which` is set (a write-only fill of the selected columns)
   auto zero_columns = [&](float* vec, const std::vector<int>& which) {
     if (!count(which)) return;
-    std::vector<double> zeros(k, 0.0);
-    DFMI_HIP(hipMemcpyAsync(factor, zeros.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
     upload(act, which);
-    scale_columns<float>(vec, n, k, factor, act, s);
+    scale_columns<float>(vec, n, k, nullptr, act, s);
     upload(act, active);
   };
   auto inactive_of = [&]() {
@@ -1767,13 +1762,10 @@ static void solve_block(Handle* h, int slot, const double* rhs_flat, double* x_f
         if (active[c] && !upd[c] && std::sqrt(rrh[c]) < std::sqrt(delta)) upd[c] = 1;
     bool pending = true;  // D += V a not yet applied
     if (any_upd) {  // flush D into x, true FP64 residual, restart the column's scaled residual at norm 1
-      block_combine(vcur, d32, d32, amat, k, k, n, s);
       pending = false;
       upload(act, upd);
-      add_scaled(x, d32, scale, n, k, act, s);
-      for (int c = 0; c < k; ++c) fh[c] = 0.0;
-      DFMI_HIP(hipMemcpyAsync(factor, fh.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      scale_columns<float>(d32, n, k, factor, act, s);
+      // D += V a over all columns; the flushed ones move into x (x += s D) and restart D at zero in the same pass
+      block_combine_flush(vcur, d32, amat, k, n, x, scale, act, s);
       // true FP64 residual of the flushed columns, written straight into R in FP32 scaled by the recursive estimate
       // of its norm, with the partial sums of |r|^2; the continuing columns are then brought to unit norm
       std::vector<double> shn(sh), inv(k, 1.0);

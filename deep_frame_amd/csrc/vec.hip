@@ -222,10 +222,13 @@ template <typename T>
 __global__ void k_scale_columns(T* __restrict__ v, long long n, const double* f, const int* select) {
   const int c = blockIdx.y;
   if (select && !select[c]) return;
-  const T w = (T)f[c];
+  const T w = f ? (T)f[c] : T(0);
   const long long off = (long long)c * n;
-  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    v[off + i] = w == T(0) ? T(0) : v[off + i] * w;
+  if (w == T(0)) {  // a zero factor stores zeros without reading the column
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) v[off + i] = T(0);
+    return;
+  }
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) v[off + i] *= w;
 }
 
 template <typename T>
@@ -250,16 +253,20 @@ void select_scalars(const double* src, double* dst, const int* select, int k, bo
 }
 
 // x += scale[c] * d for active columns (iterative-refinement correction, FP32 -> FP64)
-__global__ void k_add_scaled(double* __restrict__ x, const float* __restrict__ d, const double* scale, long long n,
-                             const int* active) {
+__global__ void k_add_scaled(double* __restrict__ x, float* __restrict__ d, const double* scale, long long n,
+                             const int* active, bool zero_d) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
   if (i >= n || (active && !active[c])) return;
-  x[(long long)c * n + i] += scale[c] * (double)d[(long long)c * n + i];
+  const long long e = (long long)c * n + i;
+  x[e] += scale[c] * (double)d[e];
+  if (zero_d) d[e] = 0.f;
 }
 
-void add_scaled(double* x, const float* d, const double* scale, long long n, int k, const int* active, hipStream_t s) {
-  hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, x, d, scale, n, active);
+void add_scaled(double* x, float* d, const double* scale, long long n, int k, const int* active, hipStream_t s,
+                bool zero_d) {
+  hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)((n + 255) / 256), k), dim3(256), 0, s, x, d, scale, n, active,
+                     zero_d);
   DFMI_HIP(hipGetLastError());
 }
 
