@@ -85,6 +85,8 @@ def parse():
                     help="mixed solver: one PCG recurrence per column")
     ap.add_argument("--chebyshev-kind", type=int, default=1, choices=[1, 4],
                     help="mixed solver: Chebyshev smoother polynomial, 1 the reference's first kind, 4 the fourth kind")
+    ap.add_argument("--chebyshev-ratio", type=float, default=20.0,
+                    help="mixed solver: the Chebyshev interval is [lambda_max / ratio, lambda_max] (the reference's 20)")
     ap.add_argument("--growth", type=float, default=1.0,
                     help="mixed solver: Chebyshev degree growth per level (degree * growth^level)")
     ap.add_argument("--fine-smoother", default="vector", choices=["matrix", "vector"],
@@ -249,7 +251,8 @@ def main():
         return
     mixed = {"precision": "float32", "refine": args.scheme == "refine", "fine_smoother": args.fine_smoother,
              "coarse_krylov": args.coarse_krylov, "krylov_levels": args.krylov_levels, "growth": args.growth,
-             "chebyshev_degree": args.degree, "block_krylov": args.block, "chebyshev_kind": args.chebyshev_kind}
+             "chebyshev_degree": args.degree, "block_krylov": args.block, "chebyshev_kind": args.chebyshev_kind,
+             "chebyshev_ratio": args.chebyshev_ratio}
     if args.inner_tolerance:
         mixed["inner_tolerance"] = args.inner_tolerance
     learned = None
@@ -285,7 +288,8 @@ def main():
         # FP64 with the mixed leg's algorithm (level-1 Krylov solve, degree growth, flexible FP64 PCG): the part
         # of the speed-up that precision alone buys
         same = {"precision": "float64", "coarse_krylov": args.coarse_krylov, "krylov_levels": args.krylov_levels,
-                "growth": args.growth, "chebyshev_degree": args.degree, "chebyshev_kind": args.chebyshev_kind}
+                "growth": args.growth, "chebyshev_degree": args.degree, "chebyshev_kind": args.chebyshev_kind,
+                "chebyshev_ratio": args.chebyshev_ratio}
         mgs, ts, reps, _ = run(same, min(args.steps, 3), 1, "fp64-mixed-algorithm")
         mgs.close()
         t_fp64_same, it64_same = float(np.mean(ts)), max(r[0] for r in reps)
@@ -318,6 +322,7 @@ def main():
                    "max_rel_res": max(r[1] for r in reports), "fine_smoother": args.fine_smoother,
                    "coarse_krylov": args.coarse_krylov, "krylov_levels": args.krylov_levels, "growth": args.growth,
                    "chebyshev_degree": args.degree, "chebyshev_kind": args.chebyshev_kind,
+                   "chebyshev_ratio": args.chebyshev_ratio,
                    "inner_tolerance": args.inner_tolerance,
                    "block_krylov": args.block, "fine_tile": list(tile[:2]) if tile else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
