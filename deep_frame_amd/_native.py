@@ -69,6 +69,8 @@ SIGNATURES = {
                            c_void_p], c_int32),
     "dfmi_block_combine2": ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int64,
                              c_void_p], c_int32),
+    "dfmi_block_combine_flush": ([c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p,
+                                  c_void_p], c_int32),
     "dfmi_gram_static": ([c_void_p, c_void_p, c_int64, c_int32, c_int32, c_void_p, c_void_p], c_int32),
     "dfmi_combine": ([c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_void_p], c_int32),
     "dfmi_project": ([c_void_p, c_void_p, c_int64, c_double, POINTER(c_double), c_int32, c_int32, c_void_p, c_void_p,

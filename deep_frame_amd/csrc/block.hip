@@ -528,6 +528,16 @@ int dfmi_block_combine2(const float* u_dev, const float* x0_dev, float* y0_dev, 
   })
 }
 
+int dfmi_block_combine_flush(const float* u_dev, float* d_dev, const float* m_dev, int32_t k, int64_t n,
+                             double* x_dev, const double* scale_dev, const int32_t* flag_dev, void* hip_stream) {
+  DFMI_BLOCK_GUARD({
+    if (!u_dev || !d_dev || !m_dev) throw std::invalid_argument("block_combine_flush: null operand");
+    if (u_dev == d_dev) throw std::invalid_argument("block_combine_flush: d must not alias u");
+    dfmi::block_combine_flush(u_dev, d_dev, m_dev, k, (long long)n, x_dev, scale_dev, flag_dev,
+                              (hipStream_t)hip_stream);
+  })
+}
+
 int dfmi_block_update(const float* v_dev, float* d_dev, const float* w_dev, float* r_dev, const float* m_dev,
                       int32_t k, int64_t n, double* rr_dev, double* work_dev, void* hip_stream) {
   DFMI_BLOCK_GUARD({

@@ -243,7 +243,10 @@ int dfmi_combine(const double* u_dev, int64_t n, int32_t k, const double* m_dev,
  *   dfmi_block_combine: Y = X + U M for kin input and kout output columns (y may alias x, not u);
  *   dfmi_block_update: D += V M, R -= W M (k columns) and rr_dev[j] = |R_j|^2 (FP64) after the update;
  *   dfmi_block_combine2: Y0 = X0 + U M0 and Y1 = X1 + U M1 (k columns each) in one pass over U (each y may alias
- *     its x, neither may alias u). */
+ *     its x, neither may alias u);
+ *   dfmi_block_combine_flush: D += U M (k columns), then for every column j with flag_dev[j] != 0 also
+ *     x_j += scale_dev[j] D_j into the FP64 multi-vector x (column j at x_dev + j * n) and D_j = 0, in the same
+ *     pass (the reliable update of the block recurrence); d must not alias u. */
 int dfmi_block_work_doubles(int64_t n, int32_t ka, int32_t kb, int64_t* count);
 int dfmi_block_gram(const float* a_dev, int32_t ka, const float* b1_dev, int32_t kb1, const float* b2_dev,
                     int32_t kb2, int64_t n, double* out_dev, double* work_dev, void* hip_stream);
@@ -254,6 +257,8 @@ int dfmi_block_update(const float* v_dev, float* d_dev, const float* w_dev, floa
 int dfmi_block_combine2(const float* u_dev, const float* x0_dev, float* y0_dev, const float* m0_dev,
                         const float* x1_dev, float* y1_dev, const float* m1_dev, int32_t k, int64_t n,
                         void* hip_stream);
+int dfmi_block_combine_flush(const float* u_dev, float* d_dev, const float* m_dev, int32_t k, int64_t n,
+                             double* x_dev, const double* scale_dev, const int32_t* flag_dev, void* hip_stream);
 
 /* SIMP interpolation E = E0 (eps + (1 - eps) rho^p), dE — topology_optimization.py:657-661;
  * inactive cells (active_dev[e] == 0) get 0. */

@@ -132,6 +132,52 @@ def test_block_combine2_matches_numpy(torch_gpu, k, n):
     assert np.array_equal(tu.cpu().numpy(), u) and np.array_equal(tx0.cpu().numpy(), x0)
 
 
+@pytest.mark.parametrize("k,n", [(42, 100003), (20, 4097), (7, 300), (1, 5)])
+def test_block_combine_flush_matches_numpy(torch_gpu, k, n):
+    """The reliable update of the block recurrence in one pass: D += U M on every column; the flagged columns then
+    move into the FP64 solution (x += s D) and restart at D = 0, the others keep x unchanged (bit for bit)."""
+    from deep_frame_amd import _native
+    torch = torch_gpu
+    rng = np.random.default_rng(200 + k)
+    u, d = (rng.normal(size=(k, n)).astype(np.float32) for _ in range(2))
+    m = (0.2 * rng.normal(size=(k, k))).astype(np.float32)
+    x = rng.normal(size=(k, n))
+    sc = rng.uniform(0.5, 2.0, size=k)
+    flag = (rng.uniform(size=k) < 0.5).astype(np.int32)
+    flag[0] = 1
+    if k > 1:
+        flag[-1] = 0
+    tu, td, tx = (torch.as_tensor(a, device="cuda") for a in (u, d, x))
+    ts, tf = torch.as_tensor(sc, device="cuda"), torch.as_tensor(flag, device="cuda")
+    tm = _coef(torch, m)
+    _native.call("dfmi_block_combine_flush", _native.ptr(tu), _native.ptr(td), _native.ptr(tm), k, n,
+                 _native.ptr(tx), _native.ptr(ts), _native.ptr(tf), _stream(torch))
+    u64 = u.astype(np.float64)
+    sums = d + m.astype(np.float64).T @ u64
+    scale = np.abs(d) + np.abs(m.astype(np.float64)).T @ np.abs(u64)
+    got_d, got_x = td.cpu().numpy(), tx.cpu().numpy()
+    on = flag.astype(bool)
+    assert np.all(got_d[on] == 0.0)
+    assert np.max(np.abs(got_d[~on] - sums[~on]) / scale[~on], initial=0.0) < 1e-6
+    assert np.array_equal(got_x[~on], x[~on])
+    want_x = x[on] + sc[on, None] * sums[on]
+    assert np.max(np.abs(got_x[on] - want_x) / (sc[on, None] * scale[on])) < 1e-6
+    assert np.array_equal(tu.cpu().numpy(), u)
+
+
+def test_block_combine_flush_rejects_aliased_input(torch_gpu):
+    from deep_frame_amd import _native
+    torch = torch_gpu
+    t = torch.zeros((4, 64), dtype=torch.float32, device="cuda")
+    x = torch.zeros((4, 64), dtype=torch.float64, device="cuda")
+    sc = torch.ones(4, dtype=torch.float64, device="cuda")
+    fl = torch.ones(4, dtype=torch.int32, device="cuda")
+    m = _coef(torch, np.zeros((4, 4), np.float32))
+    with pytest.raises(RuntimeError, match="alias"):
+        _native.call("dfmi_block_combine_flush", _native.ptr(t), _native.ptr(t), _native.ptr(m), 4, 64,
+                     _native.ptr(x), _native.ptr(sc), _native.ptr(fl), _stream(torch))
+
+
 def test_block_combine2_rejects_aliased_input(torch_gpu):
     from deep_frame_amd import _native
     torch = torch_gpu
