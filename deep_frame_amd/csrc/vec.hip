@@ -84,7 +84,7 @@ void reduce_max(const double* partial, int nblocks, int k, double* out, hipStrea
 }
 
 // first Chebyshev step from a zero iterate: x = c * Dinv * b
-// column-major: grid (blocks over 3 * nodes, columns); interleaved: one thread per stored value
+// column-major: grid (blocks over 3 * nodes, columns); interleaved: one thread per (node, column), its three values
 template <typename T>
 __global__ void k_cheb_start_cols(const T* __restrict__ b, const T* __restrict__ dinv, double c, T* __restrict__ x,
                                   long long n3, const int* active) {
@@ -98,13 +98,16 @@ __global__ void k_cheb_start_cols(const T* __restrict__ b, const T* __restrict__
 
 template <typename T>
 __global__ void k_cheb_start(const T* __restrict__ b, const T* __restrict__ dinv, double c, T* __restrict__ x,
-                             int nodes, int k, const int* active) {
-  // interleaved: o = n * 3k + 3r + comp
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nodes * 3 * k) return;
-  const int w = 3 * k, n = o / w, rem = o - n * w, r = rem / 3, comp = rem - r * 3;
+                             int slots, int k, const int* active) {
+  // interleaved: slot e = n * k + r holds the three values at 3e
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= slots) return;
+  const int n = e / k, r = e - n * k;
   if (active && !active[r]) return;
-  x[o] = (T)c * dinv[n * 3 + comp] * b[o];
+  const T cc = (T)c;
+  const long long o = 3LL * e;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) x[o + q] = cc * dinv[3 * n + q] * b[o + q];
 }
 
 template <typename T>
@@ -116,8 +119,8 @@ void cheb_start(const T* b, const T* dinv, double c, T* x, long long nodes, int 
     hipLaunchKernelGGL(k_cheb_start_cols<T>, dim3(bx, k), dim3(256), 0, s, b, dinv, c, x, n3, active);
   } else {
     if (nodes * 3 * (long long)k > (long long)INT32_MAX) throw std::runtime_error("cheb_start: level too large");
-    hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * 3 * k + 255) / 256)), dim3(256), 0, s, b, dinv, c,
-                       x, (int)nodes, k, active);
+    hipLaunchKernelGGL(k_cheb_start<T>, dim3((unsigned)((nodes * k + 255) / 256)), dim3(256), 0, s, b, dinv, c, x,
+                       (int)(nodes * k), k, active);
   }
   DFMI_HIP(hipGetLastError());
 }
