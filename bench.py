@@ -170,7 +170,7 @@ def cpu_baseline():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md"}))
+        print(json.dumps({"error": "this build runs on one GPU: --gpus above 1 is not supported (sharding across GPUs is designed, not built)"}))
         sys.exit(2)
     import numpy as np
     import torch
