@@ -518,6 +518,19 @@ struct Tiling {
   int TY, TZ, tiles_y, tiles_z, xchunk, chunks;
 };
 
+// compute units of the current device (the first query is kept: one device per process)
+static int device_cus() {
+  static int cus = 0;
+  if (cus < 1) {
+    int dev = 0, n = 0;
+    cus = (hipGetDevice(&dev) == hipSuccess &&
+           hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+              ? n
+              : 256;
+  }
+  return cus;
+}
+
 static Tiling choose_tiling(const Geom& g, int k, int nt) {
   // z tile spans the whole node column when it fits (contiguous rows, full cache lines)
   Tiling best{};
@@ -532,12 +545,17 @@ static Tiling choose_tiling(const Geom& g, int k, int nt) {
       best = Tiling{TY, TZ, ty, tz, 0, 0};
     }
   }
-  // enough workgroups to fill 256 CUs several times over; each x-chunk recomputes one element layer
+  // x-chunks: each chunk recomputes one element layer and fills the plane pipeline, and a launch runs in rounds of
+  // the workgroups the device holds at once (three per CU at the kernels' occupancy), so the chunk count with the
+  // fewest rounds x (layers per chunk + 2) wins (ties: fewer chunks)
   const long long tiles = (long long)best.tiles_y * best.tiles_z * (k > 0 ? k : 1);
-  int chunks = (int)((4096 + tiles - 1) / tiles);
-  chunks = std::max(1, std::min(chunks, std::max(1, g.nx / 32)));
-  best.xchunk = (g.nx + chunks - 1) / chunks;
-  best.chunks = (g.nx + best.xchunk - 1) / best.xchunk;
+  const long long slots = 3LL * device_cus();
+  long long best_t = -1;
+  for (int c = 1; c <= std::max(1, g.nx / 16); ++c) {
+    const int xc = (g.nx + c - 1) / c, cc = (g.nx + xc - 1) / xc;
+    const long long t = ((tiles * cc + slots - 1) / slots) * (xc + 2);
+    if (best_t < 0 || t < best_t) best_t = t, best.xchunk = xc, best.chunks = cc;
+  }
   return best;
 }
 
